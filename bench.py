@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Training-step benchmark: train steps/s + sorted keys/s at 1M splats, 1080p,
+batched adjoint vs plain, one GPU.
+
+Workload (default): 1,000,000 seeded Gaussians (means U[-1,1]^3, log-scales
+N(-4.5,0.3), SH degree 3), 16 cameras on a ring of radius 3 at 1920x1080,
+targets rendered from a copy with means jittered by N(0,0.02). One step =
+render + L2/L1 loss + adjoint backward + geometry backward for all 16 views,
+gradients summed, one Adam step over all 59 parameters of every Gaussian.
+
+value      = steps/s of the batched adjoint path (all views in one pipeline)
+plain      = the same step with the views run one after another
+keys/s     = intersections sorted per step / time of the sort kernels
+roofline   = the dominant kernel's algorithmic bytes per launch / its mean
+             launch time (hipEvents on the library stream), vs 8 TB/s
+cpu_baseline = the CPU oracle on one of the 16 views, scaled to a step
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "skyfall-viewer_amd"))
+
+import numpy as np  # noqa: E402
+
+import sv  # noqa: E402
+
+METRIC = "train steps/s + sorted keys/s at 1M splats 1080p, batched adjoint vs plain, 1 GPU"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def configure(ctx: sv.Context, args, keep_grads=0):
+    c = ctx.config()
+    c.sh_deg = 3
+    c.w_l2, c.w_l1, c.w_ssim = 0.8, 0.2, 0.0
+    c.lr_pos = c.lr_sh = c.lr_opacity = c.lr_scale = c.lr_rot = 1e-3
+    c.beta1, c.beta2, c.eps = 0.9, 0.999, 1e-15
+    c.sh_update_interval, c.pos_lr_schedule = 1, 0
+    c.batch = args.batch
+    c.persistent = args.persistent
+    c.keep_grads = keep_grads
+    ctx.set_config(c)
+
+
+def kernel_bytes(phase: str, N: int, V: int, I: int, P: int, key_bytes: int, passes: int) -> float:
+    """Algorithmic HBM bytes of one launch of a phase (per-unit figures x units)."""
+    if phase == "forward":  # per intersection: id 4 + record 36; per pixel: rgb 12 + A 4 + last 4
+        return I * 40.0 + P * 20.0
+    if phase == "backward":  # per isect: id 4 + record 36 + rect/offset 12 + adjoint record 36; per pixel 32
+        return I * 88.0 + P * 32.0
+    if phase == "vjp_adam":  # per Gaussian: 59 x (p, m, v) read + write + stats; per (G, view) 24; per isect 36
+        return N * 1480.0 + N * V * 24.0 + I * 36.0
+    if phase == "sort":
+        return passes * I * (key_bytes + 2 * (key_bytes + 4.0))
+    return 0.0
+
+
+def step_b_alg(N: int, V: int, I: int, P: int, d: int) -> float:
+    """Algorithmic bytes of one step: 1488 N + 92 N V + I (136 + 24 d) + 40 P."""
+    return 1488.0 * N + 92.0 * N * V + I * (136.0 + 24.0 * d) + 40.0 * P
+
+
+def run_mode(mode, g, cams, targets, args):
+    ctx = sv.Context(capacity=g["pos"].shape[0])
+    configure(ctx, args)
+    ctx.set_gaussians(g)
+    ctx.set_views(cams)
+    for v, t in enumerate(targets):
+        ctx.set_target(v, t)
+    views = list(range(len(cams)))
+    for i in range(args.warmup):
+        ctx.step_views(views, mode)
+    ctx.sync()
+    ctx.reset_timings()
+    ctx.enable_timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.step_views(views, mode)
+    ctx.sync()
+    t1 = time.perf_counter()
+    timings = ctx.timings()
+    ctx.enable_timing(False)
+    st = ctx.step_views(views, mode, stats=True)
+    ctx.close()
+    return (t1 - t0) / args.steps, timings, st
+
+
+def cpu_baseline(g, cams, targets, args):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    orc.lib()
+    w, h = int(cams[0].viewport[0]), int(cams[0].viewport[1])
+    view = orc.view_params(w, h, sh_deg=3)
+    t0 = time.perf_counter()
+    r = orc.render_and_grad(g, cams[0], view, target=targets[0], w_l2=0.8, w_l1=0.2)
+    t_view = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bc1, bc2 = orc.adam_bias(1, 0.9, 0.999)
+    for k in ("pos", "logscale", "quat", "logit", "sh"):
+        p = np.array(g[k], np.float32, copy=True)
+        orc.adam(p, np.zeros_like(p), np.zeros_like(p), r["grads"][k], 1e-3, 0.9, 0.999, 1e-15, bc1, bc2)
+    t_adam = time.perf_counter() - t0
+    V = len(cams)
+    return {
+        "value": 1.0 / (V * t_view + t_adam),
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"CPU oracle (C, OpenMP) render+loss+backward+geometry backward of 1 of the {V} views "
+                  f"({g['pos'].shape[0]} splats, {w}x{h}, {r['n_isect']} intersections, {t_view:.2f} s) "
+                  f"+ Adam over all parameters ({t_adam:.2f} s), scaled to {V} views",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--views", type=int, default=16)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+              "(multi-GPU sharding by views is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    N, V, W, H = args.n, args.views, args.width, args.height
+    log(f"scene: {N} splats, {V} views {W}x{H}")
+    g = sv.scene_gaussians(0, N, 1.0)
+    cams = sv.ring_cameras(V, 3.0, W, H)
+    tg = dict(g)
+    tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
+    targets = []
+    with sv.Context(capacity=N) as tc:
+        configure(tc, args)
+        tc.set_gaussians(tg)
+        tc.set_views(cams)
+        for v in range(V):
+            tc.forward(v)
+            targets.append(tc.image(v))
+    log("targets rendered")
+
+    sec_b, tim_b, st_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args)
+    log(f"batched: {1e3 * sec_b:.2f} ms/step, I={st_b.n_isect}, key bits {st_b.key_bits}, passes {st_b.sort_passes}")
+    plain = None
+    if not args.no_plain:
+        sec_p, tim_p, st_p = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
+        plain = {"ms_per_step": 1e3 * sec_p, "steps_per_s": 1.0 / sec_p,
+                 "phase_ms_per_step": {k: v[0] / args.steps for k, v in tim_p.items() if v[1]}}
+        log(f"plain: {1e3 * sec_p:.2f} ms/step")
+
+    I = int(st_b.n_isect)
+    P = V * W * H
+    key_bytes = 8 if st_b.key_bits > 32 else 4
+    passes = int(st_b.sort_passes)
+    phase = {k: (v[0] / max(v[1], 1), v[1] / args.steps) for k, v in tim_b.items() if v[1]}
+    sort_ms_step = tim_b.get("sort", (0.0, 0))[0] / args.steps
+    keys_per_s = I / (sort_ms_step * 1e-3) if sort_ms_step > 0 else None
+    # dominant single-kernel phase
+    cands = {k: phase[k][0] for k in ("forward", "backward", "vjp_adam") if k in phase}
+    dom = max(cands, key=cands.get)
+    dom_ms = cands[dom]
+    dom_bytes = kernel_bytes(dom, N, V, I, P, key_bytes, passes)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    b_alg = step_b_alg(N, V, I, P, passes)
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / sec_b,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec_b,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
+        "config": {"workload": f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step",
+                   "splats": N, "views": V, "width": W, "height": H, "tile": 16, "lds_batch": args.batch,
+                   "persistent": args.persistent},
+        "keys_per_s": keys_per_s,
+        "intersections_per_step": I,
+        "sort_key_bits": int(st_b.key_bits),
+        "sort_passes": passes,
+        "plain": plain,
+        "speedup_vs_plain": (plain["ms_per_step"] / (1e3 * sec_b)) if plain else None,
+        "phase_ms_per_launch": {k: v[0] for k, v in phase.items()},
+        "step_b_alg_bytes": b_alg,
+        "step_b_alg_GBps": b_alg / sec_b / 1e9,
+        "step_b_alg_frac": b_alg / sec_b / 1e9 / HBM_PEAK_GBPS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle, one view)...")
+        out["cpu_baseline"] = cpu_baseline(g, cams, targets, args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,264 @@
+/*
+ * sv_api.h -- C ABI of the MI355X splat training step (libsv.so).
+ *
+ * Drop-in boundary for Skyfall-Viewer's training path. The reference runs
+ * the step as WGSL passes recorded by the GaussianRenderer host
+ * (src/renderers/gaussian-renderer.ts); each entry point below replaces one
+ * host routine of that file, cited per function. A Node N-API addon
+ * (skyfall-viewer_amd/node/) binds these symbols for the TypeScript host,
+ * and Python binds them with ctypes.
+ *
+ * Conventions
+ *  - Every function returns an sv_status (0 = SV_OK) unless stated.
+ *  - Pointers are caller-owned HOST memory, copied in or out before the call
+ *    returns. No device pointer crosses the ABI.
+ *  - One sv_ctx owns one HIP stream on one device. A context is not
+ *    thread-safe; calls block until the stream is idle.
+ *  - Gaussian parameters use the reference's f32 training layout
+ *    (gaussian-renderer.ts:130-137): pos[3N], log-scale[3N], quaternion
+ *    (x,y,z,w)[4N], opacity logit[N], SH[48N] (coefficient k, channel c at
+ *    index 3k+c). Images are row-major H x W x 3 float.
+ */
+#ifndef SV_API_H
+#define SV_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sv_status {
+  SV_OK = 0,
+  SV_E_ARG = 1,      /* bad argument (null pointer, index out of range) */
+  SV_E_CAPACITY = 2, /* more Gaussians/intersections than allocated */
+  SV_E_HIP = 3,      /* HIP runtime error; see sv_last_error */
+  SV_E_STATE = 4,    /* call out of order (no views, no target, ...) */
+  SV_E_NOMEM = 5     /* device allocation failed */
+} sv_status;
+
+typedef struct sv_ctx sv_ctx;
+
+/* CameraUniforms, src/shaders/common.wgsl:21-28 and
+ * src/camera/camera-uniform.ts:8 (272 bytes, matrices column-major).
+ * viewport = (W, H) in pixels, focal = (fx, fy) in pixels. */
+typedef struct sv_camera {
+  float view[16];
+  float view_inv[16];
+  float proj[16];
+  float proj_inv[16];
+  float viewport[2];
+  float focal[2];
+} sv_camera;
+
+/* Training configuration. Defaults (sv_default_config) are the reference's:
+ * tile 16 (gaussian-renderer.ts:366), projection params 0.2/1e4/512/0.3
+ * (:474-488), background black (:490-504), loss weights (0, 0.8, 0.2)
+ * (:506-522), learning rates (:88-96), Adam 0.9/0.999/1e-8 (:2421-2423),
+ * SH update interval 16 (:84), importance decay 0.98 (importance.ts:8-13). */
+typedef struct sv_config {
+  uint32_t tile_size;  /* binning tile in pixels: 16 */
+  uint32_t batch;      /* splats staged per LDS batch: 64, 128 or 256 */
+  uint32_t raster_px;  /* pixels per raster workgroup side: 8 or 16 */
+  uint32_t sh_deg;     /* RenderSettings.sh_deg, 0..3 */
+  float bg[3];
+  float near_plane, far_plane, radius_clip, eps2d;
+  float w_l2, w_l1, w_ssim;
+  float lr_pos, lr_sh, lr_opacity, lr_scale, lr_rot;
+  float beta1, beta2, eps;
+  uint32_t sh_update_interval;
+  uint32_t pos_lr_schedule; /* 1: computePositionLearningRate (:284-298) */
+  uint32_t render_f16;      /* 1: render from f16-rounded params (A6) */
+  float importance_decay;
+  uint32_t persistent;      /* 1: raster kernels pull (view, tile) work items
+                               from a device queue in one persistent launch */
+  uint32_t use_graph;       /* 1: capture the batched step in a hipGraph */
+  uint32_t keep_grads;      /* 1: batched steps also write the gradient
+                               buffers (sv_read_grads) before Adam */
+} sv_config;
+
+/* Step modes for sv_step. */
+#define SV_MODE_PLAIN 0   /* one view at a time (the reference's structure) */
+#define SV_MODE_BATCHED 1 /* all views in one pipeline, (view,tile,depth) keys */
+
+/* Wavefront fixed-point backward flags (backward-tiles-local.wgsl). */
+#define SV_WF_REFERENCE_ORDER 0x1u /* 64-entry batches front-to-back (:262) */
+#define SV_WF_SKIP_098 0x2u        /* skip alpha > 0.98 (:329-331) */
+
+typedef struct sv_step_stats {
+  uint64_t n_isect;     /* intersections sorted this step (all views) */
+  uint32_t n_views;
+  uint32_t sort_passes; /* 8-bit digit passes of the last sort */
+  uint32_t key_bits;    /* significant key bits of the last sort */
+  uint32_t pad;
+  double l1_sum;        /* sum |pred - target| over all pixels and channels */
+} sv_step_stats;
+
+/* Per-kernel timing record (sv_timings). */
+typedef struct sv_kernel_time {
+  char name[32];
+  double total_ms;
+  uint64_t launches;
+} sv_kernel_time;
+
+/* --- library / context ------------------------------------------------ */
+/* Version string. */
+const char *sv_version(void);
+/* Fill *out with the reference defaults listed above. */
+sv_status sv_default_config(sv_config *out);
+/* get_renderer(...) buffer creation (gaussian-renderer.ts:154-161,612-768):
+ * create a context on HIP device `device` with room for `capacity`
+ * Gaussians. */
+sv_status sv_create(int device, size_t capacity, sv_ctx **out);
+sv_status sv_destroy(sv_ctx *ctx);
+/* Message of the last failing call on ctx (or of sv_create when ctx is
+ * NULL). */
+const char *sv_last_error(const sv_ctx *ctx);
+/* setTrainingParams / setLossWeights (gaussian-renderer.ts:3922-3930,
+ * 4058-4074) plus the uniform buffers of :464-522. */
+sv_status sv_set_config(sv_ctx *ctx, const sv_config *cfg);
+sv_status sv_get_config(const sv_ctx *ctx, sv_config *cfg);
+
+/* --- parameters ------------------------------------------------------- */
+/* reloadPointCloud + training_init_params (gaussian-renderer.ts:2092-2126,
+ * training-params.wgsl:35-79): upload n Gaussians; clears moments, grads,
+ * statistics and the step counter. Any pointer may not be NULL. */
+sv_status sv_set_gaussians(sv_ctx *ctx, size_t n, const float *pos,
+                           const float *logscale, const float *quat,
+                           const float *logit, const float *sh);
+/* readBufferToCPU of the train_* buffers (gaussian-renderer.ts:3129-3205).
+ * NULL pointers are skipped. */
+sv_status sv_get_gaussians(sv_ctx *ctx, float *pos, float *logscale,
+                           float *quat, float *logit, float *sh);
+/* getGaussianCount (gaussian-renderer.ts:4110-4153). */
+size_t sv_get_count(const sv_ctx *ctx);
+/* readAdamMoments / writeAdamMoments (gaussian-renderer.ts:3129-3205);
+ * which = 1 (first moment) or 2 (second moment). */
+sv_status sv_get_moments(sv_ctx *ctx, int which, float *pos, float *logscale,
+                         float *quat, float *logit, float *sh);
+sv_status sv_set_moments(sv_ctx *ctx, int which, const float *pos,
+                         const float *logscale, const float *quat,
+                         const float *logit, const float *sh);
+/* Adam step counter (adam_step_counter, gaussian-renderer.ts:2544). */
+sv_status sv_get_step(const sv_ctx *ctx, uint32_t *t);
+sv_status sv_set_step(sv_ctx *ctx, uint32_t t);
+
+/* --- views and targets ------------------------------------------------ */
+/* setTrainingCameras (gaussian-renderer.ts:4084-4094): nv cameras; the
+ * render size of view v is round(viewport). */
+sv_status sv_set_views(sv_ctx *ctx, int nv, const sv_camera *cams);
+/* setTrainingImages (gaussian-renderer.ts:4096-4105): target of view v as
+ * H x W x 3 float (the reference samples rgba8 textures at texel centres,
+ * training-loss.wgsl:94-102). */
+sv_status sv_set_target(sv_ctx *ctx, int view, const float *rgb);
+
+/* --- Function-style pieces of the plain path ------------------------- */
+/* buildTileIntersections (gaussian-renderer.ts:1780-2023): project, count,
+ * device scan, emit, radix sort, tile ranges for view v. */
+sv_status sv_build_tiles(sv_ctx *ctx, int view, uint64_t *n_isect);
+/* run_training_tiled_forward (gaussian-renderer.ts:2261-2290). */
+sv_status sv_forward(sv_ctx *ctx, int view);
+/* run_training_loss (gaussian-renderer.ts:2292-2310): residual of view v. */
+sv_status sv_loss(sv_ctx *ctx, int view);
+/* Supply dL/d(image) of view v directly (H x W x 3), replacing sv_loss. */
+sv_status sv_set_residual(sv_ctx *ctx, int view, const float *rgb);
+/* run_training_tiled_backward + run_training_backward_geom
+ * (gaussian-renderer.ts:2312-2407), order-correct adjoint: reverse
+ * traversal from A_final and last index, then the per-Gaussian SH chain
+ * and geometry VJP. Adds into the gradient buffers. */
+sv_status sv_backward(sv_ctx *ctx, int view);
+/* run_wavefront_backward (gaussian-renderer.ts:2327-2385): the reference's
+ * i32 fixed-point per-intersection sums (backward-tiles-local.wgsl) and
+ * the per-Gaussian reduce (backward-reduce.wgsl) + geometry backward.
+ * flags: SV_WF_*. Adds into the gradient buffers. */
+sv_status sv_backward_fixed(sv_ctx *ctx, int view, uint32_t flags);
+/* zeroGradBuffers (gaussian-renderer.ts:3596-3623). */
+sv_status sv_zero_grad(sv_ctx *ctx);
+/* run_adam_updates (gaussian-renderer.ts:2455-2545) with per-tensor
+ * learning rates; increments the step counter first (t = step index). */
+sv_status sv_adam(sv_ctx *ctx);
+
+/* --- whole step -------------------------------------------------------- */
+/* trainStep (gaussian-renderer.ts:3638-3660) over nv views: zero grads,
+ * render + loss + backward for every view (gradients summed), one Adam
+ * step. mode SV_MODE_PLAIN runs the views one after another;
+ * SV_MODE_BATCHED runs them through one pipeline. stats may be NULL. */
+sv_status sv_step(sv_ctx *ctx, const int *views, int nv, int mode,
+                  sv_step_stats *stats);
+/* Forward only for nv views in one batched pipeline. */
+sv_status sv_render_batched(sv_ctx *ctx, const int *views, int nv);
+
+/* --- readbacks --------------------------------------------------------- */
+sv_status sv_read_image(sv_ctx *ctx, int view, float *rgb);
+sv_status sv_read_alpha(sv_ctx *ctx, int view, float *alpha);
+/* last_ids (training-forward-tiles.wgsl:214); 0xffffffff in empty tiles. */
+sv_status sv_read_last_ids(sv_ctx *ctx, int view, uint32_t *ids);
+sv_status sv_read_residual(sv_ctx *ctx, int view, float *rgb);
+/* Sorted (isect_ids, flatten_ids) of the last sv_build_tiles of view v,
+ * with keys in the reference encoding (training-tiles.wgsl:182-199). */
+sv_status sv_read_keys(sv_ctx *ctx, int view, uint32_t *keys, uint32_t *vals);
+/* tile_offsets after the host fix-up (gaussian-renderer.ts:2014-2020):
+ * tiles+1 entries. */
+sv_status sv_read_tile_offsets(sv_ctx *ctx, int view, uint32_t *offsets);
+/* GaussProjection records of view v (common.wgsl:57-65): 12 floats per
+ * Gaussian (mean2d.xy, radius, depth, conic.abc, tiles_count as f32 bits,
+ * position_world.xyz, opacity_logit). */
+sv_status sv_read_projections(sv_ctx *ctx, int view, float *rec);
+/* Gradient buffers; geom is grad_geom (5N: conic a,b,c, mean x,y) of the
+ * last backward. NULL pointers are skipped. */
+sv_status sv_read_grads(sv_ctx *ctx, float *pos, float *logscale, float *quat,
+                        float *logit, float *sh, float *geom);
+/* i32 fixed-point tile_grads of the last sv_backward_fixed (54 per
+ * intersection, backward-reduce.wgsl:2-5). */
+sv_status sv_read_tile_grads_fixed(sv_ctx *ctx, int view, int32_t *out);
+/* ImportanceStats (importance-accum.wgsl:1-6) and grad_accum
+ * (densify-accum.wgsl:19-46), 4 floats per Gaussian each. */
+sv_status sv_read_stats(sv_ctx *ctx, float *importance4, float *grad_accum4);
+
+/* --- prune / densify ------------------------------------------------- */
+/* applyPruneMask (gaussian-renderer.ts:2917-3000): remove[i] != 0 drops
+ * Gaussian i; params and moments are compacted in order on the device. */
+sv_status sv_prune_mask(sv_ctx *ctx, const uint8_t *remove, size_t *n_removed);
+/* Prune rule: drop Gaussians with sigmoid(logit) < min_opacity
+ * (prune-decide.wgsl:178-180), decided and compacted on the device. */
+sv_status sv_prune_opacity(sv_ctx *ctx, float min_opacity, size_t *n_removed);
+/* performDensification (gaussian-renderer.ts:3211-3466) with a seeded RNG:
+ * actions[i] = 0 none, 1 clone, 2 split. Output order: non-split originals
+ * in order, then children in parent order (clone: 1, split: 2); children
+ * get zero moments. */
+sv_status sv_densify(sv_ctx *ctx, const uint8_t *actions, uint64_t seed,
+                     size_t *n_added);
+
+/* --- timing --------------------------------------------------------- */
+/* Enable per-kernel hipEvent timing on the context stream. */
+sv_status sv_enable_timing(sv_ctx *ctx, int enable);
+/* Copy up to *count records; *count receives the number available. */
+sv_status sv_timings(sv_ctx *ctx, sv_kernel_time *out, int *count);
+sv_status sv_reset_timings(sv_ctx *ctx);
+/* Block until the context stream is idle. */
+sv_status sv_sync(sv_ctx *ctx);
+
+/* --- seeded synthetic scenes (harness; host only, no device) ---------- */
+/* n Gaussians: means U[-h,h]^3, log-scales N(-4.5,0.3), quaternion
+ * normalize(N(0,1)^4) stored (x,y,z,w), logit N(0,1), SH DC N(0.5,0.2),
+ * other coefficients N(0,0.05); every value rounded to f16 (RNE). */
+sv_status sv_scene_gaussians(uint64_t seed, size_t n, float half_extent,
+                             float *pos, float *logscale, float *quat,
+                             float *logit, float *sh);
+/* pos += N(0, sigma) per component (then f16-rounded). */
+sv_status sv_scene_jitter(uint64_t seed, size_t n, float sigma, float *pos);
+/* Pinhole camera at eye looking at target, world up +y (+x if parallel),
+ * camera +z forward, fx = fy = (H/2)/tan(fovy/2), principal point at the
+ * image centre; proj = get_pinhole_projection_matrix(0.01, 4e6, ...)
+ * (training-camera.ts:29-52). */
+sv_status sv_scene_camera(const float eye[3], const float target[3], uint32_t w,
+                          uint32_t h, float fovy_deg, sv_camera *out);
+/* Seeded uniform and normal draws (SplitMix64 + Box-Muller). */
+double sv_scene_uniform(uint64_t seed, uint64_t counter);
+double sv_scene_normal(uint64_t seed, uint64_t counter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

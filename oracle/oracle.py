@@ -1,0 +1,258 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+cpu_baseline leg of bench.py, where it is the checker. The product library
+(skyfall-viewer_amd/libsv.so) never uses it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrcCamera(C.Structure):
+    _fields_ = [("view", C.c_float * 16), ("view_inv", C.c_float * 16), ("proj", C.c_float * 16),
+                ("proj_inv", C.c_float * 16), ("viewport", C.c_float * 2), ("focal", C.c_float * 2)]
+
+
+class OrcView(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tile_size", C.c_uint32),
+                ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("sh_deg", C.c_uint32),
+                ("near_plane", C.c_float), ("far_plane", C.c_float), ("radius_clip", C.c_float),
+                ("eps2d", C.c_float), ("bg", C.c_float * 3)]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_float)) for k in ("pos", "logscale", "quat", "logit", "sh")]
+
+
+class OrcGrads(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_float)) for k in ("pos", "logscale", "quat", "logit", "sh", "geom")]
+
+
+class OrcStepOut(C.Structure):
+    _fields_ = [("proj", C.c_void_p), ("keys", C.POINTER(C.c_uint32)), ("vals", C.POINTER(C.c_uint32)),
+                ("cap_isect", C.c_size_t), ("offsets", C.POINTER(C.c_uint32)),
+                ("color", C.POINTER(C.c_float)), ("alpha", C.POINTER(C.c_float)),
+                ("last_ids", C.POINTER(C.c_uint32)), ("residual", C.POINTER(C.c_float)),
+                ("tile_grads_fixed", C.POINTER(C.c_int32))]
+
+
+PROJ_DTYPE = np.dtype([("mean2d", "<f4", 2), ("radius", "<f4"), ("depth", "<f4"), ("conic", "<f4", 3),
+                       ("tiles_count", "<u4"), ("position_world", "<f4", 3), ("opacity_logit", "<f4")])
+assert PROJ_DTYPE.itemsize == 48
+
+WF_REFERENCE_ORDER, WF_SKIP_098 = 0x1, 0x2
+NO_LAST = 0xFFFFFFFF
+
+_lib = None
+_fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+_u32p = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))  # noqa: E731
+_i32p = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+_dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_expf.restype = C.c_float
+        L.orc_expf.argtypes = [C.c_float]
+        L.orc_f16_round.restype = C.c_float
+        L.orc_f16_round.argtypes = [C.c_float]
+        L.orc_encode_key.restype = C.c_uint32
+        L.orc_encode_key.argtypes = [C.c_uint32, C.c_uint32, C.c_float]
+        L.orc_count_isects.restype = C.c_uint64
+        L.orc_render_and_grad.restype = C.c_int64
+        L.orc_forward_loss_f64.restype = C.c_double
+        L.orc_compact_rows.restype = C.c_size_t
+        _lib = L
+    return _lib
+
+
+def view_params(w, h, sh_deg=3, tile=16, bg=(0.0, 0.0, 0.0), near=0.2, far=1e4, radius_clip=512.0,
+                eps2d=0.3) -> OrcView:
+    v = OrcView()
+    v.width, v.height, v.tile_size = w, h, tile
+    v.tiles_x, v.tiles_y = (w + tile - 1) // tile, (h + tile - 1) // tile
+    v.sh_deg = sh_deg
+    v.near_plane, v.far_plane, v.radius_clip, v.eps2d = near, far, radius_clip, eps2d
+    for i in range(3):
+        v.bg[i] = bg[i]
+    return v
+
+
+def camera(cam_struct) -> OrcCamera:
+    """Copy any ctypes camera with the CameraUniforms fields."""
+    o = OrcCamera()
+    C.memmove(C.byref(o), C.byref(cam_struct), C.sizeof(OrcCamera))
+    return o
+
+
+class Params:
+    """Keeps numpy arrays alive behind an OrcParams struct."""
+
+    def __init__(self, g: dict):
+        self.a = {k: np.ascontiguousarray(g[k], dtype=np.float32) for k in ("pos", "logscale", "quat", "logit", "sh")}
+        self.n = self.a["pos"].shape[0]
+        self.s = OrcParams(*[_fp(self.a[k]) for k in ("pos", "logscale", "quat", "logit", "sh")])
+
+
+def zero_grads(n: int) -> dict:
+    return {"pos": np.zeros((n, 3), np.float32), "logscale": np.zeros((n, 3), np.float32),
+            "quat": np.zeros((n, 4), np.float32), "logit": np.zeros((n,), np.float32),
+            "sh": np.zeros((n, 48), np.float32), "geom": np.zeros((n, 5), np.float32)}
+
+
+def grads_struct(g: dict) -> OrcGrads:
+    return OrcGrads(*[_fp(g[k]) for k in ("pos", "logscale", "quat", "logit", "sh", "geom")])
+
+
+def expf(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_expf_v(C.c_size_t(x.size), _fp(x), _fp(y))
+    return y
+
+
+def f16_round(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_f16_round_v(C.c_size_t(x.size), _fp(x), _fp(y))
+    return y
+
+
+def sh_color(dirs: np.ndarray, sh: np.ndarray, deg: int) -> np.ndarray:
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    s = np.ascontiguousarray(sh, np.float32).reshape(-1, 48)
+    o = np.empty((d.shape[0], 3), np.float32)
+    lib().orc_sh_color_v(C.c_size_t(d.shape[0]), _fp(d), _fp(s), C.c_uint32(deg), _fp(o))
+    return o
+
+
+def project(g: dict, cam, view: OrcView) -> np.ndarray:
+    p = Params(g)
+    out = np.zeros(p.n, PROJ_DTYPE)
+    oc = camera(cam)
+    lib().orc_project(C.c_size_t(p.n), C.byref(p.s), C.byref(oc), C.byref(view), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def emit(proj: np.ndarray, view: OrcView):
+    m = int(lib().orc_count_isects(C.c_size_t(proj.shape[0]), proj.ctypes.data_as(C.c_void_p)))
+    keys = np.zeros(max(m, 1), np.uint32)
+    vals = np.zeros(max(m, 1), np.uint32)
+    lib().orc_emit(C.c_size_t(proj.shape[0]), proj.ctypes.data_as(C.c_void_p), C.byref(view), _u32p(keys), _u32p(vals))
+    return keys[:m].copy(), vals[:m].copy()
+
+
+def sort_pairs(keys: np.ndarray, vals: np.ndarray):
+    k = np.array(keys, np.uint32, copy=True)
+    v = np.array(vals, np.uint32, copy=True)
+    lib().orc_sort_pairs(C.c_size_t(k.size), _u32p(k), _u32p(v))
+    return k, v
+
+
+def tile_offsets(keys: np.ndarray, n_tiles: int) -> np.ndarray:
+    k = np.ascontiguousarray(keys, np.uint32)
+    o = np.zeros(n_tiles + 1, np.uint32)
+    lib().orc_tile_offsets(C.c_size_t(k.size), _u32p(k), C.c_uint32(n_tiles), _u32p(o))
+    return o
+
+
+def loss(color, target, w_l2, w_l1, w_ssim=0.0):
+    c = np.ascontiguousarray(color, np.float32)
+    t = np.ascontiguousarray(target, np.float32)
+    h, w = c.shape[:2]
+    r = np.empty_like(c)
+    lib().orc_loss(_fp(c), _fp(t), C.c_uint32(w), C.c_uint32(h), C.c_float(w_l2), C.c_float(w_l1),
+                   C.c_float(w_ssim), _fp(r))
+    return r
+
+
+def adam_bias(t: int, beta1: float, beta2: float):
+    a, b = C.c_float(), C.c_float()
+    lib().orc_adam_bias(C.c_uint32(t), C.c_float(beta1), C.c_float(beta2), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def adam(param, m1, m2, grad, lr, beta1, beta2, eps, bc1, bc2s):
+    """In place on float32 arrays."""
+    lib().orc_adam(C.c_size_t(param.size), _fp(param), _fp(m1), _fp(m2), _fp(np.ascontiguousarray(grad, np.float32)),
+                   C.c_float(lr), C.c_float(beta1), C.c_float(beta2), C.c_float(eps), C.c_float(bc1),
+                   C.c_float(bc2s))
+
+
+def render_and_grad(g: dict, cam, view: OrcView, target=None, w_l2=0.8, w_l1=0.2, w_ssim=0.0,
+                    wavefront=False, wf_flags=0, cap_isect=None) -> dict:
+    """Project, sort, forward; with a target also loss, backward and the
+    geometry backward. Returns every intermediate as numpy arrays."""
+    p = Params(g)
+    n = p.n
+    proj = project(g, cam, view)
+    m = int(lib().orc_count_isects(C.c_size_t(n), proj.ctypes.data_as(C.c_void_p)))
+    cap = max(m, 1) if cap_isect is None else cap_isect
+    W, H = view.width, view.height
+    P = W * H
+    nt = view.tiles_x * view.tiles_y
+    out = {
+        "proj": np.zeros(n, PROJ_DTYPE), "keys": np.zeros(cap, np.uint32), "vals": np.zeros(cap, np.uint32),
+        "offsets": np.zeros(nt + 1, np.uint32), "color": np.zeros((H, W, 3), np.float32),
+        "alpha": np.zeros((H, W), np.float32), "last_ids": np.zeros((H, W), np.uint32),
+        "residual": np.zeros((H, W, 3), np.float32),
+    }
+    fixed = np.zeros((cap, 54), np.int32) if wavefront else None
+    so = OrcStepOut(out["proj"].ctypes.data_as(C.c_void_p), _u32p(out["keys"]), _u32p(out["vals"]), cap,
+                    _u32p(out["offsets"]), _fp(out["color"]), _fp(out["alpha"]), _u32p(out["last_ids"]),
+                    _fp(out["residual"]), _i32p(fixed) if fixed is not None else None)
+    grads = zero_grads(n)
+    gs = grads_struct(grads)
+    oc = camera(cam)
+    tgt = None if target is None else np.ascontiguousarray(target, np.float32)
+    r = lib().orc_render_and_grad(C.c_size_t(n), C.byref(p.s), C.byref(oc), C.byref(view),
+                                  _fp(tgt) if tgt is not None else None, C.c_float(w_l2), C.c_float(w_l1),
+                                  C.c_float(w_ssim), 1 if wavefront else 0, C.c_uint32(wf_flags), C.byref(so),
+                                  C.byref(gs) if tgt is not None else None)
+    if r < 0:
+        raise RuntimeError("oracle capacity")
+    out["n_isect"] = int(r)
+    out["keys"] = out["keys"][:r]
+    out["vals"] = out["vals"][:r]
+    if fixed is not None:
+        out["tile_grads_fixed"] = fixed[:r]
+    if tgt is not None:
+        out["grads"] = grads
+    return out
+
+
+def forward_loss_f64(g: dict, cam, view: OrcView, target, w_l2, w_l1, want_color=False):
+    n = g["pos"].shape[0]
+    arrs = [np.ascontiguousarray(g[k], np.float64) for k in ("pos", "logscale", "quat", "logit", "sh")]
+    t = np.ascontiguousarray(target, np.float32)
+    col = np.zeros((view.height, view.width, 3), np.float64) if want_color else None
+    oc = camera(cam)
+    val = lib().orc_forward_loss_f64(C.c_size_t(n), *[_dp(a) for a in arrs], C.byref(oc), C.byref(view), _fp(t),
+                                     C.c_double(w_l2), C.c_double(w_l1), _dp(col) if col is not None else None)
+    return (val, col) if want_color else val
+
+
+def compact_rows(src: np.ndarray, keep: np.ndarray) -> np.ndarray:
+    s = np.ascontiguousarray(src, np.float32)
+    stride = int(np.prod(s.shape[1:])) if s.ndim > 1 else 1
+    k = np.ascontiguousarray(keep, np.uint8)
+    d = np.zeros_like(s)
+    n = lib().orc_compact_rows(C.c_size_t(s.shape[0]), C.c_size_t(stride), _fp(s), k.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               _fp(d))
+    return d[:n]

@@ -1,0 +1,171 @@
+/*
+ * sv_oracle.h -- CPU restatement of Skyfall-Viewer's splat training step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP
+ * implementation under skyfall-viewer_amd/. Only tests/, the smoke check
+ * in __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
+ * The product library never links or calls it.
+ *
+ * Every function restates one WGSL entry point (or one host routine) of the
+ * reference, with the file:line it follows. Floating-point operations are
+ * written in an explicit order and the library is compiled with
+ * -ffp-contract=off, so each float is the IEEE result of that order.
+ * exp() is orc_expf(): Cody-Waite range reduction plus the Cephes expf
+ * minimax polynomial (S. L. Moshier, Cephes Math Library), so the value is
+ * reproducible on any IEEE machine (WGSL leaves exp() implementation
+ * defined, see src/shaders/training-forward-tiles.wgsl:188,192).
+ *
+ * Parity status of each function (how it is pinned) is listed in the
+ * header comment of sv_oracle.c.
+ */
+#ifndef SV_ORACLE_H
+#define SV_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CameraUniforms, src/shaders/common.wgsl:21-28 (272 bytes, column-major). */
+typedef struct orc_camera {
+  float view[16];
+  float view_inv[16];
+  float proj[16];
+  float proj_inv[16];
+  float viewport[2];
+  float focal[2];
+} orc_camera;
+
+/* TilingParams + ProjectionParams + BackgroundParams + RenderSettings.sh_deg
+ * (training-tiles.wgsl:1-15, training-forward-tiles.wgsl:10-13,
+ * common.wgsl:30-37). */
+typedef struct orc_view {
+  uint32_t width, height, tile_size, tiles_x, tiles_y;
+  uint32_t sh_deg;
+  float near_plane, far_plane, radius_clip, eps2d;
+  float bg[3];
+} orc_view;
+
+/* GaussProjection, common.wgsl:57-65 (48 bytes). */
+typedef struct orc_proj {
+  float mean2d[2];
+  float radius;
+  float depth;
+  float conic[3];
+  uint32_t tiles_count;
+  float position_world[3];
+  float opacity_logit;
+} orc_proj;
+
+/* Training parameters in the reference's f32 training layout
+ * (gaussian-renderer.ts:130-137): pos[3N], log-scale[3N], quaternion
+ * (x,y,z,w)[4N], opacity logit[N], SH[48N] (coefficient k, channel c at
+ * 3k+c). */
+typedef struct orc_params {
+  const float *pos, *logscale, *quat, *logit, *sh;
+} orc_params;
+
+/* Per-Gaussian gradient buffers (gaussian-renderer.ts:216-231). */
+typedef struct orc_grads {
+  float *pos, *logscale, *quat, *logit, *sh, *geom; /* geom: 5N conic a,b,c, mean x,y */
+} orc_grads;
+
+#define ORC_TILE_GRAD_STRIDE 54u
+
+/* Wavefront backward options (backward-tiles-local.wgsl). */
+#define ORC_WF_REFERENCE_ORDER 0x1u /* batches front-to-back, :262 */
+#define ORC_WF_SKIP_098        0x2u /* skip alpha > 0.98, :329-331 */
+
+float orc_expf(float x);
+float orc_f16_round(float x);
+
+void orc_project(size_t n, const orc_params *p, const orc_camera *cam,
+                 const orc_view *v, orc_proj *out);
+void orc_sh_color(const float dir[3], const float *sh48, uint32_t deg,
+                  float rgb[3]);
+void orc_view_dir(const orc_camera *cam, const float pos[3], float dir[3]);
+uint32_t orc_encode_key(uint32_t cam, uint32_t tile, float depth);
+uint64_t orc_count_isects(size_t n, const orc_proj *proj);
+void orc_emit(size_t n, const orc_proj *proj, const orc_view *v,
+              uint32_t *keys, uint32_t *vals);
+void orc_sort_pairs(size_t m, uint32_t *keys, uint32_t *vals);
+void orc_tile_offsets(size_t m, const uint32_t *keys, uint32_t n_tiles,
+                      uint32_t *offsets);
+void orc_forward(const orc_params *p, const orc_proj *proj,
+                 const orc_camera *cam, const orc_view *v,
+                 const uint32_t *offsets, const uint32_t *vals, size_t m,
+                 float *color, float *alpha, uint32_t *last_ids);
+void orc_loss(const float *color, const float *target, uint32_t w, uint32_t h,
+              float w_l2, float w_l1, float w_ssim, float *residual);
+void orc_backward_legacy(size_t n, const orc_params *p, const orc_proj *proj,
+                         const orc_camera *cam, const orc_view *v,
+                         const uint32_t *offsets, const uint32_t *vals,
+                         size_t m, const float *residual, const float *alpha,
+                         const uint32_t *last_ids, orc_grads *g);
+void orc_backward_wavefront(const orc_params *p, const orc_proj *proj,
+                            const orc_camera *cam, const orc_view *v,
+                            const uint32_t *offsets, const uint32_t *vals,
+                            size_t m, const float *residual,
+                            const float *alpha, const uint32_t *last_ids,
+                            uint32_t flags, int32_t *tile_grads_fixed);
+void orc_backward_reduce(size_t n, const int32_t *tile_grads_fixed,
+                         const uint32_t *vals, size_t m, orc_grads *g);
+void orc_backward_geom(size_t n, const orc_params *p, const orc_camera *cam,
+                       orc_grads *g);
+void orc_adam(size_t count, float *param, float *m1, float *m2,
+              const float *grad, float lr, float beta1, float beta2,
+              float eps, float bc1_rcp, float bc2_sqrt_rcp);
+void orc_adam_bias(uint32_t t, float beta1, float beta2, float *bc1_rcp,
+                   float *bc2_sqrt_rcp);
+void orc_accumulate_importance(size_t n, const orc_proj *proj, float decay,
+                               float *stats4);
+void orc_accumulate_grads(size_t n, const float *grad_geom, float *accum4);
+
+/* Whole-step driver: project, emit, sort, offsets, forward, loss, backward
+ * (legacy order-correct float when fixed == NULL, else the wavefront
+ * fixed-point kernel + reduce with wf_flags), geometry backward.
+ * Returns the intersection count. Buffers the caller does not need may be
+ * NULL, except the gradient planes in g. */
+typedef struct orc_step_out {
+  orc_proj *proj;           /* n */
+  uint32_t *keys, *vals;    /* capacity cap_isect each, sorted on return */
+  size_t cap_isect;
+  uint32_t *offsets;        /* tiles + 1 */
+  float *color, *alpha;     /* 3P, P */
+  uint32_t *last_ids;       /* P */
+  float *residual;          /* 3P */
+  int32_t *tile_grads_fixed;/* 54 * cap_isect, wavefront path only */
+} orc_step_out;
+
+int64_t orc_render_and_grad(size_t n, const orc_params *p,
+                            const orc_camera *cam, const orc_view *v,
+                            const float *target, float w_l2, float w_l1,
+                            float w_ssim, int use_wavefront, uint32_t wf_flags,
+                            orc_step_out *o, orc_grads *g);
+
+/* Float64 forward + loss (for central finite differences). The loss is
+ * sum over pixels and channels of w_l2/2 (p-t)^2 + w_l1 |p-t|, whose
+ * gradient is the reference residual (training-loss.wgsl:100-107). */
+double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
+                            const double *quat, const double *logit,
+                            const double *sh, const orc_camera *cam,
+                            const orc_view *v, const float *target,
+                            double w_l2, double w_l1, double *color_out);
+
+/* Stable compaction (gaussian-renderer.ts:2917-3000): rows with keep[i] != 0
+ * are copied in order. Returns the kept count. `stride` floats per row. */
+size_t orc_compact_rows(size_t n, size_t stride, const float *src,
+                        const uint8_t *keep, float *dst);
+
+/* Vector helpers for the tests. */
+void orc_expf_v(size_t n, const float *x, float *y);
+void orc_f16_round_v(size_t n, const float *x, float *y);
+void orc_sh_color_v(size_t n, const float *dirs, const float *sh48,
+                    uint32_t deg, float *rgb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1445 @@
+// sv_api.cpp -- context, device memory and step orchestration behind the
+// C ABI of include/sv_api.h. One context = one HIP stream on one device.
+//
+// Device memory layout (all planar, one plane per scalar):
+//   P, M1, M2, G      59 planes x cap floats (parameters, Adam moments, grads)
+//   geom              5 planes x cap (grad_geom of the last backward)
+//   importance, grad_accum   float4 x cap (prune/densify statistics)
+//   pixel arena       every view's pixels back to back (view v at pix_off):
+//                     colour, residual, target as 3 planes of P_v floats,
+//                     alpha and last index as 1 plane
+//   batch arrays      per (slot, Gaussian): 4 record planes, count, offset;
+//                     per intersection: key, value (ping-pong), 9-float
+//                     adjoint record; per global tile: range start, slot
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/sv_api.h"
+#include "sv_internal.hpp"
+
+using namespace sv;
+
+namespace {
+
+struct HostView {
+  sv_camera cam;
+  uint32_t W, H, tiles_x, tiles_y, n_tiles;
+  uint64_t pix_off;
+  bool has_target;
+};
+
+struct TimedPhase {
+  int id;
+  hipEvent_t a, b;
+};
+
+const char *kPhaseNames[] = {"project", "scan", "emit", "sort", "ranges", "forward", "backward",
+                             "vjp_adam", "adam", "loss", "backward_fixed", "compact", "readback_sync"};
+constexpr int kNumPhases = sizeof(kPhaseNames) / sizeof(kPhaseNames[0]);
+enum Phase {
+  PH_PROJECT,
+  PH_SCAN,
+  PH_EMIT,
+  PH_SORT,
+  PH_RANGES,
+  PH_FORWARD,
+  PH_BACKWARD,
+  PH_VJP,
+  PH_ADAM,
+  PH_LOSS,
+  PH_FIXED,
+  PH_COMPACT,
+  PH_SYNC
+};
+
+uint32_t bitlen(uint64_t v) {
+  uint32_t b = 0;
+  while (v) {
+    ++b;
+    v >>= 1;
+  }
+  return b;
+}
+
+}  // namespace
+
+struct sv_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  size_t cap = 0;
+  uint32_t N = 0;
+  sv_config cfg{};
+  std::string err;
+  uint32_t step_t = 0;
+
+  float *P = nullptr, *M1 = nullptr, *M2 = nullptr, *G = nullptr, *geom = nullptr;
+  float4 *importance = nullptr, *grad_accum = nullptr;
+
+  std::vector<HostView> views;
+  uint64_t pix_total = 0;
+  float *color = nullptr, *alpha = nullptr, *residual = nullptr, *target = nullptr;
+  uint32_t *last = nullptr;
+  std::vector<uint8_t> residual_ready;
+
+  // batch state
+  std::vector<int> slot_views;  // views of the current batch
+  std::vector<DevView> h_slots;
+  DevView *d_views = nullptr;
+  size_t d_views_cap = 0;
+  uint32_t tiles_total = 0;
+  uint32_t *tile_view = nullptr, *ranges = nullptr;
+  float *l1_partial = nullptr;
+  size_t tiles_cap = 0;
+  float4 *rec = nullptr;
+  uint32_t *counts = nullptr, *offs = nullptr;
+  size_t vg_cap = 0;
+  uint64_t *scan_tmp = nullptr;
+  size_t scan_tmp_cap = 0;
+  uint64_t *d_small = nullptr;  // [0] total, [1] max depth quantum, [2] work counter
+  void *keys = nullptr, *keys2 = nullptr;
+  uint32_t *vals = nullptr, *vals2 = nullptr;
+  uint64_t isect_cap = 0;
+  uint32_t *sort_tmp = nullptr;
+  size_t sort_tmp_cap = 0;
+  float *isect_rec = nullptr;
+  int32_t *fixed = nullptr;
+  uint64_t fixed_cap = 0;
+  uint64_t *h_small = nullptr;  // pinned
+
+  // last tile build
+  bool built = false;
+  bool forward_done = false;
+  uint64_t I = 0;
+  uint32_t dbits = 0, key_bits = 0;
+  bool key64 = false;
+  int passes = 0;
+
+  // timing
+  bool timing = false;
+  std::vector<TimedPhase> pending;
+  std::vector<hipEvent_t> event_pool;
+  double phase_ms[kNumPhases] = {0};
+  uint64_t phase_n[kNumPhases] = {0};
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+sv_status fail(sv_ctx *c, sv_status s, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  else g_create_err = buf;
+  return s;
+}
+
+#define HIP_OK(c, expr)                                                                   \
+  do {                                                                                    \
+    hipError_t _e = (expr);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return fail((c), SV_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                    \
+  } while (0)
+
+sv_status check_launch(sv_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, SV_E_HIP, "%s: %s", what, hipGetErrorString(e));
+  return SV_OK;
+}
+
+template <typename T>
+sv_status dalloc(sv_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(c, SV_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  }
+  return SV_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// Grow a device array to at least n elements (contents are not kept).
+template <typename T>
+sv_status grow(sv_ctx *c, T **p, size_t &capn, size_t n) {
+  if (n <= capn && *p) return SV_OK;
+  dfree(*p);
+  size_t want = std::max(n, capn + capn / 4);
+  sv_status s = dalloc(c, p, want);
+  if (s) {
+    capn = 0;
+    return s;
+  }
+  capn = want;
+  return SV_OK;
+}
+
+void ph_begin(sv_ctx *c, int id, TimedPhase *tp) {
+  tp->id = -1;
+  if (!c->timing) return;
+  if (c->event_pool.size() < 2) {
+    for (int i = 0; i < 16; ++i) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) == hipSuccess) c->event_pool.push_back(e);
+    }
+  }
+  if (c->event_pool.size() < 2) return;
+  tp->a = c->event_pool.back();
+  c->event_pool.pop_back();
+  tp->b = c->event_pool.back();
+  c->event_pool.pop_back();
+  tp->id = id;
+  (void)hipEventRecord(tp->a, c->st);
+}
+void ph_end(sv_ctx *c, TimedPhase *tp) {
+  if (tp->id < 0) return;
+  (void)hipEventRecord(tp->b, c->st);
+  c->pending.push_back(*tp);
+}
+void resolve_timings(sv_ctx *c) {
+  for (auto &tp : c->pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(tp.b) == hipSuccess && hipEventElapsedTime(&ms, tp.a, tp.b) == hipSuccess) {
+      c->phase_ms[tp.id] += ms;
+      c->phase_n[tp.id] += 1;
+    }
+    c->event_pool.push_back(tp.a);
+    c->event_pool.push_back(tp.b);
+  }
+  c->pending.clear();
+}
+
+struct Scoped {
+  sv_ctx *c;
+  TimedPhase tp;
+  Scoped(sv_ctx *cc, int id) : c(cc) { ph_begin(c, id, &tp); }
+  ~Scoped() { ph_end(c, &tp); }
+};
+
+sv_status sync(sv_ctx *c) {
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  resolve_timings(c);
+  return SV_OK;
+}
+
+DevConfig dev_config(const sv_config &k) {
+  DevConfig d;
+  d.bg[0] = k.bg[0];
+  d.bg[1] = k.bg[1];
+  d.bg[2] = k.bg[2];
+  d.near_plane = k.near_plane;
+  d.far_plane = k.far_plane;
+  d.radius_clip = k.radius_clip;
+  d.eps2d = k.eps2d;
+  d.w_l2 = k.w_l2;
+  d.w_l1 = k.w_l1;
+  d.w_ssim = k.w_ssim;
+  d.tile_size = k.tile_size;
+  d.sh_deg = k.sh_deg;
+  d.render_f16 = k.render_f16;
+  d.importance_decay = k.importance_decay;
+  return d;
+}
+
+sv_status check_ctx(sv_ctx *c) {
+  if (!c) return SV_E_ARG;
+  HIP_OK(c, hipSetDevice(c->device));
+  return SV_OK;
+}
+
+sv_status check_view(sv_ctx *c, int v) {
+  if (v < 0 || v >= (int)c->views.size())
+    return fail(c, SV_E_ARG, "view %d out of range (%zu views)", v, c->views.size());
+  return SV_OK;
+}
+
+// ---------------------------------------------------------------------
+// Batch: a list of views rendered through one pipeline.
+// ---------------------------------------------------------------------
+sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
+  if (nv <= 0) return fail(c, SV_E_ARG, "no views");
+  if (c->N == 0) return fail(c, SV_E_STATE, "no Gaussians");
+  for (int i = 0; i < nv; ++i)
+    if (sv_status s = check_view(c, vs[i])) return s;
+  const bool same = c->slot_views.size() == (size_t)nv &&
+                    std::equal(c->slot_views.begin(), c->slot_views.end(), vs);
+  c->built = false;
+  c->forward_done = false;
+  if (!same) {
+    c->slot_views.assign(vs, vs + nv);
+    c->h_slots.resize(nv);
+    uint32_t tb = 0;
+    for (int s = 0; s < nv; ++s) {
+      const HostView &hv = c->views[vs[s]];
+      DevView &d = c->h_slots[s];
+      std::memcpy(d.view, hv.cam.view, sizeof(d.view));
+      std::memcpy(d.proj, hv.cam.proj, sizeof(d.proj));
+      d.focal[0] = hv.cam.focal[0];
+      d.focal[1] = hv.cam.focal[1];
+      d.viewport[0] = hv.cam.viewport[0];
+      d.viewport[1] = hv.cam.viewport[1];
+      d.W = hv.W;
+      d.H = hv.H;
+      d.tiles_x = hv.tiles_x;
+      d.tiles_y = hv.tiles_y;
+      d.n_tiles = hv.n_tiles;
+      d.tile_base = tb;
+      d.pix_off = hv.pix_off;
+      tb += hv.n_tiles;
+    }
+    c->tiles_total = tb;
+    if (sv_status s = grow(c, &c->d_views, c->d_views_cap, (size_t)nv)) return s;
+    HIP_OK(c, hipMemcpyAsync(c->d_views, c->h_slots.data(), nv * sizeof(DevView), hipMemcpyHostToDevice, c->st));
+    if (c->tiles_cap < (size_t)tb + 1) {
+      dfree(c->tile_view);
+      dfree(c->ranges);
+      dfree(c->l1_partial);
+      c->tiles_cap = 0;
+      size_t want = (size_t)tb + 1;
+      if (sv_status s = dalloc(c, &c->tile_view, want)) return s;
+      if (sv_status s = dalloc(c, &c->ranges, want)) return s;
+      if (sv_status s = dalloc(c, &c->l1_partial, want)) return s;
+      c->tiles_cap = want;
+    }
+    std::vector<uint32_t> tv(tb);
+    for (int s = 0; s < nv; ++s)
+      for (uint32_t t = 0; t < c->h_slots[s].n_tiles; ++t) tv[c->h_slots[s].tile_base + t] = (uint32_t)s;
+    if (tb) HIP_OK(c, hipMemcpyAsync(c->tile_view, tv.data(), tb * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+    // the host vector must outlive the async copy
+    HIP_OK(c, hipStreamSynchronize(c->st));
+  }
+  const size_t M = (size_t)nv * c->N;
+  if (M > c->vg_cap || !c->rec) {
+    dfree(c->rec);
+    dfree(c->counts);
+    dfree(c->offs);
+    c->vg_cap = 0;
+    if (sv_status s = dalloc(c, &c->rec, 4 * M)) return s;
+    if (sv_status s = dalloc(c, &c->counts, M)) return s;
+    if (sv_status s = dalloc(c, &c->offs, M)) return s;
+    c->vg_cap = M;
+  }
+  if (sv_status s = grow(c, &c->scan_tmp, c->scan_tmp_cap, scan_tmp_words(M))) return s;
+  return SV_OK;
+}
+
+size_t rec_stride(const sv_ctx *c) { return (size_t)c->slot_views.size() * c->N; }
+
+// buildTileIntersections (gaussian-renderer.ts:1780-2023) for the batch.
+sv_status build_batch(sv_ctx *c) {
+  const int nv = (int)c->slot_views.size();
+  const DevConfig dc = dev_config(c->cfg);
+  const size_t M = (size_t)nv * c->N;
+  const size_t rs = rec_stride(c);
+  launch_fill_u32(c->st, (uint32_t *)c->d_small, 0u, 6);
+  {
+    Scoped t(c, PH_PROJECT);
+    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, rs, c->counts,
+                   (uint32_t *)(c->d_small + 1));
+  }
+  if (sv_status s = check_launch(c, "project")) return s;
+  {
+    Scoped t(c, PH_SCAN);
+    launch_scan(c->st, c->counts, c->offs, M, c->scan_tmp, c->d_small);
+  }
+  if (sv_status s = check_launch(c, "scan")) return s;
+  HIP_OK(c, hipMemcpyAsync(c->h_small, c->d_small, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
+  if (sv_status s = sync(c)) return s;
+  const uint64_t I = c->h_small[0];
+  const uint32_t maxdq = (uint32_t)(c->h_small[1] & 0xffffffffu);
+  if (I >= 0xffffffffull)
+    return fail(c, SV_E_CAPACITY, "%llu intersections exceed the 32-bit index range", (unsigned long long)I);
+  if (I > c->isect_cap) {
+    const uint64_t want = std::max<uint64_t>(I + I / 4, 1024);
+    dfree(c->keys);
+    dfree(c->keys2);
+    dfree(c->vals);
+    dfree(c->vals2);
+    dfree(c->isect_rec);
+    c->isect_cap = 0;
+    if (sv_status s = dalloc(c, (uint64_t **)&c->keys, want)) return s;
+    if (sv_status s = dalloc(c, (uint64_t **)&c->keys2, want)) return s;
+    if (sv_status s = dalloc(c, &c->vals, want)) return s;
+    if (sv_status s = dalloc(c, &c->vals2, want)) return s;
+    if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC9)) return s;
+    c->isect_cap = want;
+  }
+  c->I = I;
+  c->dbits = bitlen(maxdq);
+  const uint32_t tile_bits = bitlen(c->tiles_total > 0 ? c->tiles_total - 1 : 0);
+  c->key_bits = tile_bits + c->dbits;
+  c->key64 = c->key_bits > 32;
+  const size_t stw = c->key64 ? sort_tmp_words<uint64_t>(I) : sort_tmp_words<uint32_t>(I);
+  if (sv_status s = grow(c, &c->sort_tmp, c->sort_tmp_cap, stw)) return s;
+  {
+    Scoped t(c, PH_EMIT);
+    if (c->key64)
+      launch_emit<uint64_t>(c->st, c->rec, rs, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+                            (uint64_t *)c->keys, c->vals);
+    else
+      launch_emit<uint32_t>(c->st, c->rec, rs, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+                            (uint32_t *)c->keys, c->vals);
+  }
+  if (sv_status s = check_launch(c, "emit")) return s;
+  {
+    Scoped t(c, PH_SORT);
+    if (c->key64)
+      launch_sort<uint64_t>(c->st, (uint64_t *)c->keys, c->vals, (uint64_t *)c->keys2, c->vals2, I,
+                            c->key_bits, c->sort_tmp, &c->passes);
+    else
+      launch_sort<uint32_t>(c->st, (uint32_t *)c->keys, c->vals, (uint32_t *)c->keys2, c->vals2, I,
+                            c->key_bits, c->sort_tmp, &c->passes);
+  }
+  if (sv_status s = check_launch(c, "sort")) return s;
+  if (c->passes & 1) {
+    std::swap(c->keys, c->keys2);
+    std::swap(c->vals, c->vals2);
+  }
+  {
+    Scoped t(c, PH_RANGES);
+    if (c->key64)
+      launch_ranges<uint64_t>(c->st, (const uint64_t *)c->keys, I, c->dbits, c->tiles_total, c->ranges);
+    else
+      launch_ranges<uint32_t>(c->st, (const uint32_t *)c->keys, I, c->dbits, c->tiles_total, c->ranges);
+  }
+  if (sv_status s = check_launch(c, "ranges")) return s;
+  c->built = true;
+  return SV_OK;
+}
+
+RasterArgs raster_args(sv_ctx *c) {
+  RasterArgs a{};
+  a.views = c->d_views;
+  a.tile_view = c->tile_view;
+  a.ranges = c->ranges;
+  a.vals = c->vals;
+  a.n_isect = c->I;
+  a.rec = c->rec;
+  a.rs = rec_stride(c);
+  a.N = c->N;
+  a.n_tiles_total = c->tiles_total;
+  a.cfg = dev_config(c->cfg);
+  a.color = c->color;
+  a.alpha = c->alpha;
+  a.last = c->last;
+  a.target = c->target;
+  a.residual = nullptr;
+  a.offs = c->offs;
+  a.isect_rec = c->isect_rec;
+  a.l1_partial = c->l1_partial;
+  a.work_counter = nullptr;
+  return a;
+}
+
+uint32_t raster_grid(sv_ctx *c, RasterArgs &a) {
+  if (c->cfg.persistent) {
+    a.work_counter = (uint32_t *)(c->d_small + 2);
+    launch_fill_u32(c->st, a.work_counter, 0u, 1);
+    return std::min<uint32_t>(c->tiles_total, 256u * 8u);
+  }
+  return c->tiles_total;
+}
+
+sv_status forward_batch(sv_ctx *c) {
+  if (!c->built) return fail(c, SV_E_STATE, "tiles not built");
+  RasterArgs a = raster_args(c);
+  const uint32_t grid = raster_grid(c, a);
+  {
+    Scoped t(c, PH_FORWARD);
+    launch_forward(c->st, a, c->cfg.batch, c->cfg.raster_px, grid);
+  }
+  if (sv_status s = check_launch(c, "forward")) return s;
+  c->forward_done = true;
+  return SV_OK;
+}
+
+// use_residual: read the residual arena instead of the fused loss.
+sv_status backward_batch(sv_ctx *c, bool use_residual) {
+  if (!c->forward_done) return fail(c, SV_E_STATE, "forward not run for this batch");
+  for (int v : c->slot_views)
+    if (!use_residual && !c->views[v].has_target) return fail(c, SV_E_STATE, "view %d has no target", v);
+  RasterArgs a = raster_args(c);
+  if (use_residual) a.residual = c->residual;
+  const uint32_t grid = raster_grid(c, a);
+  {
+    Scoped t(c, PH_BACKWARD);
+    launch_backward(c->st, a, c->cfg.batch, grid);
+  }
+  return check_launch(c, "backward");
+}
+
+void adam_hparams(sv_ctx *c, uint32_t t, float lr[5], float *bc1, float *bc2s, int *update_sh) {
+  const sv_config &k = c->cfg;
+  double lr_pos = k.lr_pos;
+  if (k.pos_lr_schedule && k.lr_pos > 0.0f) {
+    // computePositionLearningRate (gaussian-renderer.ts:284-298)
+    const double lr_final = 0.0000016, delay_mult = 0.01, max_steps = 30000.0;
+    const double step = std::max<double>(t, 0.0);
+    const double tt = std::min(1.0, step / max_steps);
+    const double loglerp = std::exp(std::log((double)k.lr_pos) * (1 - tt) + std::log(lr_final) * tt);
+    const double phase = std::min(1.0, step / std::max(1.0, std::round(0.1 * max_steps)));
+    const double delay = delay_mult + (1.0 - delay_mult) * std::sin(0.5 * M_PI * phase);
+    lr_pos = loglerp * delay;
+  }
+  lr[0] = (float)lr_pos;
+  lr[1] = k.lr_scale;
+  lr[2] = k.lr_rot;
+  lr[3] = k.lr_opacity;
+  lr[4] = k.lr_sh;
+  // run_adam_on_buffer (gaussian-renderer.ts:2425-2440): bias terms in
+  // double, stored as f32.
+  const double tt = t < 1 ? 1.0 : (double)t;
+  *bc1 = (float)(1.0 / (1.0 - std::pow((double)k.beta1, tt)));
+  *bc2s = (float)(1.0 / std::sqrt(1.0 - std::pow((double)k.beta2, tt)));
+  *update_sh = (k.sh_update_interval == 0) ? 1 : ((t % k.sh_update_interval) == 0);
+}
+
+VjpArgs vjp_args(sv_ctx *c) {
+  VjpArgs v{};
+  v.P = c->P;
+  v.cap = c->cap;
+  v.N = c->N;
+  v.views = c->d_views;
+  v.nv = (int)c->slot_views.size();
+  v.cfg = dev_config(c->cfg);
+  v.counts = c->counts;
+  v.offs = c->offs;
+  v.rec = c->rec;
+  v.rs = rec_stride(c);
+  v.isect_rec = c->isect_rec;
+  v.importance = c->importance;
+  v.G = c->G;
+  v.geom = c->geom;
+  v.grad_accum = c->grad_accum;
+  v.Pw = c->P;
+  v.M1 = c->M1;
+  v.M2 = c->M2;
+  return v;
+}
+
+sv_status zero_grads(sv_ctx *c) {
+  launch_fill_f32(c->st, c->G, 0.0f, (size_t)P_NUM * c->cap);
+  launch_fill_f32(c->st, c->geom, 0.0f, (size_t)5 * c->cap);
+  return check_launch(c, "zero grads");
+}
+
+sv_status adam_planes(sv_ctx *c) {
+  c->step_t += 1;
+  float lr[5], bc1, bc2s;
+  int upd;
+  adam_hparams(c, c->step_t, lr, &bc1, &bc2s, &upd);
+  {
+    Scoped t(c, PH_ADAM);
+    launch_adam(c->st, c->P, c->M1, c->M2, c->G, c->cap, c->N, lr, c->cfg.beta1, c->cfg.beta2, c->cfg.eps,
+                bc1, bc2s, upd);
+  }
+  return check_launch(c, "adam");
+}
+
+sv_status upload_planar(sv_ctx *c, float *dst, size_t cap, const float *src, size_t n, int comps,
+                        int plane0) {
+  if (!src) return SV_OK;
+  std::vector<float> tmp(n);
+  for (int k = 0; k < comps; ++k) {
+    for (size_t i = 0; i < n; ++i) tmp[i] = src[i * comps + k];
+    HIP_OK(c, hipMemcpy(dst + (size_t)(plane0 + k) * cap, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return SV_OK;
+}
+sv_status download_planar(sv_ctx *c, const float *src, size_t cap, float *dst, size_t n, int comps,
+                          int plane0) {
+  if (!dst) return SV_OK;
+  std::vector<float> tmp(n);
+  for (int k = 0; k < comps; ++k) {
+    HIP_OK(c, hipMemcpy(tmp.data(), src + (size_t)(plane0 + k) * cap, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) dst[i * comps + k] = tmp[i];
+  }
+  return SV_OK;
+}
+
+sv_status set_params(sv_ctx *c, float *base, const float *pos, const float *ls, const float *q,
+                     const float *lg, const float *sh) {
+  if (sv_status s = upload_planar(c, base, c->cap, pos, c->N, 3, P_POS)) return s;
+  if (sv_status s = upload_planar(c, base, c->cap, ls, c->N, 3, P_SCALE)) return s;
+  if (sv_status s = upload_planar(c, base, c->cap, q, c->N, 4, P_QUAT)) return s;
+  if (sv_status s = upload_planar(c, base, c->cap, lg, c->N, 1, P_LOGIT)) return s;
+  return upload_planar(c, base, c->cap, sh, c->N, 48, P_SH);
+}
+sv_status get_params(sv_ctx *c, const float *base, float *pos, float *ls, float *q, float *lg, float *sh) {
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  if (sv_status s = download_planar(c, base, c->cap, pos, c->N, 3, P_POS)) return s;
+  if (sv_status s = download_planar(c, base, c->cap, ls, c->N, 3, P_SCALE)) return s;
+  if (sv_status s = download_planar(c, base, c->cap, q, c->N, 4, P_QUAT)) return s;
+  if (sv_status s = download_planar(c, base, c->cap, lg, c->N, 1, P_LOGIT)) return s;
+  return download_planar(c, base, c->cap, sh, c->N, 48, P_SH);
+}
+
+sv_status alloc_gaussian_arrays(sv_ctx *c, size_t cap) {
+  if (sv_status s = dalloc(c, &c->P, (size_t)P_NUM * cap)) return s;
+  if (sv_status s = dalloc(c, &c->M1, (size_t)P_NUM * cap)) return s;
+  if (sv_status s = dalloc(c, &c->M2, (size_t)P_NUM * cap)) return s;
+  if (sv_status s = dalloc(c, &c->G, (size_t)P_NUM * cap)) return s;
+  if (sv_status s = dalloc(c, &c->geom, (size_t)5 * cap)) return s;
+  if (sv_status s = dalloc(c, &c->importance, cap)) return s;
+  if (sv_status s = dalloc(c, &c->grad_accum, cap)) return s;
+  return SV_OK;
+}
+
+void clear_state(sv_ctx *c) {
+  launch_fill_f32(c->st, c->M1, 0.f, (size_t)P_NUM * c->cap);
+  launch_fill_f32(c->st, c->M2, 0.f, (size_t)P_NUM * c->cap);
+  launch_fill_f32(c->st, c->G, 0.f, (size_t)P_NUM * c->cap);
+  launch_fill_f32(c->st, c->geom, 0.f, (size_t)5 * c->cap);
+  launch_fill_f32(c->st, (float *)c->importance, 0.f, (size_t)4 * c->cap);
+  launch_fill_f32(c->st, (float *)c->grad_accum, 0.f, (size_t)4 * c->cap);
+}
+
+sv_status ensure_view_built(sv_ctx *c, int v) {
+  if (c->built && c->slot_views.size() == 1 && c->slot_views[0] == v) return SV_OK;
+  if (sv_status s = prepare_batch(c, &v, 1)) return s;
+  return build_batch(c);
+}
+
+sv_status read_pixels3(sv_ctx *c, const float *arena, int v, float *rgb) {
+  if (sv_status s = check_view(c, v)) return s;
+  if (!rgb) return fail(c, SV_E_ARG, "null output");
+  const HostView &hv = c->views[v];
+  const size_t P = (size_t)hv.W * hv.H;
+  std::vector<float> tmp(3 * P);
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  HIP_OK(c, hipMemcpy(tmp.data(), arena + 3 * hv.pix_off, 3 * P * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < P; ++i)
+    for (int ch = 0; ch < 3; ++ch) rgb[3 * i + ch] = tmp[ch * P + i];
+  return SV_OK;
+}
+
+sv_status write_pixels3(sv_ctx *c, float *arena, int v, const float *rgb) {
+  const HostView &hv = c->views[v];
+  const size_t P = (size_t)hv.W * hv.H;
+  std::vector<float> tmp(3 * P);
+  for (size_t i = 0; i < P; ++i)
+    for (int ch = 0; ch < 3; ++ch) tmp[ch * P + i] = rgb[3 * i + ch];
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  HIP_OK(c, hipMemcpy(arena + 3 * hv.pix_off, tmp.data(), 3 * P * sizeof(float), hipMemcpyHostToDevice));
+  return SV_OK;
+}
+
+// The batched step needs every view's residual either fused or read.
+bool batch_needs_residual_buffer(sv_ctx *c) { return c->cfg.w_ssim > 0.0f; }
+
+sv_status loss_views(sv_ctx *c) {
+  const DevConfig dc = dev_config(c->cfg);
+  Scoped t(c, PH_LOSS);
+  for (size_t s = 0; s < c->slot_views.size(); ++s) {
+    const HostView &hv = c->views[c->slot_views[s]];
+    launch_loss(c->st, c->h_slots[s], dc, c->color + 3 * hv.pix_off, c->target + 3 * hv.pix_off,
+                c->residual + 3 * hv.pix_off);
+  }
+  return check_launch(c, "loss");
+}
+
+sv_status step_batched(sv_ctx *c, const int *vs, int nv) {
+  if (sv_status s = prepare_batch(c, vs, nv)) return s;
+  if (sv_status s = build_batch(c)) return s;
+  if (sv_status s = forward_batch(c)) return s;
+  const bool use_res = batch_needs_residual_buffer(c);
+  if (use_res)
+    if (sv_status s = loss_views(c)) return s;
+  if (sv_status s = backward_batch(c, use_res)) return s;
+  VjpArgs va = vjp_args(c);
+  if (c->cfg.keep_grads) {
+    if (sv_status s = zero_grads(c)) return s;
+    {
+      Scoped t(c, PH_VJP);
+      launch_vjp(c->st, va);
+    }
+    if (sv_status s = check_launch(c, "vjp")) return s;
+    return adam_planes(c);
+  }
+  c->step_t += 1;
+  float lr[5], bc1, bc2s;
+  int upd;
+  adam_hparams(c, c->step_t, lr, &bc1, &bc2s, &upd);
+  va.G = nullptr;
+  va.geom = nullptr;
+  for (int k = 0; k < 5; ++k) va.lr[k] = lr[k];
+  va.beta1 = c->cfg.beta1;
+  va.beta2 = c->cfg.beta2;
+  va.eps = c->cfg.eps;
+  va.bc1_rcp = bc1;
+  va.bc2_sqrt_rcp = bc2s;
+  va.update_sh = upd;
+  {
+    Scoped t(c, PH_VJP);
+    launch_vjp(c->st, va);
+  }
+  return check_launch(c, "vjp+adam");
+}
+
+double sum_l1(sv_ctx *c) {
+  std::vector<float> t(c->tiles_total);
+  if (c->tiles_total == 0) return 0.0;
+  if (hipMemcpy(t.data(), c->l1_partial, c->tiles_total * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    return -1.0;
+  double s = 0.0;
+  for (float v : t) s += v;
+  return s;
+}
+
+// Replace the Gaussian arrays by a compacted / grown copy of n_new rows.
+sv_status swap_in(sv_ctx *c, float *Pn, float *M1n, float *M2n, size_t cap_n, uint32_t n_new) {
+  dfree(c->P);
+  dfree(c->M1);
+  dfree(c->M2);
+  c->P = Pn;
+  c->M1 = M1n;
+  c->M2 = M2n;
+  if (cap_n != c->cap) {
+    dfree(c->G);
+    dfree(c->geom);
+    dfree(c->importance);
+    dfree(c->grad_accum);
+    if (sv_status s = dalloc(c, &c->G, (size_t)P_NUM * cap_n)) return s;
+    if (sv_status s = dalloc(c, &c->geom, (size_t)5 * cap_n)) return s;
+    if (sv_status s = dalloc(c, &c->importance, cap_n)) return s;
+    if (sv_status s = dalloc(c, &c->grad_accum, cap_n)) return s;
+    c->cap = cap_n;
+  }
+  c->N = n_new;
+  // statistics restart after prune/densify (gaussian-renderer.ts:2990-2992)
+  launch_fill_f32(c->st, c->G, 0.f, (size_t)P_NUM * c->cap);
+  launch_fill_f32(c->st, c->geom, 0.f, (size_t)5 * c->cap);
+  launch_fill_f32(c->st, (float *)c->importance, 0.f, (size_t)4 * c->cap);
+  launch_fill_f32(c->st, (float *)c->grad_accum, 0.f, (size_t)4 * c->cap);
+  c->slot_views.clear();
+  c->built = false;
+  c->forward_done = false;
+  return check_launch(c, "swap_in");
+}
+
+// Compact by keep flags already on the device (keep8 bytes, keep32 words).
+sv_status compact_by_keep(sv_ctx *c, uint8_t *keep8, uint32_t *keep32, size_t *n_removed) {
+  uint32_t *offs = nullptr;
+  uint64_t *tmp = nullptr;
+  if (sv_status s = dalloc(c, &offs, c->N)) return s;
+  if (sv_status s = dalloc(c, &tmp, scan_tmp_words(c->N))) {
+    dfree(offs);
+    return s;
+  }
+  launch_scan(c->st, keep32, offs, c->N, tmp, c->d_small);
+  HIP_OK(c, hipMemcpyAsync(c->h_small, c->d_small, sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  const uint32_t n_keep = (uint32_t)c->h_small[0];
+  const size_t removed = c->N - n_keep;
+  if (n_removed) *n_removed = removed;
+  // applyPruneMask: nothing to do when nothing or everything is removed
+  // (gaussian-renderer.ts:2922-2925)
+  if (removed == 0 || n_keep == 0) {
+    if (n_removed) *n_removed = 0;
+    dfree(offs);
+    dfree(tmp);
+    return SV_OK;
+  }
+  float *Pn = nullptr, *M1n = nullptr, *M2n = nullptr;
+  if (sv_status s = dalloc(c, &Pn, (size_t)P_NUM * c->cap)) return s;
+  if (sv_status s = dalloc(c, &M1n, (size_t)P_NUM * c->cap)) return s;
+  if (sv_status s = dalloc(c, &M2n, (size_t)P_NUM * c->cap)) return s;
+  {
+    Scoped t(c, PH_COMPACT);
+    launch_compact(c->st, c->P, Pn, c->cap, c->cap, P_NUM, offs, keep8, c->N);
+    launch_compact(c->st, c->M1, M1n, c->cap, c->cap, P_NUM, offs, keep8, c->N);
+    launch_compact(c->st, c->M2, M2n, c->cap, c->cap, P_NUM, offs, keep8, c->N);
+  }
+  if (sv_status s = check_launch(c, "compact")) return s;
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  dfree(offs);
+  dfree(tmp);
+  return swap_in(c, Pn, M1n, M2n, c->cap, n_keep);
+}
+
+}  // namespace
+
+// =====================================================================
+// C ABI
+// =====================================================================
+extern "C" {
+
+const char *sv_version(void) { return "skyfall-mi355x 0.1 (gfx950)"; }
+
+sv_status sv_default_config(sv_config *o) {
+  if (!o) return SV_E_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->tile_size = 16;
+  o->batch = 64;
+  o->raster_px = 16;
+  o->sh_deg = 3;
+  o->near_plane = 0.2f;
+  o->far_plane = 10000.0f;
+  o->radius_clip = 512.0f;
+  o->eps2d = 0.3f;
+  o->w_l2 = 0.0f;
+  o->w_l1 = 0.8f;
+  o->w_ssim = 0.2f;
+  o->lr_pos = 0.00016f;
+  o->lr_sh = 0.0025f;
+  o->lr_opacity = 0.025f;
+  o->lr_scale = 0.005f;
+  o->lr_rot = 0.001f;
+  o->beta1 = 0.9f;
+  o->beta2 = 0.999f;
+  o->eps = 1e-8f;
+  o->sh_update_interval = 16;
+  o->pos_lr_schedule = 1;
+  o->render_f16 = 0;
+  o->importance_decay = 0.98f;
+  o->persistent = 0;
+  o->use_graph = 0;
+  o->keep_grads = 0;
+  return SV_OK;
+}
+
+const char *sv_last_error(const sv_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
+  if (!out) return fail(nullptr, SV_E_ARG, "null out");
+  *out = nullptr;
+  if (capacity == 0) return fail(nullptr, SV_E_ARG, "capacity must be > 0");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0)
+    return fail(nullptr, SV_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= ndev) return fail(nullptr, SV_E_ARG, "device %d out of range", device);
+  sv_ctx *c = new sv_ctx();
+  c->device = device;
+  sv_default_config(&c->cfg);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
+    fail(nullptr, SV_E_HIP, "cannot create a stream on device %d", device);
+    delete c;
+    return SV_E_HIP;
+  }
+  c->cap = capacity;
+  sv_status s = alloc_gaussian_arrays(c, capacity);
+  if (!s) s = dalloc(c, &c->d_small, 8);
+  if (!s && hipHostMalloc((void **)&c->h_small, 8 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+    s = fail(c, SV_E_NOMEM, "hipHostMalloc failed");
+  if (s) {
+    g_create_err = c->err;
+    sv_destroy(c);
+    return s;
+  }
+  clear_state(c);
+  launch_fill_f32(c->st, c->P, 0.f, (size_t)P_NUM * c->cap);
+  if (hipStreamSynchronize(c->st) != hipSuccess) {
+    g_create_err = "initial clear failed";
+    sv_destroy(c);
+    return SV_E_HIP;
+  }
+  *out = c;
+  return SV_OK;
+}
+
+sv_status sv_destroy(sv_ctx *c) {
+  if (!c) return SV_E_ARG;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (auto &tp : c->pending) {
+    c->event_pool.push_back(tp.a);
+    c->event_pool.push_back(tp.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  dfree(c->P);
+  dfree(c->M1);
+  dfree(c->M2);
+  dfree(c->G);
+  dfree(c->geom);
+  dfree(c->importance);
+  dfree(c->grad_accum);
+  dfree(c->color);
+  dfree(c->alpha);
+  dfree(c->residual);
+  dfree(c->target);
+  dfree(c->last);
+  dfree(c->d_views);
+  dfree(c->tile_view);
+  dfree(c->ranges);
+  dfree(c->l1_partial);
+  dfree(c->rec);
+  dfree(c->counts);
+  dfree(c->offs);
+  dfree(c->scan_tmp);
+  dfree(c->d_small);
+  dfree(c->keys);
+  dfree(c->keys2);
+  dfree(c->vals);
+  dfree(c->vals2);
+  dfree(c->sort_tmp);
+  dfree(c->isect_rec);
+  dfree(c->fixed);
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+  return SV_OK;
+}
+
+sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
+  if (!c || !k) return SV_E_ARG;
+  if (k->tile_size != 16) return fail(c, SV_E_ARG, "tile_size must be 16 (got %u)", k->tile_size);
+  if (k->batch != 64 && k->batch != 128 && k->batch != 256)
+    return fail(c, SV_E_ARG, "batch must be 64, 128 or 256 (got %u)", k->batch);
+  if (k->raster_px != 16) return fail(c, SV_E_ARG, "raster_px must be 16 (got %u)", k->raster_px);
+  if (k->sh_deg > 3) return fail(c, SV_E_ARG, "sh_deg must be <= 3");
+  if (k->use_graph) return fail(c, SV_E_ARG, "use_graph is not available in this build");
+  c->cfg = *k;
+  return SV_OK;
+}
+
+sv_status sv_get_config(const sv_ctx *c, sv_config *k) {
+  if (!c || !k) return SV_E_ARG;
+  *k = c->cfg;
+  return SV_OK;
+}
+
+sv_status sv_set_gaussians(sv_ctx *c, size_t n, const float *pos, const float *ls, const float *q,
+                           const float *lg, const float *sh) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!pos || !ls || !q || !lg || !sh) return fail(c, SV_E_ARG, "null parameter pointer");
+  if (n > c->cap) return fail(c, SV_E_CAPACITY, "%zu Gaussians exceed capacity %zu", n, c->cap);
+  if (n > 0xffffffffull) return fail(c, SV_E_CAPACITY, "too many Gaussians");
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  c->N = (uint32_t)n;
+  if (sv_status s = set_params(c, c->P, pos, ls, q, lg, sh)) return s;
+  clear_state(c);
+  c->step_t = 0;
+  c->slot_views.clear();
+  c->built = false;
+  c->forward_done = false;
+  return sync(c);
+}
+
+sv_status sv_get_gaussians(sv_ctx *c, float *pos, float *ls, float *q, float *lg, float *sh) {
+  if (sv_status s = check_ctx(c)) return s;
+  return get_params(c, c->P, pos, ls, q, lg, sh);
+}
+
+size_t sv_get_count(const sv_ctx *c) { return c ? c->N : 0; }
+
+sv_status sv_get_moments(sv_ctx *c, int which, float *pos, float *ls, float *q, float *lg, float *sh) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (which != 1 && which != 2) return fail(c, SV_E_ARG, "which must be 1 or 2");
+  return get_params(c, which == 1 ? c->M1 : c->M2, pos, ls, q, lg, sh);
+}
+
+sv_status sv_set_moments(sv_ctx *c, int which, const float *pos, const float *ls, const float *q,
+                         const float *lg, const float *sh) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (which != 1 && which != 2) return fail(c, SV_E_ARG, "which must be 1 or 2");
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  return set_params(c, which == 1 ? c->M1 : c->M2, pos, ls, q, lg, sh);
+}
+
+sv_status sv_get_step(const sv_ctx *c, uint32_t *t) {
+  if (!c || !t) return SV_E_ARG;
+  *t = c->step_t;
+  return SV_OK;
+}
+sv_status sv_set_step(sv_ctx *c, uint32_t t) {
+  if (!c) return SV_E_ARG;
+  c->step_t = t;
+  return SV_OK;
+}
+
+sv_status sv_set_views(sv_ctx *c, int nv, const sv_camera *cams) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (nv <= 0 || !cams) return fail(c, SV_E_ARG, "need at least one camera");
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  std::vector<HostView> vs(nv);
+  uint64_t off = 0;
+  const uint32_t ts = c->cfg.tile_size;
+  for (int i = 0; i < nv; ++i) {
+    HostView &h = vs[i];
+    h.cam = cams[i];
+    const double w = std::round((double)cams[i].viewport[0]), hh = std::round((double)cams[i].viewport[1]);
+    if (!(w >= 1.0) || !(hh >= 1.0) || w > 65535.0 * ts || hh > 65535.0 * ts)
+      return fail(c, SV_E_ARG, "camera %d has an invalid viewport", i);
+    h.W = (uint32_t)w;
+    h.H = (uint32_t)hh;
+    h.tiles_x = (h.W + ts - 1) / ts;
+    h.tiles_y = (h.H + ts - 1) / ts;
+    h.n_tiles = h.tiles_x * h.tiles_y;
+    h.pix_off = off;
+    h.has_target = false;
+    off += (uint64_t)h.W * h.H;
+  }
+  dfree(c->color);
+  dfree(c->alpha);
+  dfree(c->residual);
+  dfree(c->target);
+  dfree(c->last);
+  if (sv_status s = dalloc(c, &c->color, 3 * off)) return s;
+  if (sv_status s = dalloc(c, &c->residual, 3 * off)) return s;
+  if (sv_status s = dalloc(c, &c->target, 3 * off)) return s;
+  if (sv_status s = dalloc(c, &c->alpha, off)) return s;
+  if (sv_status s = dalloc(c, &c->last, off)) return s;
+  launch_fill_f32(c->st, c->target, 0.f, 3 * off);
+  launch_fill_f32(c->st, c->residual, 0.f, 3 * off);
+  launch_fill_f32(c->st, c->color, 0.f, 3 * off);
+  launch_fill_f32(c->st, c->alpha, 0.f, off);
+  launch_fill_u32(c->st, c->last, NO_LAST, off);
+  c->views.swap(vs);
+  c->pix_total = off;
+  c->residual_ready.assign(nv, 0);
+  c->slot_views.clear();
+  c->built = false;
+  c->forward_done = false;
+  return sync(c);
+}
+
+sv_status sv_set_target(sv_ctx *c, int v, const float *rgb) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!rgb) return fail(c, SV_E_ARG, "null target");
+  if (sv_status s = write_pixels3(c, c->target, v, rgb)) return s;
+  c->views[v].has_target = true;
+  return SV_OK;
+}
+
+sv_status sv_build_tiles(sv_ctx *c, int v, uint64_t *n_isect) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (sv_status s = prepare_batch(c, &v, 1)) return s;
+  if (sv_status s = build_batch(c)) return s;
+  if (n_isect) *n_isect = c->I;
+  return sync(c);
+}
+
+sv_status sv_forward(sv_ctx *c, int v) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (sv_status s = ensure_view_built(c, v)) return s;
+  if (sv_status s = forward_batch(c)) return s;
+  c->residual_ready[v] = 0;
+  return sync(c);
+}
+
+sv_status sv_loss(sv_ctx *c, int v) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!c->views[v].has_target) return fail(c, SV_E_STATE, "view %d has no target", v);
+  if (!(c->forward_done && c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "sv_forward(view %d) must run before sv_loss", v);
+  if (sv_status s = loss_views(c)) return s;
+  c->residual_ready[v] = 1;
+  return sync(c);
+}
+
+sv_status sv_set_residual(sv_ctx *c, int v, const float *rgb) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!rgb) return fail(c, SV_E_ARG, "null residual");
+  if (sv_status s = write_pixels3(c, c->residual, v, rgb)) return s;
+  c->residual_ready[v] = 1;
+  return SV_OK;
+}
+
+sv_status sv_backward(sv_ctx *c, int v) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!(c->forward_done && c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "sv_forward(view %d) must run before sv_backward", v);
+  const bool use_res = c->residual_ready[v] != 0;
+  if (!use_res && batch_needs_residual_buffer(c))
+    if (sv_status s = loss_views(c)) return s;
+  if (sv_status s = backward_batch(c, use_res || batch_needs_residual_buffer(c))) return s;
+  VjpArgs va = vjp_args(c);
+  {
+    Scoped t(c, PH_VJP);
+    launch_vjp(c->st, va);
+  }
+  if (sv_status s = check_launch(c, "vjp")) return s;
+  c->residual_ready[v] = 0;
+  return sync(c);
+}
+
+sv_status sv_backward_fixed(sv_ctx *c, int v, uint32_t flags) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!(c->forward_done && c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "sv_forward(view %d) must run before sv_backward_fixed", v);
+  if (!c->residual_ready[v]) {
+    if (!c->views[v].has_target) return fail(c, SV_E_STATE, "view %d has no target", v);
+    if (sv_status s = loss_views(c)) return s;
+  }
+  if (c->I > c->fixed_cap || !c->fixed) {
+    dfree(c->fixed);
+    c->fixed_cap = 0;
+    if (sv_status s = dalloc(c, &c->fixed, (size_t)std::max<uint64_t>(c->I, 1) * 54)) return s;
+    c->fixed_cap = std::max<uint64_t>(c->I, 1);
+  }
+  RasterArgs a = raster_args(c);
+  a.residual = c->residual;
+  launch_fill_f32(c->st, c->geom, 0.f, (size_t)5 * c->cap);
+  {
+    Scoped t(c, PH_FIXED);
+    launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed);
+    launch_reduce_fixed(c->st, c->fixed, c->vals, c->I, c->cap, c->G, c->geom);
+    launch_geom(c->st, c->P, c->cap, c->N, c->d_views, dev_config(c->cfg), c->geom, c->G, c->grad_accum);
+  }
+  if (sv_status s = check_launch(c, "backward_fixed")) return s;
+  c->residual_ready[v] = 0;
+  return sync(c);
+}
+
+sv_status sv_zero_grad(sv_ctx *c) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = zero_grads(c)) return s;
+  return sync(c);
+}
+
+sv_status sv_adam(sv_ctx *c) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = adam_planes(c)) return s;
+  return sync(c);
+}
+
+sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *stats) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!vs || nv <= 0) return fail(c, SV_E_ARG, "no views");
+  for (int i = 0; i < nv; ++i) {
+    if (sv_status s = check_view(c, vs[i])) return s;
+    if (!c->views[vs[i]].has_target) return fail(c, SV_E_STATE, "view %d has no target", vs[i]);
+  }
+  uint64_t total_isect = 0;
+  double l1 = 0.0;
+  sv_status s;
+  if (mode == SV_MODE_PLAIN) {
+    if ((s = zero_grads(c))) return s;
+    for (int i = 0; i < nv; ++i) {
+      const int v = vs[i];
+      if ((s = prepare_batch(c, &v, 1))) return s;
+      if ((s = build_batch(c))) return s;
+      total_isect += c->I;
+      if ((s = forward_batch(c))) return s;
+      const bool use_res = batch_needs_residual_buffer(c);
+      if (use_res && (s = loss_views(c))) return s;
+      if ((s = backward_batch(c, use_res))) return s;
+      VjpArgs va = vjp_args(c);
+      {
+        Scoped t(c, PH_VJP);
+        launch_vjp(c->st, va);
+      }
+      if ((s = check_launch(c, "vjp"))) return s;
+      if (stats) l1 += sum_l1(c);
+    }
+    if ((s = adam_planes(c))) return s;
+  } else if (mode == SV_MODE_BATCHED) {
+    if ((s = step_batched(c, vs, nv))) return s;
+    total_isect = c->I;
+    if (stats) l1 = sum_l1(c);
+  } else {
+    return fail(c, SV_E_ARG, "unknown mode %d", mode);
+  }
+  if ((s = sync(c))) return s;
+  if (stats) {
+    stats->n_isect = total_isect;
+    stats->n_views = (uint32_t)nv;
+    stats->sort_passes = (uint32_t)c->passes;
+    stats->key_bits = c->key_bits;
+    stats->pad = 0;
+    stats->l1_sum = l1;
+  }
+  return SV_OK;
+}
+
+sv_status sv_render_batched(sv_ctx *c, const int *vs, int nv) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = prepare_batch(c, vs, nv)) return s;
+  if (sv_status s = build_batch(c)) return s;
+  if (sv_status s = forward_batch(c)) return s;
+  return sync(c);
+}
+
+sv_status sv_read_image(sv_ctx *c, int v, float *rgb) {
+  if (sv_status s = check_ctx(c)) return s;
+  return read_pixels3(c, c->color, v, rgb);
+}
+
+sv_status sv_read_residual(sv_ctx *c, int v, float *rgb) {
+  if (sv_status s = check_ctx(c)) return s;
+  return read_pixels3(c, c->residual, v, rgb);
+}
+
+sv_status sv_read_alpha(sv_ctx *c, int v, float *out) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!out) return fail(c, SV_E_ARG, "null output");
+  const HostView &hv = c->views[v];
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  HIP_OK(c, hipMemcpy(out, c->alpha + hv.pix_off, (size_t)hv.W * hv.H * sizeof(float), hipMemcpyDeviceToHost));
+  return SV_OK;
+}
+
+sv_status sv_read_last_ids(sv_ctx *c, int v, uint32_t *out) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!out) return fail(c, SV_E_ARG, "null output");
+  const HostView &hv = c->views[v];
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  HIP_OK(c, hipMemcpy(out, c->last + hv.pix_off, (size_t)hv.W * hv.H * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return SV_OK;
+}
+
+sv_status sv_read_keys(sv_ctx *c, int v, uint32_t *keys, uint32_t *vals) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!(c->built && c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "tiles of view %d are not the last build", v);
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  const uint64_t I = c->I;
+  if (vals && I) HIP_OK(c, hipMemcpy(vals, c->vals, I * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (keys && I) {
+    const uint64_t dmask = c->dbits ? ((1ull << c->dbits) - 1ull) : 0ull;
+    if (c->key64) {
+      std::vector<uint64_t> k(I);
+      HIP_OK(c, hipMemcpy(k.data(), c->keys, I * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      for (uint64_t i = 0; i < I; ++i)
+        keys[i] = (uint32_t)((((k[i] >> c->dbits) & 0xfffu) << 16) | (k[i] & dmask));
+    } else {
+      std::vector<uint32_t> k(I);
+      HIP_OK(c, hipMemcpy(k.data(), c->keys, I * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint64_t i = 0; i < I; ++i)
+        keys[i] = (uint32_t)(((((uint64_t)k[i] >> c->dbits) & 0xfffu) << 16) | (k[i] & dmask));
+    }
+  }
+  return SV_OK;
+}
+
+sv_status sv_read_tile_offsets(sv_ctx *c, int v, uint32_t *out) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!out) return fail(c, SV_E_ARG, "null output");
+  if (!(c->built && c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "tiles of view %d are not the last build", v);
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  HIP_OK(c, hipMemcpy(out, c->ranges, ((size_t)c->tiles_total + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return SV_OK;
+}
+
+sv_status sv_read_projections(sv_ctx *c, int v, float *out) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!out) return fail(c, SV_E_ARG, "null output");
+  if (!(c->built && c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "tiles of view %d are not the last build", v);
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  const size_t N = c->N, rs = rec_stride(c);
+  std::vector<float4> r(4 * N);
+  for (int k = 0; k < 4; ++k)
+    HIP_OK(c, hipMemcpy(r.data() + k * N, c->rec + k * rs, N * sizeof(float4), hipMemcpyDeviceToHost));
+  std::vector<float> pos(3 * N), lg(N);
+  if (sv_status s = get_params(c, c->P, pos.data(), nullptr, nullptr, lg.data(), nullptr)) return s;
+  for (size_t g = 0; g < N; ++g) {
+    const float4 r0 = r[g], r1 = r[N + g], r2 = r[2 * N + g], r3 = r[3 * N + g];
+    float *o = out + 12 * g;
+    if (r2.y == 0.0f) {  // culled: the record is all zero
+      for (int k = 0; k < 12; ++k) o[k] = 0.0f;
+      continue;
+    }
+    o[0] = r0.x;
+    o[1] = r0.y;
+    o[2] = r2.y;
+    o[3] = r2.z;
+    o[4] = r0.z;
+    o[5] = r0.w;
+    o[6] = r1.x;
+    std::memcpy(&o[7], &r3.z, 4);
+    o[8] = pos[3 * g];
+    o[9] = pos[3 * g + 1];
+    o[10] = pos[3 * g + 2];
+    o[11] = lg[g];
+  }
+  return SV_OK;
+}
+
+sv_status sv_read_grads(sv_ctx *c, float *pos, float *ls, float *q, float *lg, float *sh, float *geom) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = get_params(c, c->G, pos, ls, q, lg, sh)) return s;
+  if (geom) {
+    std::vector<float> t(c->N);
+    for (int k = 0; k < 5; ++k) {
+      HIP_OK(c, hipMemcpy(t.data(), c->geom + (size_t)k * c->cap, c->N * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t g = 0; g < c->N; ++g) geom[5 * g + k] = t[g];
+    }
+  }
+  return SV_OK;
+}
+
+sv_status sv_read_tile_grads_fixed(sv_ctx *c, int v, int32_t *out) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!out) return fail(c, SV_E_ARG, "null output");
+  if (!c->fixed || !(c->slot_views.size() == 1 && c->slot_views[0] == v))
+    return fail(c, SV_E_STATE, "no fixed-point backward for view %d", v);
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  if (c->I) HIP_OK(c, hipMemcpy(out, c->fixed, c->I * 54 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return SV_OK;
+}
+
+sv_status sv_read_stats(sv_ctx *c, float *imp, float *acc) {
+  if (sv_status s = check_ctx(c)) return s;
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  if (imp) HIP_OK(c, hipMemcpy(imp, c->importance, c->N * sizeof(float4), hipMemcpyDeviceToHost));
+  if (acc) HIP_OK(c, hipMemcpy(acc, c->grad_accum, c->N * sizeof(float4), hipMemcpyDeviceToHost));
+  return SV_OK;
+}
+
+sv_status sv_prune_mask(sv_ctx *c, const uint8_t *remove, size_t *n_removed) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!remove) return fail(c, SV_E_ARG, "null mask");
+  if (c->N == 0) {
+    if (n_removed) *n_removed = 0;
+    return SV_OK;
+  }
+  uint8_t *rm = nullptr, *keep8 = nullptr;
+  uint32_t *keep32 = nullptr;
+  if (sv_status s = dalloc(c, &rm, c->N)) return s;
+  if (sv_status s = dalloc(c, &keep8, c->N)) return s;
+  if (sv_status s = dalloc(c, &keep32, c->N)) return s;
+  HIP_OK(c, hipMemcpy(rm, remove, c->N, hipMemcpyHostToDevice));
+  launch_u8_to_u32(c->st, rm, keep32, c->N, 1);
+  std::vector<uint8_t> keep(c->N);
+  for (size_t i = 0; i < c->N; ++i) keep[i] = remove[i] ? 0 : 1;
+  HIP_OK(c, hipMemcpy(keep8, keep.data(), c->N, hipMemcpyHostToDevice));
+  sv_status s = compact_by_keep(c, keep8, keep32, n_removed);
+  dfree(rm);
+  dfree(keep8);
+  dfree(keep32);
+  return s;
+}
+
+sv_status sv_prune_opacity(sv_ctx *c, float min_opacity, size_t *n_removed) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (c->N == 0) {
+    if (n_removed) *n_removed = 0;
+    return SV_OK;
+  }
+  uint8_t *keep8 = nullptr;
+  uint32_t *keep32 = nullptr;
+  if (sv_status s = dalloc(c, &keep8, c->N)) return s;
+  if (sv_status s = dalloc(c, &keep32, c->N)) return s;
+  launch_prune_flags(c->st, c->P, c->cap, c->N, min_opacity, keep8, keep32);
+  sv_status s = compact_by_keep(c, keep8, keep32, n_removed);
+  dfree(keep8);
+  dfree(keep32);
+  return s;
+}
+
+sv_status sv_densify(sv_ctx *c, const uint8_t *actions, uint64_t seed, size_t *n_added) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!actions) return fail(c, SV_E_ARG, "null actions");
+  const uint32_t N = c->N;
+  if (N == 0) {
+    if (n_added) *n_added = 0;
+    return SV_OK;
+  }
+  size_t n_clone = 0, n_split = 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    if (actions[i] > 2) return fail(c, SV_E_ARG, "action %u at %u is not 0, 1 or 2", actions[i], i);
+    n_clone += actions[i] == 1;
+    n_split += actions[i] == 2;
+  }
+  const size_t n_new = (size_t)N - n_split + n_clone + 2 * n_split;
+  if (n_added) *n_added = n_new - N;
+  if (n_clone == 0 && n_split == 0) return SV_OK;
+  size_t cap_n = c->cap;
+  if (n_new > cap_n) {
+    // ensureCapacity (gaussian-renderer.ts:2044-2064): next power of two
+    cap_n = 1;
+    while (cap_n < n_new) cap_n <<= 1;
+  }
+  uint8_t *act = nullptr;
+  uint32_t *flags = nullptr, *keep_offs = nullptr, *child_offs = nullptr;
+  uint64_t *tmp = nullptr;
+  if (sv_status s = dalloc(c, &act, N)) return s;
+  if (sv_status s = dalloc(c, &flags, N)) return s;
+  if (sv_status s = dalloc(c, &keep_offs, N)) return s;
+  if (sv_status s = dalloc(c, &child_offs, N)) return s;
+  if (sv_status s = dalloc(c, &tmp, scan_tmp_words(N))) return s;
+  HIP_OK(c, hipMemcpy(act, actions, N, hipMemcpyHostToDevice));
+  launch_u8_to_u32(c->st, act, flags, N, 2);
+  launch_scan(c->st, flags, keep_offs, N, tmp, c->d_small);
+  launch_u8_to_u32(c->st, act, flags, N, 3);
+  launch_scan(c->st, flags, child_offs, N, tmp, c->d_small + 1);
+  float *Pn = nullptr, *M1n = nullptr, *M2n = nullptr;
+  if (sv_status s = dalloc(c, &Pn, (size_t)P_NUM * cap_n)) return s;
+  if (sv_status s = dalloc(c, &M1n, (size_t)P_NUM * cap_n)) return s;
+  if (sv_status s = dalloc(c, &M2n, (size_t)P_NUM * cap_n)) return s;
+  {
+    Scoped t(c, PH_COMPACT);
+    launch_densify(c->st, c->P, c->M1, c->M2, c->cap, N, act, keep_offs, child_offs,
+                   (uint32_t)(N - n_split), seed, Pn, M1n, M2n, cap_n);
+  }
+  if (sv_status s = check_launch(c, "densify")) return s;
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  dfree(act);
+  dfree(flags);
+  dfree(keep_offs);
+  dfree(child_offs);
+  dfree(tmp);
+  return swap_in(c, Pn, M1n, M2n, cap_n, (uint32_t)n_new);
+}
+
+sv_status sv_enable_timing(sv_ctx *c, int enable) {
+  if (!c) return SV_E_ARG;
+  c->timing = enable != 0;
+  return SV_OK;
+}
+
+sv_status sv_timings(sv_ctx *c, sv_kernel_time *out, int *count) {
+  if (!c || !count) return SV_E_ARG;
+  if (sv_status s = sync(c)) return s;
+  const int n = std::min(*count, kNumPhases);
+  for (int i = 0; i < n && out; ++i) {
+    std::memset(out[i].name, 0, sizeof(out[i].name));
+    std::snprintf(out[i].name, sizeof(out[i].name), "%s", kPhaseNames[i]);
+    out[i].total_ms = c->phase_ms[i];
+    out[i].launches = c->phase_n[i];
+  }
+  *count = kNumPhases;
+  return SV_OK;
+}
+
+sv_status sv_reset_timings(sv_ctx *c) {
+  if (!c) return SV_E_ARG;
+  if (sv_status s = sync(c)) return s;
+  for (int i = 0; i < kNumPhases; ++i) {
+    c->phase_ms[i] = 0.0;
+    c->phase_n[i] = 0;
+  }
+  return SV_OK;
+}
+
+sv_status sv_sync(sv_ctx *c) {
+  if (!c) return SV_E_ARG;
+  return sync(c);
+}
+
+}  // extern "C"

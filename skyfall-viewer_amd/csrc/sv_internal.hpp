@@ -1,0 +1,128 @@
+// sv_internal.hpp -- launch wrappers exported by the .hip translation units
+// to the context code in sv_api.cpp. All launches go to the given stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sv_device.hpp"
+
+namespace sv {
+
+// sv_project.hip --------------------------------------------------------
+// Projection + SH colour + tile count for nv views per Gaussian
+// (project_gaussians_for_tiling, training-tiles.wgsl:39-180, with the SH
+// colour of training-forward-tiles.wgsl:65-105 evaluated once per view).
+// rec: 4 float4 per (slot, Gaussian); counts: u32 per (slot, Gaussian).
+void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
+                    int nv, const DevConfig &cfg, float4 *rec, size_t rs, uint32_t *counts,
+                    uint32_t *max_dq);
+// Exclusive scan of u32 counts (M entries) into u32 offsets; total (u64)
+// written to *total. tmp must hold scan_tmp_words(M) u64 words.
+size_t scan_tmp_words(size_t M);
+void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t M, uint64_t *tmp,
+                 uint64_t *total);
+// emit_intersections (training-tiles.wgsl:201-247) for nv slots.
+template <typename K>
+void launch_emit(hipStream_t st, const float4 *rec, size_t rs, const uint32_t *counts,
+                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits,
+                 K *keys, uint32_t *vals);
+// Tile ranges = lower_bound per global tile (build_tile_offsets +
+// host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
+template <typename K>
+void launch_ranges(hipStream_t st, const K *keys, uint64_t I, uint32_t dbits, uint32_t n_tiles,
+                   uint32_t *ranges);
+// Residual of training_loss (training-loss.wgsl:86-126) for one slot.
+void launch_loss(hipStream_t st, const DevView &hv, const DevConfig &cfg, const float *color,
+                 const float *target, float *residual);
+
+// sv_sort.hip -----------------------------------------------------------
+// Stable LSD radix sort of (key, value) pairs over bits [0, bits).
+// Result ends in (keys, vals); (keys2, vals2) is scratch of the same size.
+template <typename K>
+size_t sort_tmp_words(uint64_t n);
+template <typename K>
+void launch_sort(hipStream_t st, K *keys, uint32_t *vals, K *keys2, uint32_t *vals2, uint64_t n,
+                 uint32_t bits, uint32_t *tmp, int *passes_out);
+
+// sv_raster.hip ---------------------------------------------------------
+struct RasterArgs {
+  const DevView *views;
+  const uint32_t *tile_view;  // global tile -> slot
+  const uint32_t *ranges;     // n_tiles_total + 1
+  const uint32_t *vals;       // sorted Gaussian ids
+  uint64_t n_isect;           // intersections (bound of vals / adjoint records)
+  const float4 *rec;          // per (slot, Gaussian) projection records, 4 planes
+  size_t rs;                  // record plane stride (float4 units)
+  uint32_t N;                 // Gaussians (record stride per slot)
+  uint32_t n_tiles_total;
+  DevConfig cfg;
+  // forward outputs / backward inputs (pixel arrays of the batch)
+  float *color;     // 3 planes per view
+  float *alpha;
+  uint32_t *last;
+  const float *target;    // 3 planes per view
+  const float *residual;  // 3 planes per view, or null for the fused loss
+  // backward outputs
+  const uint32_t *offs;   // per (slot, Gaussian) first intersection (emit order)
+  float *isect_rec;       // 9 floats per intersection (emit order)
+  float *l1_partial;      // per-tile sum |pred - target| (may be null)
+  uint32_t *work_counter; // persistent queue counter (null: one block per tile)
+};
+void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t raster_px,
+                    uint32_t grid_blocks);
+void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
+// backward_tiles_local (backward-tiles-local.wgsl:185-382) for one slot:
+// i32 sums per (intersection, slot of 54).
+void launch_backward_fixed(hipStream_t st, const RasterArgs &a, const float *P, size_t cap,
+                           uint32_t flags, int32_t *fixed);
+
+// sv_optim.hip ----------------------------------------------------------
+struct VjpArgs {
+  const float *P;  // parameter planes
+  size_t cap;
+  uint32_t N;
+  const DevView *views;
+  int nv;
+  DevConfig cfg;
+  const uint32_t *counts;  // per (slot, Gaussian)
+  const uint32_t *offs;
+  const float4 *rec;       // projection records (radius for the importance stats)
+  size_t rs;
+  const float *isect_rec;  // 9 floats per intersection
+  float4 *importance;      // importance statistics (may be null)
+  float *G;                // gradient planes (accumulate), or null when fused
+  float *geom;             // 5 planes: grad_geom of the last slot (may be null)
+  float4 *grad_accum;      // densify statistics (may be null)
+  // fused Adam (when G == null)
+  float *Pw;               // parameters to update (== P)
+  float *M1, *M2;
+  float lr[5];             // pos, scale, quat, logit, sh
+  float beta1, beta2, eps, bc1_rcp, bc2_sqrt_rcp;
+  int update_sh;
+};
+void launch_vjp(hipStream_t st, const VjpArgs &a);
+void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G, size_t cap,
+                 uint32_t N, const float lr[5], float beta1, float beta2, float eps, float bc1_rcp,
+                 float bc2_sqrt_rcp, int update_sh);
+// backward_reduce (backward-reduce.wgsl:35-71) + geometry backward input.
+void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals, uint64_t I,
+                         size_t cap, float *G, float *geom);
+// training_backward_geom (training-backward-geom.wgsl:36-299) from the
+// per-Gaussian grad_geom planes of one view.
+void launch_geom(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *view,
+                 const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum);
+// Stable compaction of rows by keep flags (exclusive scan of flags).
+void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src, size_t cap_dst,
+                    int planes, const uint32_t *keep_offs, const uint8_t *keep, uint32_t N);
+void launch_prune_flags(hipStream_t st, const float *P, size_t cap, uint32_t N, float min_opacity,
+                        uint8_t *keep, uint32_t *keep_u32);
+void launch_densify(hipStream_t st, const float *P, const float *M1, const float *M2, size_t cap,
+                    uint32_t N, const uint8_t *actions, const uint32_t *keep_offs,
+                    const uint32_t *child_offs, uint32_t n_keep, uint64_t seed, float *Pn,
+                    float *M1n, float *M2n, size_t cap_n);
+void launch_u8_to_u32(hipStream_t st, const uint8_t *a, uint32_t *o, uint32_t N, int mode);
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
+void launch_fill_f32(hipStream_t st, float *p, float v, size_t n);
+
+}  // namespace sv

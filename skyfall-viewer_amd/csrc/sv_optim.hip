@@ -1,0 +1,611 @@
+// sv_optim.hip -- per-Gaussian backward (SH chain + geometry VJP), Adam,
+// the fixed-point reduce, and prune/densify compaction (gfx950).
+//
+// All per-Gaussian kernels use one thread per Gaussian over the planar
+// parameter layout (plane k of parameter k, stride cap), so every load and
+// store of a wave is one contiguous 256-byte segment.
+#include "sv_internal.hpp"
+
+namespace sv {
+
+// View-independent part of training_backward_geom
+// (training-backward-geom.wgsl:127-145).
+struct GeomPre {
+  float wx, wy, wz;
+  float sigma[3];
+  float q[4];  // normalised (x, y, z, w)
+  M3 R, M, cov3D;
+};
+
+__device__ __forceinline__ GeomPre geom_pre(const float prm[11]) {
+  GeomPre g;
+  g.wx = prm[P_POS];
+  g.wy = prm[P_POS + 1];
+  g.wz = prm[P_POS + 2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g.sigma[k] = sv_expf(clampf(prm[P_SCALE + k], -10.0f, 10.0f));
+  const float d = ((prm[P_QUAT] * prm[P_QUAT] + prm[P_QUAT + 1] * prm[P_QUAT + 1]) +
+                   prm[P_QUAT + 2] * prm[P_QUAT + 2]) +
+                  prm[P_QUAT + 3] * prm[P_QUAT + 3];
+  const float l = sqrtf(d);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) g.q[k] = prm[P_QUAT + k] / l;
+  g.R = quat_to_mat3(g.q[0], g.q[1], g.q[2], g.q[3]);
+  M3 S;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) S.m[i][j] = (i == j) ? g.sigma[i] : 0.0f;
+  g.M = mul(S, g.R);
+  g.cov3D = mul(transpose(g.M), g.M);
+  return g;
+}
+
+// training_backward_geom (training-backward-geom.wgsl:36-299) for one
+// view: dL/d(conic a,b,c; mean x,y) -> dL/d(position, log-scale,
+// quaternion). Operation order is that of the WGSL source (frustum clamp
+// 1.3 tan_fov, EWA covariance with EPS_2D = 0.3, 3DGS conic VJP, M = S R,
+// cov3D = M^T M, the quaternion formula on the normalised q, the mean path
+// with +fy, dL/dlog(sigma) = dL/dsigma * sigma). Returns false where the
+// shader returns early.
+__device__ __forceinline__ bool geom_vjp(const DevView &V, const GeomPre &G, float gA, float gB,
+                                         float gC, float gmx, float gmy, float dpos[3],
+                                         float dls[3], float dq[4]) {
+  const float conic_mag = (fabsf(gA) + fabsf(gB)) + fabsf(gC);
+  const float mean_mag = fabsf(gmx) + fabsf(gmy);
+  if (conic_mag < 1e-10f && mean_mag < 1e-10f) return false;
+  float t4[4];
+  m4_mul_pos(V.view, G.wx, G.wy, G.wz, t4);
+  const float tx = t4[0], ty = t4[1], tz = t4[2];
+  if (tz <= 0.0f) return false;
+  const float fx = V.focal[0], fy = V.focal[1];
+  const float tan_fovx = V.viewport[0] / (2.0f * fx);
+  const float tan_fovy = V.viewport[1] / (2.0f * fy);
+  const float limx = 1.3f * tan_fovx, limy = 1.3f * tan_fovy;
+  const float txtz = tx / tz, tytz = ty / tz;
+  const float tcx = clampf(txtz, -limx, limx) * tz;
+  const float tcy = clampf(tytz, -limy, limy) * tz;
+  const float tcz = tz;
+  const float xgm = (txtz >= -limx && txtz <= limx) ? 1.0f : 0.0f;
+  const float ygm = (tytz >= -limy && tytz <= limy) ? 1.0f : 0.0f;
+  const float tz_inv = 1.0f / tcz;
+  const float tz2_inv = tz_inv * tz_inv;
+  M3 J;
+  J.m[0][0] = fx * tz_inv;
+  J.m[0][1] = 0.0f;
+  J.m[0][2] = ((-fx) * tcx) * tz2_inv;
+  J.m[1][0] = 0.0f;
+  J.m[1][1] = fy * tz_inv;
+  J.m[1][2] = ((-fy) * tcy) * tz2_inv;
+  J.m[2][0] = 0.0f;
+  J.m[2][1] = 0.0f;
+  J.m[2][2] = 0.0f;
+  M3 W;  // mat3x3(view[0].x, view[1].x, view[2].x, view[0].y, ...)
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) W.m[c][r] = V.view[r * 4 + c];
+  const M3 Tm = mul(W, J);
+  const M3 cov2D = mul(mul(transpose(Tm), transpose(G.cov3D)), Tm);
+  const float a = cov2D.m[0][0] + 0.3f;
+  const float b = cov2D.m[0][1];
+  const float c = cov2D.m[1][1] + 0.3f;
+  const float det = a * c - b * b;
+  if (det <= 0.0f) return false;
+  // STEP 1
+  const float d2i = 1.0f / (det * det + 1e-7f);
+  const float dL_da = d2i * ((((-c) * c) * gA + ((2.0f * b) * c) * gB) + (det - a * c) * gC);
+  const float dL_dc = d2i * ((((-a) * a) * gC + ((2.0f * a) * b) * gB) + (det - a * c) * gA);
+  const float dL_db = (d2i * 2.0f) * ((((b * c) * gA) - (det + (2.0f * b) * b) * gB) + (a * b) * gC);
+  // STEP 2
+#define TM(i, j) Tm.m[i][j]
+  const float d00 = ((TM(0, 0) * TM(0, 0)) * dL_da + (TM(0, 0) * TM(1, 0)) * dL_db) + (TM(1, 0) * TM(1, 0)) * dL_dc;
+  const float d11 = ((TM(0, 1) * TM(0, 1)) * dL_da + (TM(0, 1) * TM(1, 1)) * dL_db) + (TM(1, 1) * TM(1, 1)) * dL_dc;
+  const float d22 = ((TM(0, 2) * TM(0, 2)) * dL_da + (TM(0, 2) * TM(1, 2)) * dL_db) + (TM(1, 2) * TM(1, 2)) * dL_dc;
+  const float d01 = (((2.0f * TM(0, 0)) * TM(0, 1)) * dL_da + (TM(0, 0) * TM(1, 1) + TM(0, 1) * TM(1, 0)) * dL_db) +
+                    ((2.0f * TM(1, 0)) * TM(1, 1)) * dL_dc;
+  const float d02 = (((2.0f * TM(0, 0)) * TM(0, 2)) * dL_da + (TM(0, 0) * TM(1, 2) + TM(0, 2) * TM(1, 0)) * dL_db) +
+                    ((2.0f * TM(1, 0)) * TM(1, 2)) * dL_dc;
+  const float d12 = (((2.0f * TM(0, 2)) * TM(0, 1)) * dL_da + (TM(0, 1) * TM(1, 2) + TM(0, 2) * TM(1, 1)) * dL_db) +
+                    ((2.0f * TM(1, 1)) * TM(1, 2)) * dL_dc;
+  M3 dS;
+  dS.m[0][0] = d00;
+  dS.m[0][1] = 0.5f * d01;
+  dS.m[0][2] = 0.5f * d02;
+  dS.m[1][0] = 0.5f * d01;
+  dS.m[1][1] = d11;
+  dS.m[1][2] = 0.5f * d12;
+  dS.m[2][0] = 0.5f * d02;
+  dS.m[2][1] = 0.5f * d12;
+  dS.m[2][2] = d22;
+  // STEP 3
+  M3 M2;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) M2.m[i][j] = 2.0f * G.M.m[i][j];
+  const M3 dM = mul(M2, dS);
+  const M3 dMt = transpose(dM);
+  const M3 Rt = transpose(G.R);
+  // STEP 4
+  const float ds0 = dot3(Rt.m[0][0], Rt.m[0][1], Rt.m[0][2], dMt.m[0][0], dMt.m[0][1], dMt.m[0][2]);
+  const float ds1 = dot3(Rt.m[1][0], Rt.m[1][1], Rt.m[1][2], dMt.m[1][0], dMt.m[1][1], dMt.m[1][2]);
+  const float ds2 = dot3(Rt.m[2][0], Rt.m[2][1], Rt.m[2][2], dMt.m[2][0], dMt.m[2][1], dMt.m[2][2]);
+  // STEP 5
+  M3 Ms = dMt;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    Ms.m[0][j] = Ms.m[0][j] * G.sigma[0];
+    Ms.m[1][j] = Ms.m[1][j] * G.sigma[1];
+    Ms.m[2][j] = Ms.m[2][j] * G.sigma[2];
+  }
+  const float r = G.q[3], x = G.q[0], y = G.q[1], z = G.q[2];
+  dq[3] = ((2.0f * z) * (Ms.m[0][1] - Ms.m[1][0]) + (2.0f * y) * (Ms.m[2][0] - Ms.m[0][2])) +
+          (2.0f * x) * (Ms.m[1][2] - Ms.m[2][1]);
+  dq[0] = (((2.0f * y) * (Ms.m[1][0] + Ms.m[0][1]) + (2.0f * z) * (Ms.m[2][0] + Ms.m[0][2])) +
+           (2.0f * r) * (Ms.m[1][2] - Ms.m[2][1])) -
+          (4.0f * x) * (Ms.m[2][2] + Ms.m[1][1]);
+  dq[1] = (((2.0f * x) * (Ms.m[1][0] + Ms.m[0][1]) + (2.0f * r) * (Ms.m[2][0] - Ms.m[0][2])) +
+           (2.0f * z) * (Ms.m[1][2] + Ms.m[2][1])) -
+          (4.0f * y) * (Ms.m[2][2] + Ms.m[0][0]);
+  dq[2] = (((2.0f * r) * (Ms.m[0][1] - Ms.m[1][0]) + (2.0f * x) * (Ms.m[2][0] + Ms.m[0][2])) +
+           (2.0f * y) * (Ms.m[1][2] + Ms.m[2][1])) -
+          (4.0f * z) * (Ms.m[1][1] + Ms.m[0][0]);
+  // STEP 6
+#define C3(i, j) G.cov3D.m[i][j]
+  const float dT00 = ((2.0f * ((TM(0, 0) * C3(0, 0) + TM(0, 1) * C3(0, 1)) + TM(0, 2) * C3(0, 2))) * dL_da) +
+                     (((TM(1, 0) * C3(0, 0) + TM(1, 1) * C3(0, 1)) + TM(1, 2) * C3(0, 2)) * dL_db);
+  const float dT01 = ((2.0f * ((TM(0, 0) * C3(1, 0) + TM(0, 1) * C3(1, 1)) + TM(0, 2) * C3(1, 2))) * dL_da) +
+                     (((TM(1, 0) * C3(1, 0) + TM(1, 1) * C3(1, 1)) + TM(1, 2) * C3(1, 2)) * dL_db);
+  const float dT02 = ((2.0f * ((TM(0, 0) * C3(2, 0) + TM(0, 1) * C3(2, 1)) + TM(0, 2) * C3(2, 2))) * dL_da) +
+                     (((TM(1, 0) * C3(2, 0) + TM(1, 1) * C3(2, 1)) + TM(1, 2) * C3(2, 2)) * dL_db);
+  const float dT10 = ((2.0f * ((TM(1, 0) * C3(0, 0) + TM(1, 1) * C3(0, 1)) + TM(1, 2) * C3(0, 2))) * dL_dc) +
+                     (((TM(0, 0) * C3(0, 0) + TM(0, 1) * C3(0, 1)) + TM(0, 2) * C3(0, 2)) * dL_db);
+  const float dT11 = ((2.0f * ((TM(1, 0) * C3(1, 0) + TM(1, 1) * C3(1, 1)) + TM(1, 2) * C3(1, 2))) * dL_dc) +
+                     (((TM(0, 0) * C3(1, 0) + TM(0, 1) * C3(1, 1)) + TM(0, 2) * C3(1, 2)) * dL_db);
+  const float dT12 = ((2.0f * ((TM(1, 0) * C3(2, 0) + TM(1, 1) * C3(2, 1)) + TM(1, 2) * C3(2, 2))) * dL_dc) +
+                     (((TM(0, 0) * C3(2, 0) + TM(0, 1) * C3(2, 1)) + TM(0, 2) * C3(2, 2)) * dL_db);
+#undef C3
+#undef TM
+  const float dJ00 = (W.m[0][0] * dT00 + W.m[0][1] * dT01) + W.m[0][2] * dT02;
+  const float dJ02 = (W.m[2][0] * dT00 + W.m[2][1] * dT01) + W.m[2][2] * dT02;
+  const float dJ11 = (W.m[1][0] * dT10 + W.m[1][1] * dT11) + W.m[1][2] * dT12;
+  const float dJ12 = (W.m[2][0] * dT10 + W.m[2][1] * dT11) + W.m[2][2] * dT12;
+  const float tz3_inv = tz2_inv * tz_inv;
+  const float dtx_cov = xgm * (((-fx) * tz2_inv) * dJ02);
+  const float dty_cov = ygm * (((-fy) * tz2_inv) * dJ12);
+  const float dtz_cov = ((((-fx) * tz2_inv) * dJ00 - (fy * tz2_inv) * dJ11) + (((2.0f * fx) * tcx) * tz3_inv) * dJ02) +
+                        (((2.0f * fy) * tcy) * tz3_inv) * dJ12;
+  // STEP 7
+  const float dtx_mean = (gmx * fx) * tz_inv;
+  const float dty_mean = (gmy * fy) * tz_inv;
+  const float dtz_mean = ((((-gmx) * fx) * tx) * tz2_inv) - (((gmy * fy) * ty) * tz2_inv);
+  const float l0 = dtx_cov + dtx_mean, l1 = dty_cov + dty_mean, l2 = dtz_cov + dtz_mean;
+  const M3 Wt = transpose(W);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) dpos[j] = (Wt.m[0][j] * l0 + Wt.m[1][j] * l1) + Wt.m[2][j] * l2;
+  // STEP 8
+  dls[0] = ds0 * G.sigma[0];
+  dls[1] = ds1 * G.sigma[1];
+  dls[2] = ds2 * G.sigma[2];
+  return true;
+}
+
+// accumulate_grads (densify-accum.wgsl:19-46).
+__device__ __forceinline__ void densify_accum(float4 &acc, float gA, float gB, float gC, float gmx,
+                                              float gmy) {
+  const float n2 = sqrtf(gmx * gmx + gmy * gmy);
+  const float na = sqrtf((gA * gA + gB * gB) + gC * gC);
+  if (n2 > 1e-10f || na > 1e-10f) {
+    acc.x = acc.x + n2;
+    acc.y = 0.0f;
+    acc.z = acc.z + na;
+    acc.w = acc.w + 1.0f;
+  }
+}
+
+// adam_step (adam.wgsl:18-38) on one element.
+__device__ __forceinline__ void adam_elem(float &p, float &m1, float &m2, float g, float lr, float b1,
+                                          float b2, float eps, float bc1, float bc2s) {
+  const float a = b1 * m1 + (1.0f - b1) * g;
+  const float b = b2 * m2 + ((1.0f - b2) * g) * g;
+  const float denom = sqrtf(b) * bc2s + eps;
+  const float step = lr * bc1;
+  p = p - step * (a / denom);
+  m1 = a;
+  m2 = b;
+}
+
+__device__ __forceinline__ int tensor_of_plane(int k) {
+  return k < P_SCALE ? 0 : (k < P_QUAT ? 1 : (k < P_LOGIT ? 2 : (k < P_SH ? 3 : 4)));
+}
+
+// ---------------------------------------------------------------------
+// Per-Gaussian backward over the views of a step: sum the Gaussian's
+// per-tile adjoint records of each view (emit order, fixed), apply the SH
+// chain (grad_sh[k][c] = Y_k(dir_view) * sum res*alpha*T), the opacity
+// logit gradient and the geometry VJP, then either add into the gradient
+// planes (plain path) or run Adam on all 59 parameters in place (batched
+// path; every Gaussian is updated, as the reference's Adam dispatches
+// cover all active elements).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.N) return;
+  float prm[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) {
+    const float v = a.P[(size_t)k * a.cap + g];
+    prm[k] = a.cfg.render_f16 ? f16_round(v) : v;
+  }
+  const GeomPre pre = geom_pre(prm);
+  float gsh[48];
+#pragma unroll
+  for (int k = 0; k < 48; ++k) gsh[k] = 0.0f;
+  float glogit = 0.0f;
+  float gp[3] = {0.f, 0.f, 0.f}, gs[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f};
+  bool any = false;
+  float4 dacc = a.grad_accum ? a.grad_accum[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float geo[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float4 imp = a.importance ? a.importance[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int s = 0; s < a.nv; ++s) {
+    const size_t vg = (size_t)s * a.N + g;
+    const uint32_t cnt = a.counts[vg];
+    // accumulate_importance (importance-accum.wgsl:19-40), once per view
+    imp.x = imp.x * a.cfg.importance_decay;
+    imp.y = imp.y * a.cfg.importance_decay;
+    imp.z = imp.z * a.cfg.importance_decay;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) geo[q] = 0.0f;
+    if (cnt == 0u) continue;
+    imp.x = imp.x + (float)cnt;
+    imp.y = imp.y + 1.0f;
+    imp.z = fmaxf(imp.z, a.rec[2 * a.rs + vg].y);
+    any = true;
+    const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
+    float r[REC9];
+#pragma unroll
+    for (int q = 0; q < REC9; ++q) r[q] = rp[q];
+    for (uint32_t j = 1; j < cnt; ++j) {
+#pragma unroll
+      for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[(size_t)j * REC9 + q];
+    }
+    const DevView &V = a.views[s];
+    float pc[4];
+    m4_mul_pos(V.view, pre.wx, pre.wy, pre.wz, pc);
+    const float nx = -pc[0], ny = -pc[1], nz = -pc[2];
+    const float len = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+    float b[16];
+    sh_basis(nx / len, ny / len, nz / len, a.cfg.sh_deg, b);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      gsh[3 * k + 0] = gsh[3 * k + 0] + b[k] * r[0];
+      gsh[3 * k + 1] = gsh[3 * k + 1] + b[k] * r[1];
+      gsh[3 * k + 2] = gsh[3 * k + 2] + b[k] * r[2];
+    }
+    glogit = glogit + r[3];
+    float dp[3], dl[3], dqq[4];
+    if (geom_vjp(V, pre, r[4], r[5], r[6], r[7], r[8], dp, dl, dqq)) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        gp[k] = gp[k] + dp[k];
+        gs[k] = gs[k] + dl[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) gq[k] = gq[k] + dqq[k];
+    }
+    densify_accum(dacc, r[4], r[5], r[6], r[7], r[8]);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) geo[q] = r[4 + q];
+  }
+  if (a.grad_accum) a.grad_accum[g] = dacc;
+  if (a.importance) a.importance[g] = imp;
+  if (a.geom) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) a.geom[(size_t)q * a.cap + g] = geo[q];
+  }
+  float grad[P_NUM];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    grad[P_POS + k] = gp[k];
+    grad[P_SCALE + k] = gs[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) grad[P_QUAT + k] = gq[k];
+  grad[P_LOGIT] = glogit;
+#pragma unroll
+  for (int k = 0; k < 48; ++k) grad[P_SH + k] = gsh[k];
+  if (a.G) {
+    if (!any) return;
+#pragma unroll
+    for (int k = 0; k < P_NUM; ++k) {
+      float *dst = a.G + (size_t)k * a.cap + g;
+      *dst = *dst + grad[k];
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < P_NUM; ++k) {
+    const int t = tensor_of_plane(k);
+    if (t == 4 && !a.update_sh) continue;
+    const float lr = a.lr[t];
+    if (t == 0 && !(lr > 0.0f)) continue;
+    const size_t o = (size_t)k * a.cap + g;
+    float p = a.Pw[o], m1 = a.M1[o], m2 = a.M2[o];
+    adam_elem(p, m1, m2, grad[k], lr, a.beta1, a.beta2, a.eps, a.bc1_rcp, a.bc2_sqrt_rcp);
+    a.Pw[o] = p;
+    a.M1[o] = m1;
+    a.M2[o] = m2;
+  }
+}
+
+void launch_vjp(hipStream_t st, const VjpArgs &a) {
+  if (a.N == 0) return;
+  hipLaunchKernelGGL(k_vjp, dim3((a.N + 255) / 256), dim3(256), 0, st, a);
+}
+
+// run_adam_updates (gaussian-renderer.ts:2455-2545) over the gradient
+// planes: one thread per (plane, Gaussian), learning rate per tensor.
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, float *__restrict__ M1,
+                                              float *__restrict__ M2, const float *__restrict__ G,
+                                              size_t cap, uint32_t N, float lr0, float lr1, float lr2,
+                                              float lr3, float lr4, float b1, float b2, float eps,
+                                              float bc1, float bc2s, int update_sh) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (g >= N) return;
+  const int t = tensor_of_plane(k);
+  if (t == 4 && !update_sh) return;
+  const float lr = t == 0 ? lr0 : (t == 1 ? lr1 : (t == 2 ? lr2 : (t == 3 ? lr3 : lr4)));
+  if (t == 0 && !(lr > 0.0f)) return;
+  const size_t o = (size_t)k * cap + g;
+  float p = P[o], m1 = M1[o], m2 = M2[o];
+  adam_elem(p, m1, m2, G[o], lr, b1, b2, eps, bc1, bc2s);
+  P[o] = p;
+  M1[o] = m1;
+  M2[o] = m2;
+}
+
+void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G, size_t cap,
+                 uint32_t N, const float lr[5], float beta1, float beta2, float eps, float bc1_rcp,
+                 float bc2_sqrt_rcp, int update_sh) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_adam, dim3((N + 255) / 256, P_NUM), dim3(256), 0, st, P, M1, M2, G, cap, N,
+                     lr[0], lr[1], lr[2], lr[3], lr[4], beta1, beta2, eps, bc1_rcp, bc2_sqrt_rcp,
+                     update_sh);
+}
+
+// backward_reduce (backward-reduce.wgsl:35-71): per intersection, the 54
+// fixed-point sums / 65536 with |v| > 1e-10 are added with float atomics
+// to grad_opacity, grad_geom and grad_sh (order not deterministic, as in
+// the reference).
+__global__ __launch_bounds__(256) void k_reduce_fixed(const int32_t *__restrict__ fixed,
+                                                      const uint32_t *__restrict__ vals, uint64_t I,
+                                                      size_t cap, float *__restrict__ G,
+                                                      float *__restrict__ geom) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const uint32_t g = vals[i];
+  if ((size_t)g >= cap) return;
+  const int32_t *s = fixed + i * 54;
+  const float inv = 1.0f / 65536.0f;
+  float v = (float)s[0] * inv;
+  if (fabsf(v) > 1e-10f) atomicAdd(&G[(size_t)P_LOGIT * cap + g], v);
+  for (int k = 0; k < 5; ++k) {
+    v = (float)s[1 + k] * inv;
+    if (fabsf(v) > 1e-10f) atomicAdd(&geom[(size_t)k * cap + g], v);
+  }
+  for (int k = 0; k < 48; ++k) {
+    v = (float)s[6 + k] * inv;
+    if (fabsf(v) > 1e-10f) atomicAdd(&G[(size_t)(P_SH + k) * cap + g], v);
+  }
+}
+
+void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals, uint64_t I,
+                         size_t cap, float *G, float *geom) {
+  if (I == 0) return;
+  hipLaunchKernelGGL(k_reduce_fixed, dim3((uint32_t)((I + 255) / 256)), dim3(256), 0, st, fixed, vals,
+                     I, cap, G, geom);
+}
+
+// training_backward_geom from the grad_geom planes (wavefront path) +
+// accumulate_grads.
+__global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_t cap, uint32_t N,
+                                              const DevView *__restrict__ view, DevConfig cfg,
+                                              const float *__restrict__ geom, float *__restrict__ G,
+                                              float4 *__restrict__ grad_accum) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  float prm[11];
+#pragma unroll
+  for (int k = 0; k < 11; ++k) {
+    const float v = P[(size_t)k * cap + g];
+    prm[k] = cfg.render_f16 ? f16_round(v) : v;
+  }
+  float r[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) r[q] = geom[(size_t)q * cap + g];
+  if (grad_accum) {
+    float4 acc = grad_accum[g];
+    densify_accum(acc, r[0], r[1], r[2], r[3], r[4]);
+    grad_accum[g] = acc;
+  }
+  const GeomPre pre = geom_pre(prm);
+  float dp[3], dl[3], dq[4];
+  if (!geom_vjp(*view, pre, r[0], r[1], r[2], r[3], r[4], dp, dl, dq)) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    G[(size_t)(P_POS + k) * cap + g] = G[(size_t)(P_POS + k) * cap + g] + dp[k];
+    G[(size_t)(P_SCALE + k) * cap + g] = G[(size_t)(P_SCALE + k) * cap + g] + dl[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) G[(size_t)(P_QUAT + k) * cap + g] = G[(size_t)(P_QUAT + k) * cap + g] + dq[k];
+}
+
+void launch_geom(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *view,
+                 const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_geom, dim3((N + 255) / 256), dim3(256), 0, st, P, cap, N, view, cfg, geom, G,
+                     grad_accum);
+}
+
+// ---------------------------------------------------------------------
+// Prune / densify on the device.
+// ---------------------------------------------------------------------
+// mode 0: o = a != 0 (keep flags given directly); mode 1: o = (a == 0)
+// (a = remove flags); mode 2: o = (a != 2) (densify: originals kept unless
+// split); mode 3: o = child count (clone 1, split 2).
+__global__ void k_u8_to_u32(const uint8_t *__restrict__ a, uint32_t *__restrict__ o, uint32_t N,
+                            int mode) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t v = a[i];
+  uint32_t r;
+  if (mode == 0) r = v != 0u;
+  else if (mode == 1) r = v == 0u;
+  else if (mode == 2) r = v != 2u;
+  else r = v == 1u ? 1u : (v == 2u ? 2u : 0u);
+  o[i] = r;
+}
+void launch_u8_to_u32(hipStream_t st, const uint8_t *a, uint32_t *o, uint32_t N, int mode) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_u8_to_u32, dim3((N + 255) / 256), dim3(256), 0, st, a, o, N, mode);
+}
+
+// prune_decide's opacity rule (prune-decide.wgsl:168-180).
+__global__ void k_prune_flags(const float *__restrict__ P, size_t cap, uint32_t N, float min_opacity,
+                              uint8_t *__restrict__ keep, uint32_t *__restrict__ keep_u32) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  const float op = sv_sigmoid(P[(size_t)P_LOGIT * cap + g]);
+  const uint32_t k = (op < min_opacity) ? 0u : 1u;
+  keep[g] = (uint8_t)k;
+  keep_u32[g] = k;
+}
+void launch_prune_flags(hipStream_t st, const float *P, size_t cap, uint32_t N, float min_opacity,
+                        uint8_t *keep, uint32_t *keep_u32) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_prune_flags, dim3((N + 255) / 256), dim3(256), 0, st, P, cap, N, min_opacity,
+                     keep, keep_u32);
+}
+
+// Order-preserving compaction of `planes` planes (applyPruneMask,
+// gaussian-renderer.ts:2917-3000): row g goes to keep_offs[g].
+__global__ __launch_bounds__(256) void k_compact(const float *__restrict__ src, float *__restrict__ dst,
+                                                 size_t cap_src, size_t cap_dst, int planes,
+                                                 const uint32_t *__restrict__ keep_offs,
+                                                 const uint8_t *__restrict__ keep, uint32_t N) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N || !keep[g]) return;
+  const uint32_t o = keep_offs[g];
+  for (int k = 0; k < planes; ++k) dst[(size_t)k * cap_dst + o] = src[(size_t)k * cap_src + g];
+}
+void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src, size_t cap_dst,
+                    int planes, const uint32_t *keep_offs, const uint8_t *keep, uint32_t N) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_compact, dim3((N + 255) / 256), dim3(256), 0, st, src, dst, cap_src, cap_dst,
+                     planes, keep_offs, keep, N);
+}
+
+// Counter-based SplitMix64 stream: value `ctr` of stream `seed`.
+__device__ __forceinline__ uint64_t rng_u64(uint64_t seed, uint64_t ctr) {
+  uint64_t z = seed + (ctr + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double rng_uniform(uint64_t seed, uint64_t ctr) {
+  return (double)(rng_u64(seed, ctr) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// performDensification (gaussian-renderer.ts:3211-3466) with the seeded
+// stream above in place of Math.random: originals that are not split go
+// to keep_offs[g]; children go to n_keep + child_offs[g]. Clone: one copy
+// moved by max(sigma) * 0.25 * (0.5 + 0.5 u) along a random direction
+// ((:3390-3398). Split: two copies at R * N(0, sigma) with R =
+// quatToMat3(rot) (:3086-3098, un-normalised quaternion as there) and
+// log-scale - ln(1.6) (:3421-3424). Children keep SH, opacity and
+// rotation and get zero moments (:3334-3340). Stream counters per parent:
+// 16 g + {0..3} clone, 16 g + {0..7} split.
+__global__ __launch_bounds__(256) void k_densify(const float *__restrict__ P, const float *__restrict__ M1,
+                                                 const float *__restrict__ M2, size_t cap, uint32_t N,
+                                                 const uint8_t *__restrict__ actions,
+                                                 const uint32_t *__restrict__ keep_offs,
+                                                 const uint32_t *__restrict__ child_offs, uint32_t n_keep,
+                                                 uint64_t seed, float *__restrict__ Pn,
+                                                 float *__restrict__ M1n, float *__restrict__ M2n,
+                                                 size_t cap_n) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  const uint32_t act = actions[g];
+  if (act != 2u) {
+    const uint32_t o = keep_offs[g];
+    for (int k = 0; k < P_NUM; ++k) {
+      Pn[(size_t)k * cap_n + o] = P[(size_t)k * cap + g];
+      M1n[(size_t)k * cap_n + o] = M1[(size_t)k * cap + g];
+      M2n[(size_t)k * cap_n + o] = M2[(size_t)k * cap + g];
+    }
+  }
+  if (act == 0u) return;
+  const uint32_t c0 = n_keep + child_offs[g];
+  const double ls0 = P[(size_t)(P_SCALE + 0) * cap + g], ls1 = P[(size_t)(P_SCALE + 1) * cap + g],
+               ls2 = P[(size_t)(P_SCALE + 2) * cap + g];
+  const double px = P[(size_t)P_POS * cap + g], py = P[(size_t)(P_POS + 1) * cap + g],
+               pz = P[(size_t)(P_POS + 2) * cap + g];
+  const uint64_t base = 16ull * g;
+  const int nchild = act == 1u ? 1 : 2;
+  for (int c = 0; c < nchild; ++c) {
+    const uint32_t o = c0 + (uint32_t)c;
+    for (int k = 0; k < P_NUM; ++k) {
+      Pn[(size_t)k * cap_n + o] = P[(size_t)k * cap + g];
+      M1n[(size_t)k * cap_n + o] = 0.0f;
+      M2n[(size_t)k * cap_n + o] = 0.0f;
+    }
+    double off[3];
+    if (act == 1u) {
+      const double js = fmax(fmax(exp(ls0), exp(ls1)), exp(ls2));
+      const double mag = js * 0.25 * (0.5 + rng_uniform(seed, base + 0) * 0.5);
+      double d[3] = {rng_uniform(seed, base + 1) * 2.0 - 1.0, rng_uniform(seed, base + 2) * 2.0 - 1.0,
+                     rng_uniform(seed, base + 3) * 2.0 - 1.0};
+      double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      if (nrm == 0.0) nrm = 1.0;
+      for (int k = 0; k < 3; ++k) off[k] = d[k] / nrm * mag;
+    } else {
+      const double sg[3] = {exp(ls0), exp(ls1), exp(ls2)};
+      const uint64_t cb = base + 4ull * (uint64_t)c;
+      const double u1 = rng_uniform(seed, cb + 0), u2 = rng_uniform(seed, cb + 1);
+      const double u3 = rng_uniform(seed, cb + 2), u4 = rng_uniform(seed, cb + 3);
+      const double r1 = sqrt(-2.0 * log(1.0 - u1)), r3 = sqrt(-2.0 * log(1.0 - u3));
+      const double n0 = r1 * cos(2.0 * M_PI * u2), n1 = r1 * sin(2.0 * M_PI * u2);
+      const double n2 = r3 * cos(2.0 * M_PI * u4);
+      const double ls[3] = {n0 * sg[0], n1 * sg[1], n2 * sg[2]};
+      const double x = P[(size_t)(P_QUAT + 0) * cap + g], y = P[(size_t)(P_QUAT + 1) * cap + g],
+                   z = P[(size_t)(P_QUAT + 2) * cap + g], w = P[(size_t)(P_QUAT + 3) * cap + g];
+      const double x2 = x + x, y2 = y + y, z2 = z + z;
+      const double xx = x * x2, xy = x * y2, xz = x * z2, yy = y * y2, yz = y * z2, zz = z * z2;
+      const double wx = w * x2, wy = w * y2, wz = w * z2;
+      const double Rm[3][3] = {{1 - (yy + zz), xy - wz, xz + wy},
+                               {xy + wz, 1 - (xx + zz), yz - wx},
+                               {xz - wy, yz + wx, 1 - (xx + yy)}};
+      for (int r = 0; r < 3; ++r) off[r] = Rm[r][0] * ls[0] + Rm[r][1] * ls[1] + Rm[r][2] * ls[2];
+      const double red = log(0.8 * 2.0);
+      Pn[(size_t)(P_SCALE + 0) * cap_n + o] = (float)(ls0 - red);
+      Pn[(size_t)(P_SCALE + 1) * cap_n + o] = (float)(ls1 - red);
+      Pn[(size_t)(P_SCALE + 2) * cap_n + o] = (float)(ls2 - red);
+    }
+    Pn[(size_t)(P_POS + 0) * cap_n + o] = (float)(px + off[0]);
+    Pn[(size_t)(P_POS + 1) * cap_n + o] = (float)(py + off[1]);
+    Pn[(size_t)(P_POS + 2) * cap_n + o] = (float)(pz + off[2]);
+  }
+}
+
+void launch_densify(hipStream_t st, const float *P, const float *M1, const float *M2, size_t cap,
+                    uint32_t N, const uint8_t *actions, const uint32_t *keep_offs,
+                    const uint32_t *child_offs, uint32_t n_keep, uint64_t seed, float *Pn, float *M1n,
+                    float *M2n, size_t cap_n) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_densify, dim3((N + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, N, actions,
+                     keep_offs, child_offs, n_keep, seed, Pn, M1n, M2n, cap_n);
+}
+
+}  // namespace sv

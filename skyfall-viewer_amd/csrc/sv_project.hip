@@ -1,0 +1,455 @@
+// sv_project.hip -- per-Gaussian projection, device prefix sum, key emit,
+// tile ranges and the loss residual (gfx950).
+#include "sv_internal.hpp"
+
+namespace sv {
+
+// ---------------------------------------------------------------------
+// Projection: project_gaussians_for_tiling (training-tiles.wgsl:39-180)
+// fused with the per-view SH colour (training-forward-tiles.wgsl:65-105,
+// evaluated once per (Gaussian, view) instead of once per pixel; the
+// inputs of that evaluation depend only on the Gaussian and the view, so
+// the value is the same) and the opacity sigmoid (:188).
+//
+// One thread per Gaussian, looping over the views of the step, so the 59
+// parameters are read once per step. Records are four float4 planes of
+// stride rs: R0 = (mean.x, mean.y, conic.a, conic.b), R1 = (conic.c,
+// opacity, r, g), R2 = (b, radius, depth, depth_q bits), R3 = (rect
+// x0|y0<<16, rect x1|y1<<16, tiles_count, 0).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, size_t cap, uint32_t N,
+                                                 const DevView *__restrict__ views, int nv,
+                                                 DevConfig cfg, float4 *__restrict__ rec, size_t rs,
+                                                 uint32_t *__restrict__ counts,
+                                                 uint32_t *__restrict__ max_dq) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t local_max_dq = 0;
+  if (g < N) {
+    float prm[11];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+      const float v = P[(size_t)k * cap + g];
+      prm[k] = cfg.render_f16 ? f16_round(v) : v;
+    }
+    const float wx = prm[P_POS], wy = prm[P_POS + 1], wz = prm[P_POS + 2];
+    float sigma[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sigma[k] = sv_expf(clampf(prm[P_SCALE + k], -10.0f, 10.0f));
+    const float qd = ((prm[P_QUAT] * prm[P_QUAT] + prm[P_QUAT + 1] * prm[P_QUAT + 1]) +
+                      prm[P_QUAT + 2] * prm[P_QUAT + 2]) +
+                     prm[P_QUAT + 3] * prm[P_QUAT + 3];
+    const float ql = sqrtf(qd);
+    const M3 R = quat_to_mat3(prm[P_QUAT] / ql, prm[P_QUAT + 1] / ql, prm[P_QUAT + 2] / ql,
+                              prm[P_QUAT + 3] / ql);
+    M3 S;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) S.m[i][j] = (i == j) ? sigma[i] : 0.0f;
+    // C3D = transpose(S * R) * S * R  (training-tiles.wgsl:101)
+    const M3 C3D = mul(mul(transpose(mul(S, R)), S), R);
+    const float logit = prm[P_LOGIT];
+    const float opac = sv_sigmoid(logit);
+
+    for (int s = 0; s < nv; ++s) {
+      const DevView &V = views[s];
+      const size_t vg = (size_t)s * N + g;
+      float pc[4], clip[4];
+      m4_mul_pos(V.view, wx, wy, wz, pc);
+      m4_mul_v4(V.proj, pc, clip);
+      const float w = clip[3];
+      const float cx = clip[0] / w, cy = clip[1] / w;
+      bool alive = !(fabsf(cx) > 1.2f || fabsf(cy) > 1.2f || pc[2] <= 0.0f);
+      const float depth = pc[2];
+      if (depth < cfg.near_plane || depth > cfg.far_plane) alive = false;
+      float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
+      uint32_t count = 0;
+      if (alive) {
+        const float tx = pc[0], ty = pc[1], tz = pc[2];
+        const float fx = V.focal[0], fy = V.focal[1];
+        M3 J;
+        J.m[0][0] = fx / tz;
+        J.m[0][1] = 0.0f;
+        J.m[0][2] = -(fx * tx) / (tz * tz);
+        J.m[1][0] = 0.0f;
+        J.m[1][1] = fy / tz;
+        J.m[1][2] = -(fy * ty) / (tz * tz);
+        J.m[2][0] = 0.0f;
+        J.m[2][1] = 0.0f;
+        J.m[2][2] = 0.0f;
+        M3 Rwc;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int r = 0; r < 3; ++r) Rwc.m[c][r] = V.view[c * 4 + r];
+        const M3 T = mul(J, Rwc);
+        const M3 C2D = mul(mul(T, C3D), transpose(T));
+        const float c00 = C2D.m[0][0] + cfg.eps2d;
+        const float c11 = C2D.m[1][1] + cfg.eps2d;
+        const float c01 = C2D.m[0][1];
+        const float det = c00 * c11 - c01 * c01;
+        if (det <= 0.0f) alive = false;
+        const float mid = 0.5f * (c00 + c11);
+        const float disc = sqrtf(fmaxf(mid * mid - det, 0.0f));
+        const float l1 = mid + disc, l2 = mid - disc;
+        const float radius = ceilf(3.0f * sqrtf(fmaxf(l1, l2)));
+        if (radius > cfg.radius_clip || radius < 1.0f) alive = false;
+        if (alive) {
+          const float det_inv = 1.0f / det;
+          const float aa = c11 * det_inv;
+          const float bb = -c01 * det_inv;
+          const float cc = c00 * det_inv;
+          const float mx = (0.5f * (cx + 1.0f)) * (float)V.W;
+          const float my = (0.5f * (cy + 1.0f)) * (float)V.H;
+          // tile rect, training-tiles.wgsl:161-171
+          const float ts = (float)cfg.tile_size;
+          const float tlx = mx / ts, tly = my / ts, trad = radius / ts;
+          uint32_t x0 = f2u_sat(floorf(tlx - trad)), y0 = f2u_sat(floorf(tly - trad));
+          uint32_t x1 = f2u_sat(ceilf(tlx + trad)), y1 = f2u_sat(ceilf(tly + trad));
+          x0 = min(x0, V.tiles_x);
+          y0 = min(y0, V.tiles_y);
+          x1 = min(x1, V.tiles_x);
+          y1 = min(y1, V.tiles_y);
+          const int32_t nx = (int32_t)x1 - (int32_t)x0, ny = (int32_t)y1 - (int32_t)y0;
+          count = (nx <= 0 || ny <= 0) ? 0u : (uint32_t)(nx * ny);
+          // viewDir = normalize(-(view * pos).xyz), training-forward-tiles.wgsl:185
+          const float dxv = -pc[0], dyv = -pc[1], dzv = -pc[2];
+          const float len = sqrtf(dot3(dxv, dyv, dzv, dxv, dyv, dzv));
+          float b[16];
+          sh_basis(dxv / len, dyv / len, dzv / len, cfg.sh_deg, b);
+          float rgb[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            float coef[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+              const float v = P[(size_t)(P_SH + 3 * k + c) * cap + g];
+              coef[k] = cfg.render_f16 ? f16_round(v) : v;
+            }
+            rgb[c] = sh_channel(b, coef, 1, cfg.sh_deg);
+          }
+          const uint32_t dq = f2u_sat(clampf(depth, 0.0f, 65535.0f)) & 0xffffu;
+          if (count > 0u) local_max_dq = max(local_max_dq, dq);
+          r0 = make_float4(mx, my, aa, bb);
+          r1 = make_float4(cc, opac, rgb[0], rgb[1]);
+          r2 = make_float4(rgb[2], radius, depth, __uint_as_float(dq));
+          r3 = make_float4(__uint_as_float(x0 | (y0 << 16)), __uint_as_float(x1 | (y1 << 16)),
+                           __uint_as_float(count), 0.0f);
+        }
+      }
+      rec[vg] = r0;
+      rec[rs + vg] = r1;
+      rec[2 * rs + vg] = r2;
+      rec[3 * rs + vg] = r3;
+      counts[vg] = count;
+    }
+  }
+  // block max of the depth quantum, then one atomic per block
+  uint32_t m = local_max_dq;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  __shared__ uint32_t s_m[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) s_m[wave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t mm = max(max(s_m[0], s_m[1]), max(s_m[2], s_m[3]));
+    if (mm) atomicMax(max_dq, mm);
+  }
+}
+
+void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
+                    int nv, const DevConfig &cfg, float4 *rec, size_t rs, uint32_t *counts,
+                    uint32_t *max_dq) {
+  if (N == 0) return;
+  const uint32_t blocks = (N + 255u) / 256u;
+  hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, rs,
+                     counts, max_dq);
+}
+
+// ---------------------------------------------------------------------
+// Exclusive prefix sum of u32 counts (replaces the host cumsum of
+// gaussian-renderer.ts:1814-1828). Reduce-then-scan, 4096 per block.
+// ---------------------------------------------------------------------
+constexpr int SCAN_T = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_T * SCAN_ITEMS;
+
+__device__ __forceinline__ uint64_t block_exclusive_scan_u64(uint64_t v, uint64_t *s_w,
+                                                             uint64_t *block_total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint64_t wbase = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+    if (w < wave) wbase += s_w[w];
+    tot += s_w[w];
+  }
+  if (block_total) *block_total = tot;
+  return wbase + x - v;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const uint32_t *__restrict__ in, size_t M,
+                                                        uint64_t *__restrict__ bsum) {
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE;
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + (size_t)j * SCAN_T + threadIdx.x;
+    if (i < M) s += in[i];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  __shared__ uint64_t s_w[SCAN_T / 64];
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int w = 0; w < SCAN_T / 64; ++w) t += s_w[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// Single workgroup: exclusive scan of the block sums in place; total out.
+__global__ __launch_bounds__(1024) void k_scan_top(uint64_t *__restrict__ bsum, size_t nb,
+                                                   uint64_t *__restrict__ total) {
+  __shared__ uint64_t s_w[16];
+  uint64_t carry = 0;
+  for (size_t base = 0; base < nb; base += 1024) {
+    const size_t i = base + threadIdx.x;
+    const uint64_t v = (i < nb) ? bsum[i] : 0;
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan_u64(v, s_w, &tot);
+    if (i < nb) bsum[i] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_apply(const uint32_t *__restrict__ in, size_t M,
+                                                       const uint64_t *__restrict__ bsum,
+                                                       uint32_t *__restrict__ out) {
+  const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint64_t s = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + j;
+    v[j] = (i < M) ? in[i] : 0u;
+    s += v[j];
+  }
+  __shared__ uint64_t s_w[SCAN_T / 64];
+  uint64_t run = block_exclusive_scan_u64(s, s_w, nullptr) + bsum[blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < SCAN_ITEMS; ++j) {
+    const size_t i = base + j;
+    if (i < M) out[i] = (uint32_t)run;
+    run += v[j];
+  }
+}
+
+size_t scan_tmp_words(size_t M) { return (M + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t M, uint64_t *tmp,
+                 uint64_t *total) {
+  const size_t nb = (M + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 0) {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tmp, (size_t)0, total);
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(SCAN_T), 0, st, counts, M, tmp);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tmp, nb, total);
+  hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)nb), dim3(SCAN_T), 0, st, counts, M, tmp, offs);
+}
+
+// ---------------------------------------------------------------------
+// emit_intersections (training-tiles.wgsl:201-247). Key = (global tile <<
+// dbits) | depth_q with depth_q = u32(clamp(depth, 0, 65535)) & 0xffff
+// (encode_isect_key, :182-199). dbits is the bit width of the largest
+// depth_q of the step, so the key order equals the reference key order
+// (camera, tile, depth_q) restricted to the keys present. Value =
+// Gaussian index. Order within a Gaussian: ty-major, then tx (:238-246).
+// ---------------------------------------------------------------------
+template <typename K>
+__global__ __launch_bounds__(256) void k_emit(const float4 *__restrict__ rec, size_t rs,
+                                              const uint32_t *__restrict__ counts,
+                                              const uint32_t *__restrict__ offs, uint32_t N, int nv,
+                                              const DevView *__restrict__ views, uint32_t dbits,
+                                              K *__restrict__ keys, uint32_t *__restrict__ vals) {
+  const size_t vg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vg >= (size_t)nv * N) return;
+  const uint32_t cnt = counts[vg];
+  if (cnt == 0u) return;
+  const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
+  const DevView &V = views[s];
+  const float4 r2 = rec[2 * rs + vg], r3 = rec[3 * rs + vg];
+  const uint32_t dq = __float_as_uint(r2.w);
+  const uint32_t a = __float_as_uint(r3.x), b = __float_as_uint(r3.y);
+  const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu, y1 = b >> 16;
+  uint32_t p = offs[vg];
+  for (uint32_t ty = y0; ty < y1; ++ty)
+    for (uint32_t tx = x0; tx < x1; ++tx) {
+      const K tile = (K)(V.tile_base + ty * V.tiles_x + tx);
+      keys[p] = (tile << dbits) | (K)dq;
+      vals[p] = g;
+      ++p;
+    }
+}
+
+template <typename K>
+void launch_emit(hipStream_t st, const float4 *rec, size_t rs, const uint32_t *counts,
+                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits,
+                 K *keys, uint32_t *vals) {
+  const size_t M = (size_t)nv * N;
+  if (M == 0) return;
+  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, rs, counts,
+                     offs, N, nv, views, dbits, keys, vals);
+}
+template void launch_emit<uint32_t>(hipStream_t, const float4 *, size_t, const uint32_t *,
+                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
+                                    uint32_t *, uint32_t *);
+template void launch_emit<uint64_t>(hipStream_t, const float4 *, size_t, const uint32_t *,
+                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
+                                    uint64_t *, uint32_t *);
+
+// ---------------------------------------------------------------------
+// Tile ranges: ranges[t] = first sorted index whose tile is >= t, which is
+// what build_tile_offsets' atomicMin plus the host reverse-min fix-up
+// computes (training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
+// Every entry is written by exactly one thread; no atomics.
+// ---------------------------------------------------------------------
+template <typename K>
+__global__ __launch_bounds__(256) void k_ranges(const K *__restrict__ keys, uint64_t I, uint32_t dbits,
+                                                uint32_t n_tiles, uint32_t *__restrict__ ranges) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const uint32_t t = (uint32_t)(keys[i] >> dbits);
+  const uint32_t lo = (i == 0) ? 0u : (uint32_t)(keys[i - 1] >> dbits) + 1u;
+  for (uint32_t u = lo; u <= t && u <= n_tiles; ++u) ranges[u] = (uint32_t)i;
+  if (i == I - 1)
+    for (uint32_t u = t + 1u; u <= n_tiles; ++u) ranges[u] = (uint32_t)I;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float *p, float v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+}
+void launch_fill_f32(hipStream_t st, float *p, float v, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+}
+
+template <typename K>
+void launch_ranges(hipStream_t st, const K *keys, uint64_t I, uint32_t dbits, uint32_t n_tiles,
+                   uint32_t *ranges) {
+  if (I == 0) {
+    launch_fill_u32(st, ranges, 0u, (size_t)n_tiles + 1);
+    return;
+  }
+  hipLaunchKernelGGL(k_ranges<K>, dim3((uint32_t)((I + 255) / 256)), dim3(256), 0, st, keys, I, dbits,
+                     n_tiles, ranges);
+}
+template void launch_ranges<uint32_t>(hipStream_t, const uint32_t *, uint64_t, uint32_t, uint32_t,
+                                      uint32_t *);
+template void launch_ranges<uint64_t>(hipStream_t, const uint64_t *, uint64_t, uint32_t, uint32_t,
+                                      uint32_t *);
+
+// ---------------------------------------------------------------------
+// training_loss (training-loss.wgsl:86-126) for one view: residual =
+// w_l2 (p - t) + w_l1 sign(p - t) [+ w_ssim * dssim (p - t), :33-84].
+// Colour planes: c * P + y * W + x within the view's block of 3P floats.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float signf_ref(float x) {
+  return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_loss(uint32_t W, uint32_t H, DevConfig cfg,
+                                              const float *__restrict__ color,
+                                              const float *__restrict__ target,
+                                              float *__restrict__ residual) {
+  const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= W || y >= H) return;
+  const size_t P = (size_t)W * H;
+  const size_t pix = (size_t)y * W + x;
+  float gs[3] = {0.f, 0.f, 0.f};
+  if (cfg.w_ssim > 0.0f) {
+    float mux[3] = {0, 0, 0}, muy[3] = {0, 0, 0};
+    const float n = 25.0f;
+    for (int dy = -2; dy <= 2; ++dy)
+      for (int dx = -2; dx <= 2; ++dx) {
+        const int xp = min(max((int)x + dx, 0), (int)W - 1);
+        const int yp = min(max((int)y + dy, 0), (int)H - 1);
+        const size_t q = (size_t)yp * W + xp;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          mux[c] = mux[c] + color[c * P + q];
+          muy[c] = muy[c] + target[c * P + q];
+        }
+      }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mux[c] = mux[c] / n;
+      muy[c] = muy[c] / n;
+    }
+    float sx2[3] = {0, 0, 0}, sy2[3] = {0, 0, 0}, sxy[3] = {0, 0, 0};
+    for (int dy = -2; dy <= 2; ++dy)
+      for (int dx = -2; dx <= 2; ++dx) {
+        const int xp = min(max((int)x + dx, 0), (int)W - 1);
+        const int yp = min(max((int)y + dy, 0), (int)H - 1);
+        const size_t q = (size_t)yp * W + xp;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float a = color[c * P + q] - mux[c];
+          const float b = target[c * P + q] - muy[c];
+          sx2[c] = sx2[c] + a * a;
+          sy2[c] = sy2[c] + b * b;
+          sxy[c] = sxy[c] + a * b;
+        }
+      }
+    const float C1 = 0.01f * 0.01f, C2 = 0.03f * 0.03f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      sx2[c] = sx2[c] / n;
+      sy2[c] = sy2[c] / n;
+      sxy[c] = sxy[c] / n;
+      const float num1 = (2.0f * mux[c]) * muy[c] + C1;
+      const float num2 = 2.0f * sxy[c] + C2;
+      const float den1 = (mux[c] * mux[c] + muy[c] * muy[c]) + C1;
+      const float den2 = (sx2[c] + sy2[c]) + C2;
+      const float ssim = (num1 * num2) / (den1 * den2);
+      const float dssim = (1.0f - ssim) * 0.5f;
+      gs[c] = dssim * (color[c * P + pix] - target[c * P + pix]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float d = color[c * P + pix] - target[c * P + pix];
+    float g = cfg.w_l2 * d + cfg.w_l1 * signf_ref(d);
+    if (cfg.w_ssim > 0.0f) g = g + cfg.w_ssim * gs[c];
+    residual[c * P + pix] = g;
+  }
+}
+
+void launch_loss(hipStream_t st, const DevView &hv, const DevConfig &cfg, const float *color,
+                 const float *target, float *residual) {
+  const dim3 grid((hv.W + 15) / 16, (hv.H + 15) / 16);
+  hipLaunchKernelGGL(k_loss, grid, dim3(256), 0, st, hv.W, hv.H, cfg, color, target, residual);
+}
+
+}  // namespace sv

@@ -1,0 +1,246 @@
+// sv_sort.hip -- stable LSD radix sort of (key, u32 value) pairs for gfx950.
+//
+// Replaces the reference's 4-pass WGSL sort (src/sort/sort.ts:260-370,
+// src/sort/radix_sort.wgsl), whose ranking emulates 32-lane subgroup
+// matches through LDS. Here each pass is reduce-then-scan with no
+// inter-workgroup waiting:
+//   count    per 4096-key tile digit counts -> hist[digit][tile]
+//   rowsum   per digit total over tiles
+//   rowscan  hist -> global output offset of (digit, tile)
+//   scatter  wave64 ballot match ranking (8 ballots per key row), per-wave
+//            digit counters, block scan, LDS staging, coalesced scatter
+// Every pass is stable, so the whole sort equals a stable sort by key.
+#include "sv_internal.hpp"
+
+namespace sv {
+
+constexpr int SORT_T = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_T * SORT_ITEMS;  // 4096 keys per workgroup
+constexpr int RADIX = 256;
+constexpr int NWAVE = SORT_T / 64;
+
+template <typename K>
+__global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ keys, uint64_t n,
+                                                      uint32_t shift, uint32_t nb,
+                                                      uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_h[NWAVE][RADIX];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_h[0][0])[i] = 0u;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE + (uint64_t)wave * (64 * SORT_ITEMS);
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (uint32_t)(keys[i] >> shift) & (RADIX - 1);
+      atomicAdd(&s_h[wave][d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < RADIX; d += SORT_T) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) c += s_h[w][d];
+    hist[(size_t)d * nb + blockIdx.x] = c;
+  }
+}
+
+__device__ __forceinline__ uint32_t block_reduce_u32(uint32_t v, uint32_t *s_w) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_w[w];
+  __syncthreads();
+  return t;
+}
+
+// One workgroup per digit: total of the digit over all tiles.
+__global__ __launch_bounds__(SORT_T) void k_sort_rowsum(const uint32_t *__restrict__ hist, uint32_t nb,
+                                                       uint32_t *__restrict__ dtot) {
+  __shared__ uint32_t s_w[NWAVE];
+  const uint32_t d = blockIdx.x;
+  uint32_t s = 0;
+  for (uint32_t b = threadIdx.x; b < nb; b += SORT_T) s += hist[(size_t)d * nb + b];
+  s = block_reduce_u32(s, s_w);
+  if (threadIdx.x == 0) dtot[d] = s;
+}
+
+// One workgroup per digit: hist row -> exclusive scan + digit base.
+__global__ __launch_bounds__(SORT_T) void k_sort_rowscan(uint32_t *__restrict__ hist, uint32_t nb,
+                                                        const uint32_t *__restrict__ dtot) {
+  __shared__ uint32_t s_w[NWAVE];
+  const uint32_t d = blockIdx.x;
+  uint32_t part = (threadIdx.x < d) ? dtot[threadIdx.x] : 0u;
+  uint32_t carry = block_reduce_u32(part, s_w);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t base = 0; base < nb; base += SORT_T) {
+    const uint32_t b = base + threadIdx.x;
+    const uint32_t v = (b < nb) ? hist[(size_t)d * nb + b] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) {
+      if (w < wave) wbase += s_w[w];
+      tot += s_w[w];
+    }
+    if (b < nb) hist[(size_t)d * nb + b] = carry + wbase + x - v;
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ kin,
+                                                        const uint32_t *__restrict__ vin,
+                                                        K *__restrict__ kout,
+                                                        uint32_t *__restrict__ vout, uint64_t n,
+                                                        uint32_t shift, uint32_t nb,
+                                                        const uint32_t *__restrict__ hist) {
+  __shared__ uint32_t s_wcnt[NWAVE][RADIX];
+  __shared__ uint32_t s_lbase[RADIX];
+  __shared__ uint32_t s_gbase[RADIX];
+  __shared__ K s_keys[SORT_TILE];
+  __shared__ uint32_t s_vals[SORT_TILE];
+  __shared__ uint32_t s_w[NWAVE];
+
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_wcnt[0][0])[i] = 0u;
+  for (int d = threadIdx.x; d < RADIX; d += SORT_T) s_gbase[d] = hist[(size_t)d * nb + blockIdx.x];
+  __syncthreads();
+
+  const uint64_t tile0 = (uint64_t)blockIdx.x * SORT_TILE;
+  const uint64_t base = tile0 + (uint64_t)wave * (64 * SORT_ITEMS);
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  K key[SORT_ITEMS];
+  uint32_t val[SORT_ITEMS];
+  uint32_t rank[SORT_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 64 + lane;
+    const bool valid = i < n;
+    key[j] = valid ? kin[i] : (K)0;
+    val[j] = valid ? vin[i] : 0u;
+    const uint32_t d = (uint32_t)(key[j] >> shift) & (RADIX - 1);
+    // lanes holding the same digit: AND of ballots over the 8 digit bits
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t bb = __ballot(on);
+      m &= on ? bb : ~bb;
+    }
+    uint32_t before = 0;
+    if (valid) before = s_wcnt[wave][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid) {
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      if (lane == leader) s_wcnt[wave][d] = before + (uint32_t)__popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    rank[j] = before + (uint32_t)__popcll(m & lt_mask);
+  }
+  __syncthreads();
+  // per digit: prefix over waves, block total, block exclusive scan
+  {
+    const int d = threadIdx.x;  // SORT_T == RADIX
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w) {
+      const uint32_t c = s_wcnt[w][d];
+      s_wcnt[w][d] = tot;
+      tot += c;
+    }
+    uint32_t x = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t wb = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVE; ++w)
+      if (w < wave) wb += s_w[w];
+    s_lbase[d] = wb + x - tot;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 64 + lane;
+    if (i < n) {
+      const uint32_t d = (uint32_t)(key[j] >> shift) & (RADIX - 1);
+      const uint32_t lpos = s_lbase[d] + s_wcnt[wave][d] + rank[j];
+      if (lpos < (uint32_t)SORT_TILE) {
+        s_keys[lpos] = key[j];
+        s_vals[lpos] = val[j];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)SORT_TILE ? (n - tile0) : SORT_TILE);
+  for (uint32_t k = threadIdx.x; k < cnt; k += SORT_T) {
+    const K kk = s_keys[k];
+    const uint32_t d = (uint32_t)(kk >> shift) & (RADIX - 1);
+    const uint32_t g = s_gbase[d] + (k - s_lbase[d]);
+    if ((uint64_t)g < n) {
+      kout[g] = kk;
+      vout[g] = s_vals[k];
+    }
+  }
+}
+
+template <typename K>
+size_t sort_tmp_words(uint64_t n) {
+  const uint64_t nb = (n + SORT_TILE - 1) / SORT_TILE;
+  return (size_t)(nb * RADIX + RADIX);
+}
+
+template <typename K>
+void launch_sort(hipStream_t st, K *keys, uint32_t *vals, K *keys2, uint32_t *vals2, uint64_t n,
+                 uint32_t bits, uint32_t *tmp, int *passes_out) {
+  const uint32_t nb = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+  const int passes = (int)((bits + 7) / 8);
+  *passes_out = passes;
+  if (n <= 1 || passes == 0) {
+    *passes_out = 0;
+    return;
+  }
+  uint32_t *hist = tmp;
+  uint32_t *dtot = tmp + (size_t)nb * RADIX;
+  K *kin = keys, *kout = keys2;
+  uint32_t *vin = vals, *vout = vals2;
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t shift = 8u * (uint32_t)p;
+    hipLaunchKernelGGL(k_sort_count<K>, dim3(nb), dim3(SORT_T), 0, st, kin, n, shift, nb, hist);
+    hipLaunchKernelGGL(k_sort_rowsum, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
+    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
+    hipLaunchKernelGGL(k_sort_scatter<K>, dim3(nb), dim3(SORT_T), 0, st, kin, vin, kout, vout, n,
+                       shift, nb, hist);
+    K *tk = kin;
+    kin = kout;
+    kout = tk;
+    uint32_t *tv = vin;
+    vin = vout;
+    vout = tv;
+  }
+}
+
+template size_t sort_tmp_words<uint32_t>(uint64_t);
+template size_t sort_tmp_words<uint64_t>(uint64_t);
+template void launch_sort<uint32_t>(hipStream_t, uint32_t *, uint32_t *, uint32_t *, uint32_t *,
+                                    uint64_t, uint32_t, uint32_t *, int *);
+template void launch_sort<uint64_t>(hipStream_t, uint64_t *, uint32_t *, uint64_t *, uint32_t *,
+                                    uint64_t, uint32_t, uint32_t *, int *);
+
+}  // namespace sv
