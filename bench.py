@@ -140,7 +140,20 @@ def main():
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (torch) three times, to scale the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": x.numel() * 4}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
               "(multi-GPU sharding by views is a design only)", file=sys.stderr)
@@ -164,10 +177,16 @@ def main():
 
     sec_b, tim_b, st_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args)
     log(f"batched: {1e3 * sec_b:.2f} ms/step, I={st_b.n_isect}, key bits {st_b.key_bits}, passes {st_b.sort_passes}")
-    plain = None
+    plain = sequential = None
     if not args.no_plain:
-        sec_p, tim_p, st_p = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
+        sec_q, tim_q, _ = run_mode(sv.MODE_SEQUENTIAL, g, cams, targets, args)
+        sequential = {"ms_per_step": 1e3 * sec_q, "steps_per_s": 1.0 / sec_q,
+                      "phase_ms_per_step": {k: v[0] / args.steps for k, v in tim_q.items() if v[1]}}
+        log(f"sequential adjoint: {1e3 * sec_q:.2f} ms/step")
+        sec_p, tim_p, _ = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
         plain = {"ms_per_step": 1e3 * sec_p, "steps_per_s": 1.0 / sec_p,
+                 "method": "reference method per view: wavefront fixed-point backward, atomic reduce, "
+                           "geometry backward, Adam",
                  "phase_ms_per_step": {k: v[0] / args.steps for k, v in tim_p.items() if v[1]}}
         log(f"plain: {1e3 * sec_p:.2f} ms/step")
 
@@ -207,6 +226,8 @@ def main():
         "sort_passes": passes,
         "plain": plain,
         "speedup_vs_plain": (plain["ms_per_step"] / (1e3 * sec_b)) if plain else None,
+        "sequential_adjoint": sequential,
+        "speedup_vs_sequential": (sequential["ms_per_step"] / (1e3 * sec_b)) if sequential else None,
         "phase_ms_per_launch": {k: v[0] for k, v in phase.items()},
         "step_b_alg_bytes": b_alg,
         "step_b_alg_GBps": b_alg / sec_b / 1e9,

@@ -76,11 +76,18 @@ typedef struct sv_config {
   uint32_t use_graph;       /* 1: capture the batched step in a hipGraph */
   uint32_t keep_grads;      /* 1: batched steps also write the gradient
                                buffers (sv_read_grads) before Adam */
+  uint32_t plain_wf_flags;  /* SV_WF_* flags of the SV_MODE_PLAIN backward */
 } sv_config;
 
 /* Step modes for sv_step. */
-#define SV_MODE_PLAIN 0   /* one view at a time (the reference's structure) */
-#define SV_MODE_BATCHED 1 /* all views in one pipeline, (view,tile,depth) keys */
+#define SV_MODE_PLAIN 0      /* the reference's method, one view at a time:
+                                wavefront fixed-point backward (54 i32 slots per
+                                intersection), atomic reduce, separate
+                                geometry backward, then Adam */
+#define SV_MODE_BATCHED 1    /* adjoint backward, all views in one pipeline
+                                with (view, tile, depth) keys */
+#define SV_MODE_SEQUENTIAL 2 /* adjoint backward, one view at a time (same
+                                kernels and results as SV_MODE_BATCHED) */
 
 /* Wavefront fixed-point backward flags (backward-tiles-local.wgsl). */
 #define SV_WF_REFERENCE_ORDER 0x1u /* 64-entry batches front-to-back (:262) */

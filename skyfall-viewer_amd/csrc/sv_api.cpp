@@ -349,11 +349,10 @@ sv_status build_batch(sv_ctx *c) {
   const int nv = (int)c->slot_views.size();
   const DevConfig dc = dev_config(c->cfg);
   const size_t M = (size_t)nv * c->N;
-  const size_t rs = rec_stride(c);
   launch_fill_u32(c->st, (uint32_t *)c->d_small, 0u, 6);
   {
     Scoped t(c, PH_PROJECT);
-    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, rs, c->counts,
+    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->counts,
                    (uint32_t *)(c->d_small + 1));
   }
   if (sv_status s = check_launch(c, "project")) return s;
@@ -393,10 +392,10 @@ sv_status build_batch(sv_ctx *c) {
   {
     Scoped t(c, PH_EMIT);
     if (c->key64)
-      launch_emit<uint64_t>(c->st, c->rec, rs, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+      launch_emit<uint64_t>(c->st, c->rec, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
                             (uint64_t *)c->keys, c->vals);
     else
-      launch_emit<uint32_t>(c->st, c->rec, rs, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+      launch_emit<uint32_t>(c->st, c->rec, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
                             (uint32_t *)c->keys, c->vals);
   }
   if (sv_status s = check_launch(c, "emit")) return s;
@@ -434,7 +433,6 @@ RasterArgs raster_args(sv_ctx *c) {
   a.vals = c->vals;
   a.n_isect = c->I;
   a.rec = c->rec;
-  a.rs = rec_stride(c);
   a.N = c->N;
   a.n_tiles_total = c->tiles_total;
   a.cfg = dev_config(c->cfg);
@@ -465,7 +463,7 @@ sv_status forward_batch(sv_ctx *c) {
   const uint32_t grid = raster_grid(c, a);
   {
     Scoped t(c, PH_FORWARD);
-    launch_forward(c->st, a, c->cfg.batch, c->cfg.raster_px, grid);
+    launch_forward(c->st, a, c->cfg.batch, grid);
   }
   if (sv_status s = check_launch(c, "forward")) return s;
   c->forward_done = true;
@@ -524,7 +522,6 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.counts = c->counts;
   v.offs = c->offs;
   v.rec = c->rec;
-  v.rs = rec_stride(c);
   v.isect_rec = c->isect_rec;
   v.importance = c->importance;
   v.G = c->G;
@@ -655,6 +652,30 @@ sv_status loss_views(sv_ctx *c) {
                 c->residual + 3 * hv.pix_off);
   }
   return check_launch(c, "loss");
+}
+
+// run_wavefront_backward + run_training_backward_geom
+// (gaussian-renderer.ts:2312-2407) for the single-view batch, reading the
+// residual arena. Adds into G; grad_geom of the view is left in geom.
+sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
+  if (c->I > c->fixed_cap || !c->fixed) {
+    dfree(c->fixed);
+    c->fixed_cap = 0;
+    const uint64_t want = std::max<uint64_t>(c->I + c->I / 4, 1024);
+    if (sv_status s = dalloc(c, &c->fixed, (size_t)want * 54)) return s;
+    c->fixed_cap = want;
+  }
+  RasterArgs a = raster_args(c);
+  a.residual = c->residual;
+  launch_fill_f32(c->st, c->geom, 0.f, (size_t)5 * c->cap);
+  {
+    Scoped t(c, PH_FIXED);
+    launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed);
+    launch_reduce_fixed(c->st, c->fixed, c->vals, c->I, c->cap, c->G, c->geom);
+    launch_geom(c->st, c->P, c->cap, c->N, c->d_views, dev_config(c->cfg), c->geom, c->G,
+                stats ? c->grad_accum : nullptr, c->rec, c->counts, stats ? c->importance : nullptr);
+  }
+  return check_launch(c, "backward_fixed");
 }
 
 sv_status step_batched(sv_ctx *c, const int *vs, int nv) {
@@ -814,6 +835,7 @@ sv_status sv_default_config(sv_config *o) {
   o->persistent = 0;
   o->use_graph = 0;
   o->keep_grads = 0;
+  o->plain_wf_flags = 0;
   return SV_OK;
 }
 
@@ -907,6 +929,7 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
     return fail(c, SV_E_ARG, "batch must be 64, 128 or 256 (got %u)", k->batch);
   if (k->raster_px != 16) return fail(c, SV_E_ARG, "raster_px must be 16 (got %u)", k->raster_px);
   if (k->sh_deg > 3) return fail(c, SV_E_ARG, "sh_deg must be <= 3");
+  if (k->plain_wf_flags > 3) return fail(c, SV_E_ARG, "plain_wf_flags must be a combination of SV_WF_*");
   if (k->use_graph) return fail(c, SV_E_ARG, "use_graph is not available in this build");
   c->cfg = *k;
   return SV_OK;
@@ -1088,22 +1111,7 @@ sv_status sv_backward_fixed(sv_ctx *c, int v, uint32_t flags) {
     if (!c->views[v].has_target) return fail(c, SV_E_STATE, "view %d has no target", v);
     if (sv_status s = loss_views(c)) return s;
   }
-  if (c->I > c->fixed_cap || !c->fixed) {
-    dfree(c->fixed);
-    c->fixed_cap = 0;
-    if (sv_status s = dalloc(c, &c->fixed, (size_t)std::max<uint64_t>(c->I, 1) * 54)) return s;
-    c->fixed_cap = std::max<uint64_t>(c->I, 1);
-  }
-  RasterArgs a = raster_args(c);
-  a.residual = c->residual;
-  launch_fill_f32(c->st, c->geom, 0.f, (size_t)5 * c->cap);
-  {
-    Scoped t(c, PH_FIXED);
-    launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed);
-    launch_reduce_fixed(c->st, c->fixed, c->vals, c->I, c->cap, c->G, c->geom);
-    launch_geom(c->st, c->P, c->cap, c->N, c->d_views, dev_config(c->cfg), c->geom, c->G, c->grad_accum);
-  }
-  if (sv_status s = check_launch(c, "backward_fixed")) return s;
+  if (sv_status s = backward_fixed_batch(c, flags, true)) return s;
   c->residual_ready[v] = 0;
   return sync(c);
 }
@@ -1131,6 +1139,18 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
   double l1 = 0.0;
   sv_status s;
   if (mode == SV_MODE_PLAIN) {
+    if ((s = zero_grads(c))) return s;
+    for (int i = 0; i < nv; ++i) {
+      const int v = vs[i];
+      if ((s = prepare_batch(c, &v, 1))) return s;
+      if ((s = build_batch(c))) return s;
+      total_isect += c->I;
+      if ((s = forward_batch(c))) return s;
+      if ((s = loss_views(c))) return s;
+      if ((s = backward_fixed_batch(c, c->cfg.plain_wf_flags, true))) return s;
+    }
+    if ((s = adam_planes(c))) return s;
+  } else if (mode == SV_MODE_SEQUENTIAL) {
     if ((s = zero_grads(c))) return s;
     for (int i = 0; i < nv; ++i) {
       const int v = vs[i];
@@ -1250,27 +1270,26 @@ sv_status sv_read_projections(sv_ctx *c, int v, float *out) {
   if (!(c->built && c->slot_views.size() == 1 && c->slot_views[0] == v))
     return fail(c, SV_E_STATE, "tiles of view %d are not the last build", v);
   HIP_OK(c, hipStreamSynchronize(c->st));
-  const size_t N = c->N, rs = rec_stride(c);
+  const size_t N = c->N;
   std::vector<float4> r(4 * N);
-  for (int k = 0; k < 4; ++k)
-    HIP_OK(c, hipMemcpy(r.data() + k * N, c->rec + k * rs, N * sizeof(float4), hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(r.data(), c->rec, 4 * N * sizeof(float4), hipMemcpyDeviceToHost));
   std::vector<float> pos(3 * N), lg(N);
   if (sv_status s = get_params(c, c->P, pos.data(), nullptr, nullptr, lg.data(), nullptr)) return s;
   for (size_t g = 0; g < N; ++g) {
-    const float4 r0 = r[g], r1 = r[N + g], r2 = r[2 * N + g], r3 = r[3 * N + g];
+    const float4 r0 = r[4 * g], r1 = r[4 * g + 1], r3 = r[4 * g + 3];
     float *o = out + 12 * g;
-    if (r2.y == 0.0f) {  // culled: the record is all zero
+    if (r3.w == 0.0f) {  // culled (radius 0): the record is all zero
       for (int k = 0; k < 12; ++k) o[k] = 0.0f;
       continue;
     }
     o[0] = r0.x;
     o[1] = r0.y;
-    o[2] = r2.y;
-    o[3] = r2.z;
+    o[2] = r3.w;
+    o[3] = r3.z;
     o[4] = r0.z;
     o[5] = r0.w;
     o[6] = r1.x;
-    std::memcpy(&o[7], &r3.z, 4);
+    std::memcpy(&o[7], &r3.y, 4);
     o[8] = pos[3 * g];
     o[9] = pos[3 * g + 1];
     o[10] = pos[3 * g + 2];

@@ -13,10 +13,10 @@ namespace sv {
 // Projection + SH colour + tile count for nv views per Gaussian
 // (project_gaussians_for_tiling, training-tiles.wgsl:39-180, with the SH
 // colour of training-forward-tiles.wgsl:65-105 evaluated once per view).
-// rec: 4 float4 per (slot, Gaussian); counts: u32 per (slot, Gaussian).
+// rec: 64-byte record (4 float4) per (slot, Gaussian); counts: u32 per
+// (slot, Gaussian).
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
-                    int nv, const DevConfig &cfg, float4 *rec, size_t rs, uint32_t *counts,
-                    uint32_t *max_dq);
+                    int nv, const DevConfig &cfg, float4 *rec, uint32_t *counts, uint32_t *max_dq);
 // Exclusive scan of u32 counts (M entries) into u32 offsets; total (u64)
 // written to *total. tmp must hold scan_tmp_words(M) u64 words.
 size_t scan_tmp_words(size_t M);
@@ -24,9 +24,8 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
                  uint64_t *total);
 // emit_intersections (training-tiles.wgsl:201-247) for nv slots.
 template <typename K>
-void launch_emit(hipStream_t st, const float4 *rec, size_t rs, const uint32_t *counts,
-                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits,
-                 K *keys, uint32_t *vals);
+void launch_emit(hipStream_t st, float4 *rec, const uint32_t *counts, const uint32_t *offs, uint32_t N,
+                 int nv, const DevView *views, uint32_t dbits, K *keys, uint32_t *vals);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
 // host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
 template <typename K>
@@ -52,8 +51,7 @@ struct RasterArgs {
   const uint32_t *ranges;     // n_tiles_total + 1
   const uint32_t *vals;       // sorted Gaussian ids
   uint64_t n_isect;           // intersections (bound of vals / adjoint records)
-  const float4 *rec;          // per (slot, Gaussian) projection records, 4 planes
-  size_t rs;                  // record plane stride (float4 units)
+  const float4 *rec;          // 64-byte record per (slot, Gaussian)
   uint32_t N;                 // Gaussians (record stride per slot)
   uint32_t n_tiles_total;
   DevConfig cfg;
@@ -69,8 +67,7 @@ struct RasterArgs {
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent queue counter (null: one block per tile)
 };
-void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t raster_px,
-                    uint32_t grid_blocks);
+void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 // backward_tiles_local (backward-tiles-local.wgsl:185-382) for one slot:
 // i32 sums per (intersection, slot of 54).
@@ -88,7 +85,6 @@ struct VjpArgs {
   const uint32_t *counts;  // per (slot, Gaussian)
   const uint32_t *offs;
   const float4 *rec;       // projection records (radius for the importance stats)
-  size_t rs;
   const float *isect_rec;  // 9 floats per intersection
   float4 *importance;      // importance statistics (may be null)
   float *G;                // gradient planes (accumulate), or null when fused
@@ -110,8 +106,11 @@ void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *v
                          size_t cap, float *G, float *geom);
 // training_backward_geom (training-backward-geom.wgsl:36-299) from the
 // per-Gaussian grad_geom planes of one view.
+// Also accumulate_importance (importance-accum.wgsl:19-40) for that view
+// from its records and counts.
 void launch_geom(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *view,
-                 const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum);
+                 const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum,
+                 const float4 *rec, const uint32_t *counts, float4 *importance);
 // Stable compaction of rows by keep flags (exclusive scan of flags).
 void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src, size_t cap_dst,
                     int planes, const uint32_t *keep_offs, const uint8_t *keep, uint32_t N);

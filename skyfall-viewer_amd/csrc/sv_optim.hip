@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     if (cnt == 0u) continue;
     imp.x = imp.x + (float)cnt;
     imp.y = imp.y + 1.0f;
-    imp.z = fmaxf(imp.z, a.rec[2 * a.rs + vg].y);
+    imp.z = fmaxf(imp.z, a.rec[vg * 4 + 3].w);
     any = true;
     const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
     float r[REC9];
@@ -413,9 +413,25 @@ void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *v
 __global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_t cap, uint32_t N,
                                               const DevView *__restrict__ view, DevConfig cfg,
                                               const float *__restrict__ geom, float *__restrict__ G,
-                                              float4 *__restrict__ grad_accum) {
+                                              float4 *__restrict__ grad_accum,
+                                              const float4 *__restrict__ rec,
+                                              const uint32_t *__restrict__ counts,
+                                              float4 *__restrict__ importance) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= N) return;
+  if (importance) {
+    float4 imp = importance[g];
+    imp.x = imp.x * cfg.importance_decay;
+    imp.y = imp.y * cfg.importance_decay;
+    imp.z = imp.z * cfg.importance_decay;
+    const uint32_t cnt = counts[g];
+    if (cnt > 0u) {
+      imp.x = imp.x + (float)cnt;
+      imp.y = imp.y + 1.0f;
+      imp.z = fmaxf(imp.z, rec[(size_t)g * 4 + 3].w);
+    }
+    importance[g] = imp;
+  }
   float prm[11];
 #pragma unroll
   for (int k = 0; k < 11; ++k) {
@@ -443,10 +459,11 @@ __global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_
 }
 
 void launch_geom(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *view,
-                 const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum) {
+                 const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum,
+                 const float4 *rec, const uint32_t *counts, float4 *importance) {
   if (N == 0) return;
   hipLaunchKernelGGL(k_geom, dim3((N + 255) / 256), dim3(256), 0, st, P, cap, N, view, cfg, geom, G,
-                     grad_accum);
+                     grad_accum, rec, counts, importance);
 }
 
 // ---------------------------------------------------------------------

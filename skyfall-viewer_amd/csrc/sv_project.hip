@@ -12,14 +12,20 @@ namespace sv {
 // the value is the same) and the opacity sigmoid (:188).
 //
 // One thread per Gaussian, looping over the views of the step, so the 59
-// parameters are read once per step. Records are four float4 planes of
-// stride rs: R0 = (mean.x, mean.y, conic.a, conic.b), R1 = (conic.c,
-// opacity, r, g), R2 = (b, radius, depth, depth_q bits), R3 = (rect
-// x0|y0<<16, rect x1|y1<<16, tiles_count, 0).
+// parameters are read once per step. Each (view, Gaussian) record is 64
+// bytes (one memory burst) of four float4:
+//   q0 = (mean.x, mean.y, conic.a, conic.b)
+//   q1 = (conic.c, opacity, r, g)
+//   q2 = (b, power_max, rect x0|y0<<16, rect x1|y1<<16)
+//   q3 = (first intersection (set by emit), tiles_count, depth, radius)
+// power_max = ln(255 opacity) + 0.02: any pair with a larger power has
+// opacity * exp(-power) < 1/255 even with the ulp error of exp, so the
+// rasterizers may skip it without evaluating exp (the skip changes no
+// result).
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, size_t cap, uint32_t N,
                                                  const DevView *__restrict__ views, int nv,
-                                                 DevConfig cfg, float4 *__restrict__ rec, size_t rs,
+                                                 DevConfig cfg, float4 *__restrict__ rec,
                                                  uint32_t *__restrict__ counts,
                                                  uint32_t *__restrict__ max_dq) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -130,17 +136,18 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           }
           const uint32_t dq = f2u_sat(clampf(depth, 0.0f, 65535.0f)) & 0xffffu;
           if (count > 0u) local_max_dq = max(local_max_dq, dq);
+          const float pmax = opac > 0.0f ? logf(255.0f * opac) + 0.02f : -1.0f;
           r0 = make_float4(mx, my, aa, bb);
           r1 = make_float4(cc, opac, rgb[0], rgb[1]);
-          r2 = make_float4(rgb[2], radius, depth, __uint_as_float(dq));
-          r3 = make_float4(__uint_as_float(x0 | (y0 << 16)), __uint_as_float(x1 | (y1 << 16)),
-                           __uint_as_float(count), 0.0f);
+          r2 = make_float4(rgb[2], pmax, __uint_as_float(x0 | (y0 << 16)), __uint_as_float(x1 | (y1 << 16)));
+          r3 = make_float4(0.0f, __uint_as_float(count), depth, radius);
         }
       }
-      rec[vg] = r0;
-      rec[rs + vg] = r1;
-      rec[2 * rs + vg] = r2;
-      rec[3 * rs + vg] = r3;
+      float4 *q = rec + vg * 4;
+      q[0] = r0;
+      q[1] = r1;
+      q[2] = r2;
+      q[3] = r3;
       counts[vg] = count;
     }
   }
@@ -159,11 +166,10 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
 }
 
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
-                    int nv, const DevConfig &cfg, float4 *rec, size_t rs, uint32_t *counts,
-                    uint32_t *max_dq) {
+                    int nv, const DevConfig &cfg, float4 *rec, uint32_t *counts, uint32_t *max_dq) {
   if (N == 0) return;
   const uint32_t blocks = (N + 255u) / 256u;
-  hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, rs,
+  hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec,
                      counts, max_dq);
 }
 
@@ -270,7 +276,8 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 }
 
 // ---------------------------------------------------------------------
-// emit_intersections (training-tiles.wgsl:201-247). Key = (global tile <<
+// emit_intersections (training-tiles.wgsl:201-247); also stores the
+// Gaussian's first intersection in its record. Key = (global tile <<
 // dbits) | depth_q with depth_q = u32(clamp(depth, 0, 65535)) & 0xffff
 // (encode_isect_key, :182-199). dbits is the bit width of the largest
 // depth_q of the step, so the key order equals the reference key order
@@ -278,7 +285,7 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // Gaussian index. Order within a Gaussian: ty-major, then tx (:238-246).
 // ---------------------------------------------------------------------
 template <typename K>
-__global__ __launch_bounds__(256) void k_emit(const float4 *__restrict__ rec, size_t rs,
+__global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
                                               const uint32_t *__restrict__ counts,
                                               const uint32_t *__restrict__ offs, uint32_t N, int nv,
                                               const DevView *__restrict__ views, uint32_t dbits,
@@ -289,11 +296,14 @@ __global__ __launch_bounds__(256) void k_emit(const float4 *__restrict__ rec, si
   if (cnt == 0u) return;
   const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
   const DevView &V = views[s];
-  const float4 r2 = rec[2 * rs + vg], r3 = rec[3 * rs + vg];
-  const uint32_t dq = __float_as_uint(r2.w);
-  const uint32_t a = __float_as_uint(r3.x), b = __float_as_uint(r3.y);
+  const float4 r2 = rec[vg * 4 + 2];
+  float4 r3 = rec[vg * 4 + 3];
+  const uint32_t dq = f2u_sat(clampf(r3.z, 0.0f, 65535.0f)) & 0xffffu;
+  const uint32_t a = __float_as_uint(r2.z), b = __float_as_uint(r2.w);
   const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu, y1 = b >> 16;
   uint32_t p = offs[vg];
+  r3.x = __uint_as_float(p);
+  rec[vg * 4 + 3] = r3;
   for (uint32_t ty = y0; ty < y1; ++ty)
     for (uint32_t tx = x0; tx < x1; ++tx) {
       const K tile = (K)(V.tile_base + ty * V.tiles_x + tx);
@@ -304,20 +314,17 @@ __global__ __launch_bounds__(256) void k_emit(const float4 *__restrict__ rec, si
 }
 
 template <typename K>
-void launch_emit(hipStream_t st, const float4 *rec, size_t rs, const uint32_t *counts,
-                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits,
-                 K *keys, uint32_t *vals) {
+void launch_emit(hipStream_t st, float4 *rec, const uint32_t *counts, const uint32_t *offs, uint32_t N,
+                 int nv, const DevView *views, uint32_t dbits, K *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, rs, counts,
-                     offs, N, nv, views, dbits, keys, vals);
+  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, counts, offs,
+                     N, nv, views, dbits, keys, vals);
 }
-template void launch_emit<uint32_t>(hipStream_t, const float4 *, size_t, const uint32_t *,
-                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
-                                    uint32_t *, uint32_t *);
-template void launch_emit<uint64_t>(hipStream_t, const float4 *, size_t, const uint32_t *,
-                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
-                                    uint64_t *, uint32_t *);
+template void launch_emit<uint32_t>(hipStream_t, float4 *, const uint32_t *, const uint32_t *, uint32_t,
+                                    int, const DevView *, uint32_t, uint32_t *, uint32_t *);
+template void launch_emit<uint64_t>(hipStream_t, float4 *, const uint32_t *, const uint32_t *, uint32_t,
+                                    int, const DevView *, uint32_t, uint64_t *, uint32_t *);
 
 // ---------------------------------------------------------------------
 // Tile ranges: ranges[t] = first sorted index whose tile is >= t, which is

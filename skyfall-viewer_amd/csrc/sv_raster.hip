@@ -1,14 +1,17 @@
 // sv_raster.hip -- tiled forward rasterizer and adjoint backward for gfx950.
 //
-// One 256-thread workgroup (four wave64) owns one 16x16 pixel tile of one
-// view; its sorted splat list is staged through LDS in batches of B
-// records. Work items are (view, global tile); with a work counter the
-// grid is persistent and pulls items from a device queue.
+// One wave64 owns one 16x16 pixel tile of one view: lane l shades the
+// pixels (x = l & 15, y = (l >> 4) + 4 j), j = 0..3, so one wave-wide
+// reduction in the backward covers all 256 pixels of the tile. The tile's
+// sorted splat list is staged through LDS in batches of B 64-byte records.
+// Work items are (view, global tile); with a work counter the grid is
+// persistent and pulls items from a device queue.
 #include "sv_internal.hpp"
 
 namespace sv {
 
-constexpr int RT = 256;  // threads per raster workgroup (16 x 16 pixels)
+constexpr int WT = 64;  // threads per tile (one wave)
+constexpr int PPL = 4;  // pixels per lane
 
 __device__ __forceinline__ bool next_tile(const RasterArgs &a, uint32_t &tg, uint32_t *s_tg) {
   if (a.work_counter == nullptr) return false;
@@ -26,6 +29,51 @@ __device__ __forceinline__ uint32_t first_tile(const RasterArgs &a, uint32_t *s_
   return *s_tg;
 }
 
+// DPP move of a float (0 in lanes disabled by the row mask).
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over the 64 lanes; the total is exact in lane 63 (fixed association
+// (R3 + R2) + (R1 + R0) of the row sums; deterministic).
+__device__ __forceinline__ float wave_total63(float v) {
+  v = v + dpp<0xB1>(v);        // quad_perm [1,0,3,2]
+  v = v + dpp<0x4E>(v);        // quad_perm [2,3,0,1]
+  v = v + dpp<0x124>(v);       // row_ror:4
+  v = v + dpp<0x128>(v);       // row_ror:8 -> every lane holds its row sum
+  v = v + dpp<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v = v + dpp<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t m) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  return m;
+}
+
+__device__ __forceinline__ float sign_ref(float x) {
+  return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+}
+
+struct TileGeom {
+  uint32_t s, tx, ty, px, py0;
+  const DevView *V;
+};
+
+__device__ __forceinline__ TileGeom tile_geom(const RasterArgs &a, uint32_t tg) {
+  TileGeom t;
+  t.s = a.tile_view[tg];
+  t.V = &a.views[t.s];
+  const uint32_t lt = tg - t.V->tile_base;
+  t.tx = lt % t.V->tiles_x;
+  t.ty = lt / t.V->tiles_x;
+  t.px = t.tx * 16u + (threadIdx.x & 15u);
+  t.py0 = t.ty * 16u + (threadIdx.x >> 4);
+  return t;
+}
+
 // ---------------------------------------------------------------------
 // training_tiled_forward (training-forward-tiles.wgsl:107-215). Per pixel
 // (centre +0.5) over the tile list front to back: power = 0.5(a dx^2 +
@@ -35,96 +83,107 @@ __device__ __forceinline__ uint32_t first_tile(const RasterArgs &a, uint32_t *s_
 // C + (1 - A) bg. Empty tile: bg, A = 0, last = NO_LAST.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(RT) void k_forward(RasterArgs a) {
-  __shared__ float4 s_x0[B];
-  __shared__ float4 s_x1[B];
-  __shared__ float s_x2[B];
+__global__ __launch_bounds__(WT) void k_forward(RasterArgs a) {
+  __shared__ float4 s_q0[B];
+  __shared__ float4 s_q1[B];
+  __shared__ float2 s_q2[B];
   __shared__ uint32_t s_tg;
   uint32_t tg = first_tile(a, &s_tg);
   while (tg < a.n_tiles_total) {
-    const uint32_t s = a.tile_view[tg];
-    const DevView &V = a.views[s];
-    const uint32_t lt = tg - V.tile_base;
-    const uint32_t tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    const uint32_t px = tx * 16u + (threadIdx.x & 15u), py = ty * 16u + (threadIdx.x >> 4);
-    const bool inside = px < V.W && py < V.H;
+    const TileGeom t = tile_geom(a, tg);
+    const DevView &V = *t.V;
     const size_t P = (size_t)V.W * V.H;
-    const size_t lp = (size_t)py * V.W + px;
-    const size_t pix = V.pix_off + lp;
     float *col = a.color + 3 * V.pix_off;
     const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
+    const bool xin = t.px < V.W;
     if (start >= end) {
-      if (inside) {
-        col[lp] = a.cfg.bg[0];
-        col[P + lp] = a.cfg.bg[1];
-        col[2 * P + lp] = a.cfg.bg[2];
-        a.alpha[pix] = 0.0f;
-        a.last[pix] = NO_LAST;
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        const uint32_t py = t.py0 + 4u * j;
+        if (xin && py < V.H) {
+          const size_t lp = (size_t)py * V.W + t.px;
+          col[lp] = a.cfg.bg[0];
+          col[P + lp] = a.cfg.bg[1];
+          col[2 * P + lp] = a.cfg.bg[2];
+          a.alpha[V.pix_off + lp] = 0.0f;
+          a.last[V.pix_off + lp] = NO_LAST;
+        }
       }
     } else {
-      const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-      const size_t vbase = (size_t)s * a.N;
-      float c0 = 0.f, c1 = 0.f, c2 = 0.f, A = 0.f;
-      uint32_t last = start;
-      bool done = !inside;
+      const float fx = (float)t.px + 0.5f;
+      const size_t vbase = (size_t)t.s * a.N;
+      float c0[PPL], c1[PPL], c2[PPL], A[PPL], fy[PPL];
+      uint32_t last[PPL];
+      bool done[PPL];
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        c0[j] = c1[j] = c2[j] = A[j] = 0.0f;
+        last[j] = start;
+        const uint32_t py = t.py0 + 4u * j;
+        fy[j] = (float)py + 0.5f;
+        done[j] = !(xin && py < V.H);
+      }
       for (uint32_t b0 = start; b0 < end; b0 += B) {
         const uint32_t cnt = min((uint32_t)B, end - b0);
-        for (uint32_t k = threadIdx.x; k < cnt; k += RT) {
-          const size_t vg = vbase + min(a.vals[b0 + k], a.N - 1u);
-          s_x0[k] = a.rec[vg];
-          s_x1[k] = a.rec[a.rs + vg];
-          s_x2[k] = a.rec[2 * a.rs + vg].x;
+        for (uint32_t k = threadIdx.x; k < cnt; k += WT) {
+          const float4 *q = a.rec + (vbase + min(a.vals[b0 + k], a.N - 1u)) * 4;
+          s_q0[k] = q[0];
+          s_q1[k] = q[1];
+          const float4 q2 = q[2];
+          s_q2[k] = make_float2(q2.x, q2.y);
         }
         __syncthreads();
-        if (!done) {
-          for (uint32_t k = 0; k < cnt; ++k) {
-            last = b0 + k;
-            const float4 x0 = s_x0[k];
-            const float4 x1 = s_x1[k];
-            const float dx = fx - x0.x, dy = fy - x0.y;
+        for (uint32_t k = 0; k < cnt; ++k) {
+          const float4 x0 = s_q0[k];
+          const float4 x1 = s_q1[k];
+          const float2 x2 = s_q2[k];
+          const float dx = fx - x0.x;
+#pragma unroll
+          for (int j = 0; j < PPL; ++j) {
+            if (done[j]) continue;
+            last[j] = b0 + k;
+            const float dy = fy[j] - x0.y;
             const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-            if (sg < 0.0f) continue;
+            if (sg < 0.0f || sg > x2.y) continue;
             const float gauss = sv_expf(-sg);
             const float as = fminf(x1.y * gauss, SV_MAX_ALPHA);
             if (as < SV_MIN_ALPHA) continue;
-            const float contrib = as * (1.0f - A);
-            c0 = c0 + x1.z * contrib;
-            c1 = c1 + x1.w * contrib;
-            c2 = c2 + s_x2[k] * contrib;
-            A = A + contrib;
-            if (1.0f - A < SV_T_EPS) {
-              done = true;
-              break;
-            }
+            const float contrib = as * (1.0f - A[j]);
+            c0[j] = c0[j] + x1.z * contrib;
+            c1[j] = c1[j] + x1.w * contrib;
+            c2[j] = c2[j] + x2.x * contrib;
+            A[j] = A[j] + contrib;
+            if (1.0f - A[j] < SV_T_EPS) done[j] = true;
           }
+          if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
         }
-        if (__syncthreads_and(done)) break;
+        if (__syncthreads_and((done[0] && done[1]) && (done[2] && done[3]))) break;
       }
-      if (inside) {
-        const float T = 1.0f - A;
-        col[lp] = c0 + T * a.cfg.bg[0];
-        col[P + lp] = c1 + T * a.cfg.bg[1];
-        col[2 * P + lp] = c2 + T * a.cfg.bg[2];
-        a.alpha[pix] = A;
-        a.last[pix] = last;
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        const uint32_t py = t.py0 + 4u * j;
+        if (xin && py < V.H) {
+          const size_t lp = (size_t)py * V.W + t.px;
+          const float T = 1.0f - A[j];
+          col[lp] = c0[j] + T * a.cfg.bg[0];
+          col[P + lp] = c1[j] + T * a.cfg.bg[1];
+          col[2 * P + lp] = c2[j] + T * a.cfg.bg[2];
+          a.alpha[V.pix_off + lp] = A[j];
+          a.last[V.pix_off + lp] = last[j];
+        }
       }
     }
     if (!next_tile(a, tg, &s_tg)) break;
   }
 }
 
-// Position of an intersection in emit order: the Gaussian's first slot
-// plus its row-major index inside the Gaussian's tile rectangle
-// (training-tiles.wgsl:238-246).
-__device__ __forceinline__ uint32_t emit_pos(const RasterArgs &a, size_t vg, uint32_t tx, uint32_t ty) {
-  const float4 r3 = a.rec[3 * a.rs + vg];
-  const uint32_t u = __float_as_uint(r3.x), w = __float_as_uint(r3.y);
+// Position of an intersection in emit order: the Gaussian's first
+// intersection plus its row-major index inside the Gaussian's tile
+// rectangle (training-tiles.wgsl:238-246).
+__device__ __forceinline__ uint32_t emit_pos(const float4 &q2, const float4 &q3, uint32_t tx, uint32_t ty) {
+  const uint32_t u = __float_as_uint(q2.z), w = __float_as_uint(q2.w);
   const uint32_t x0 = u & 0xffffu, y0 = u >> 16, x1 = w & 0xffffu;
-  return a.offs[vg] + (ty - y0) * (x1 - x0) + (tx - x0);
-}
-
-__device__ __forceinline__ float sign_ref(float x) {
-  return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+  return __float_as_uint(q3.x) + (ty - y0) * (x1 - x0) + (tx - x0);
 }
 
 // ---------------------------------------------------------------------
@@ -132,166 +191,155 @@ __device__ __forceinline__ float sign_ref(float x) {
 // training_tiled_backward (training-backward-tiles.wgsl:318-396): reverse
 // traversal from A_final and the last index, T_before = T / max(1 -
 // alpha, 1e-4), no alpha > 0.98 skip. Instead of per-pixel float CAS
-// atomics into 54 per-Gaussian slots, each wave sums the 9 screen-space
-// quantities (sum of res * alpha * T_before per channel, dL/dlogit,
-// dL/dconic abc, dL/dmean xy) of one splat over its 64 pixels with a
-// fixed butterfly, the four waves are summed in order, and the tile's
-// record is written at the intersection's emit position. The SH chain is
+// atomics into 54 per-Gaussian slots, the wave sums the 9 screen-space
+// quantities of one splat (sum of res * alpha * T_before per channel,
+// dL/dlogit, dL/dconic abc, dL/dmean xy) over the tile's 256 pixels (in
+// registers over each lane's 4 pixels, then a fixed DPP tree) and writes
+// one 36-byte record at the intersection's emit position. The SH chain is
 // applied per Gaussian afterwards (exact: the SH basis depends only on the
 // Gaussian and the view). Deterministic.
-// The residual is either fused (w_l2 (p - t) + w_l1 sign(p - t)) or read
-// from a.residual.
+// The residual is fused (w_l2 (p - t) + w_l1 sign(p - t)) or read from
+// a.residual.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(RT) void k_backward(RasterArgs a) {
-  __shared__ float4 s_x0[B];
-  __shared__ float4 s_x1[B];
-  __shared__ float s_x2[B];
+__global__ __launch_bounds__(WT) void k_backward(RasterArgs a) {
+  __shared__ float4 s_q0[B];
+  __shared__ float4 s_q1[B];
+  __shared__ float2 s_q2[B];
   __shared__ uint32_t s_p[B];
-  __shared__ float s_part[RT / 64][B][REC9];
-  __shared__ uint32_t s_wmax[RT / 64];
-  __shared__ float s_l1[RT / 64];
+  __shared__ float s_out[B * REC9];
   __shared__ uint32_t s_tg;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int lane = threadIdx.x;
   uint32_t tg = first_tile(a, &s_tg);
   while (tg < a.n_tiles_total) {
-    const uint32_t s = a.tile_view[tg];
-    const DevView &V = a.views[s];
-    const uint32_t lt = tg - V.tile_base;
-    const uint32_t tx = lt % V.tiles_x, ty = lt / V.tiles_x;
-    const uint32_t px = tx * 16u + (threadIdx.x & 15u), py = ty * 16u + (threadIdx.x >> 4);
-    const bool inside = px < V.W && py < V.H;
+    const TileGeom t = tile_geom(a, tg);
+    const DevView &V = *t.V;
     const size_t P = (size_t)V.W * V.H;
-    const size_t lp = (size_t)py * V.W + px;
-    const size_t pix = V.pix_off + lp;
     const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
-    const size_t vbase = (size_t)s * a.N;
+    const size_t vbase = (size_t)t.s * a.N;
+    const bool xin = t.px < V.W;
     if (start < end) {
-      float res0 = 0.f, res1 = 0.f, res2 = 0.f, T = 1.0f, l1 = 0.f;
-      uint32_t lastp = 0;
-      bool valid = false;
-      if (inside) {
-        lastp = a.last[pix];
-        T = 1.0f - a.alpha[pix];
-        const float *col = a.color + 3 * V.pix_off;
-        const float *tgt = a.target + 3 * V.pix_off;
-        const float d0 = col[lp] - tgt[lp];
-        const float d1 = col[P + lp] - tgt[P + lp];
-        const float d2 = col[2 * P + lp] - tgt[2 * P + lp];
-        l1 = (fabsf(d0) + fabsf(d1)) + fabsf(d2);
-        if (a.residual) {
-          const float *r = a.residual + 3 * V.pix_off;
-          res0 = r[lp];
-          res1 = r[P + lp];
-          res2 = r[2 * P + lp];
-        } else {
-          res0 = a.cfg.w_l2 * d0 + a.cfg.w_l1 * sign_ref(d0);
-          res1 = a.cfg.w_l2 * d1 + a.cfg.w_l1 * sign_ref(d1);
-          res2 = a.cfg.w_l2 * d2 + a.cfg.w_l1 * sign_ref(d2);
-        }
-        valid = (lastp != NO_LAST) && lastp >= start;
-      }
-      float acc0 = a.cfg.bg[0], acc1 = a.cfg.bg[1], acc2 = a.cfg.bg[2];
-      const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
-      // wave / tile maximum of the last index (+1), and the tile L1 sum
-      uint32_t m = valid ? lastp + 1u : 0u;
-      float l1w = l1;
+      float r0[PPL], r1[PPL], r2[PPL], T[PPL], acc0[PPL], acc1[PPL], acc2[PPL], fy[PPL];
+      uint32_t lastp[PPL];
+      bool valid[PPL];
+      float l1 = 0.0f;
+      uint32_t m = 0;
+      const float *colv = a.color + 3 * V.pix_off;
+      const float *tgt = a.target + 3 * V.pix_off;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
-        l1w = l1w + __shfl_xor(l1w, o, 64);
+      for (int j = 0; j < PPL; ++j) {
+        const uint32_t py = t.py0 + 4u * j;
+        fy[j] = (float)py + 0.5f;
+        r0[j] = r1[j] = r2[j] = 0.0f;
+        T[j] = 1.0f;
+        lastp[j] = 0;
+        valid[j] = false;
+        acc0[j] = a.cfg.bg[0];
+        acc1[j] = a.cfg.bg[1];
+        acc2[j] = a.cfg.bg[2];
+        if (xin && py < V.H) {
+          const size_t lp = (size_t)py * V.W + t.px;
+          lastp[j] = a.last[V.pix_off + lp];
+          T[j] = 1.0f - a.alpha[V.pix_off + lp];
+          const float d0 = colv[lp] - tgt[lp];
+          const float d1 = colv[P + lp] - tgt[P + lp];
+          const float d2 = colv[2 * P + lp] - tgt[2 * P + lp];
+          l1 = l1 + ((fabsf(d0) + fabsf(d1)) + fabsf(d2));
+          if (a.residual) {
+            const float *rr = a.residual + 3 * V.pix_off;
+            r0[j] = rr[lp];
+            r1[j] = rr[P + lp];
+            r2[j] = rr[2 * P + lp];
+          } else {
+            r0[j] = a.cfg.w_l2 * d0 + a.cfg.w_l1 * sign_ref(d0);
+            r1[j] = a.cfg.w_l2 * d1 + a.cfg.w_l1 * sign_ref(d1);
+            r2[j] = a.cfg.w_l2 * d2 + a.cfg.w_l1 * sign_ref(d2);
+          }
+          valid[j] = (lastp[j] != NO_LAST) && lastp[j] >= start;
+          if (valid[j]) m = max(m, lastp[j] + 1u);
+        }
       }
-      if (lane == 0) {
-        s_wmax[wave] = m;
-        s_l1[wave] = l1w;
-      }
-      __syncthreads();
-      const uint32_t wend = s_wmax[wave];  // this wave's entries lie below wend
-      uint32_t tend = max(max(s_wmax[0], s_wmax[1]), max(s_wmax[2], s_wmax[3]));
+      const float l1t = wave_total63(l1);
+      if (lane == 63 && a.l1_partial) a.l1_partial[tg] = l1t;
+      uint32_t tend = wave_max_u32(m);
       if (tend < start) tend = start;
-      if (threadIdx.x == 0 && a.l1_partial)
-        a.l1_partial[tg] = ((s_l1[0] + s_l1[1]) + s_l1[2]) + s_l1[3];
       // entries no pixel reaches get zero records
-      for (uint32_t i = tend + threadIdx.x; i < end; i += RT) {
-        const size_t vg = vbase + min(a.vals[i], a.N - 1u);
-        const uint32_t p = emit_pos(a, vg, tx, ty);
+      for (uint32_t i = tend + lane; i < end; i += WT) {
+        const float4 *q = a.rec + (vbase + min(a.vals[i], a.N - 1u)) * 4;
+        const uint32_t p = emit_pos(q[2], q[3], t.tx, t.ty);
         if ((uint64_t)p < a.n_isect) {
           float *o = a.isect_rec + (size_t)p * REC9;
 #pragma unroll
-          for (int q = 0; q < REC9; ++q) o[q] = 0.0f;
+          for (int q9 = 0; q9 < REC9; ++q9) o[q9] = 0.0f;
         }
       }
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < cnt; k += RT) {
-          const size_t vg = vbase + min(a.vals[lo + k], a.N - 1u);
-          s_x0[k] = a.rec[vg];
-          s_x1[k] = a.rec[a.rs + vg];
-          s_x2[k] = a.rec[2 * a.rs + vg].x;
-          s_p[k] = emit_pos(a, vg, tx, ty);
+        for (uint32_t k = lane; k < cnt; k += WT) {
+          const float4 *q = a.rec + (vbase + min(a.vals[lo + k], a.N - 1u)) * 4;
+          s_q0[k] = q[0];
+          s_q1[k] = q[1];
+          const float4 q2 = q[2];
+          s_q2[k] = make_float2(q2.x, q2.y);
+          s_p[k] = emit_pos(q2, q[3], t.tx, t.ty);
         }
-        for (uint32_t z = threadIdx.x; z < (RT / 64) * (uint32_t)B * REC9; z += RT)
-          (&s_part[0][0][0])[z] = 0.0f;
+        for (uint32_t z = lane; z < cnt * REC9; z += WT) s_out[z] = 0.0f;
         __syncthreads();
-        // entries this wave must visit: indices < wend
-        int kmax = (int)cnt - 1;
-        if (wend <= lo) kmax = -1;
-        else if (wend - lo < cnt) kmax = (int)(wend - lo) - 1;
-        for (int k = kmax; k >= 0; --k) {
+        const float fx = (float)t.px + 0.5f;
+        for (int k = (int)cnt - 1; k >= 0; --k) {
           const uint32_t i = lo + (uint32_t)k;
-          float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f,
-                v8 = 0.f;
+          const float4 x0 = s_q0[k];
+          const float4 x1 = s_q1[k];
+          const float2 x2 = s_q2[k];
+          const float dx = fx - x0.x;
+          float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
           bool hit = false;
-          if (valid && i <= lastp) {
-            const float4 x0 = s_x0[k];
-            const float4 x1 = s_x1[k];
-            const float dx = fx - x0.x, dy = fy - x0.y;
+#pragma unroll
+          for (int j = 0; j < PPL; ++j) {
+            if (!(valid[j] && i <= lastp[j])) continue;
+            const float dy = fy[j] - x0.y;
             const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-            if (sg >= 0.0f) {
-              const float gauss = sv_expf(-sg);
-              const float op = x1.y;
-              const float as = fminf(op * gauss, SV_MAX_ALPHA);
-              if (as >= SV_MIN_ALPHA) {
-                hit = true;
-                const float oma = fmaxf(1.0f - as, 0.0001f);
-                const float Tb = T / oma;
-                const float contrib = as * Tb;
-                v0 = res0 * contrib;
-                v1 = res1 * contrib;
-                v2 = res2 * contrib;
-                const float cr = x1.z, cg = x1.w, cb = s_x2[k];
-                const float fg = (res0 * cr + res1 * cg) + res2 * cb;
-                const float bgt = (res0 * acc0 + res1 * acc1) + res2 * acc2;
-                const float dL_dalpha = (fg - bgt) * Tb;
-                v3 = (dL_dalpha * as) * (1.0f - op);
-                const float dL_dG = dL_dalpha * op;
-                const float dL_ds = dL_dG * (-gauss);
-                v4 = ((dL_ds * 0.5f) * dx) * dx;
-                v5 = (dL_ds * dx) * dy;
-                v6 = ((dL_ds * 0.5f) * dy) * dy;
-                v7 = -(dL_ds * (x0.z * dx + x0.w * dy));
-                v8 = -(dL_ds * (x0.w * dx + x1.x * dy));
-                acc0 = as * cr + (1.0f - as) * acc0;
-                acc1 = as * cg + (1.0f - as) * acc1;
-                acc2 = as * cb + (1.0f - as) * acc2;
-                T = Tb;
-              }
-            }
+            if (sg < 0.0f || sg > x2.y) continue;
+            const float gauss = sv_expf(-sg);
+            const float op = x1.y;
+            const float as = fminf(op * gauss, SV_MAX_ALPHA);
+            if (as < SV_MIN_ALPHA) continue;
+            hit = true;
+            const float oma = fmaxf(1.0f - as, 0.0001f);
+            const float Tb = T[j] / oma;
+            const float contrib = as * Tb;
+            v0 = v0 + r0[j] * contrib;
+            v1 = v1 + r1[j] * contrib;
+            v2 = v2 + r2[j] * contrib;
+            const float cr = x1.z, cg = x1.w, cb = x2.x;
+            const float fg = (r0[j] * cr + r1[j] * cg) + r2[j] * cb;
+            const float bgt = (r0[j] * acc0[j] + r1[j] * acc1[j]) + r2[j] * acc2[j];
+            const float dL_dalpha = (fg - bgt) * Tb;
+            v3 = v3 + (dL_dalpha * as) * (1.0f - op);
+            const float dL_ds = (dL_dalpha * op) * (-gauss);
+            v4 = v4 + ((dL_ds * 0.5f) * dx) * dx;
+            v5 = v5 + (dL_ds * dx) * dy;
+            v6 = v6 + ((dL_ds * 0.5f) * dy) * dy;
+            v7 = v7 - dL_ds * (x0.z * dx + x0.w * dy);
+            v8 = v8 - dL_ds * (x0.w * dx + x1.x * dy);
+            acc0[j] = as * cr + (1.0f - as) * acc0[j];
+            acc1[j] = as * cg + (1.0f - as) * acc1[j];
+            acc2[j] = as * cb + (1.0f - as) * acc2[j];
+            T[j] = Tb;
           }
-          if (__ballot(hit) == 0ull) continue;
-          v0 = wave_sum(v0);
-          v1 = wave_sum(v1);
-          v2 = wave_sum(v2);
-          v3 = wave_sum(v3);
-          v4 = wave_sum(v4);
-          v5 = wave_sum(v5);
-          v6 = wave_sum(v6);
-          v7 = wave_sum(v7);
-          v8 = wave_sum(v8);
-          if (lane == 0) {
-            float *o = &s_part[wave][k][0];
+          if (!__any(hit)) continue;
+          v0 = wave_total63(v0);
+          v1 = wave_total63(v1);
+          v2 = wave_total63(v2);
+          v3 = wave_total63(v3);
+          v4 = wave_total63(v4);
+          v5 = wave_total63(v5);
+          v6 = wave_total63(v6);
+          v7 = wave_total63(v7);
+          v8 = wave_total63(v8);
+          if (lane == 63) {
+            float *o = &s_out[k * REC9];
             o[0] = v0;
             o[1] = v1;
             o[2] = v2;
@@ -304,17 +352,15 @@ __global__ __launch_bounds__(RT) void k_backward(RasterArgs a) {
           }
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < cnt; k += RT) {
-          if ((uint64_t)s_p[k] >= a.n_isect) continue;
-          float *o = a.isect_rec + (size_t)s_p[k] * REC9;
-#pragma unroll
-          for (int q = 0; q < REC9; ++q)
-            o[q] = ((s_part[0][k][q] + s_part[1][k][q]) + s_part[2][k][q]) + s_part[3][k][q];
+        for (uint32_t z = lane; z < cnt * REC9; z += WT) {
+          const uint32_t k = z / REC9, q9 = z - k * REC9;
+          const uint32_t p = s_p[k];
+          if ((uint64_t)p < a.n_isect) a.isect_rec[(size_t)p * REC9 + q9] = s_out[z];
         }
+        __syncthreads();
         hi = lo;
       }
-    } else if (threadIdx.x == 0 && a.l1_partial) {
-      // empty tile: the loss still sees bg vs target
+    } else if (lane == 0 && a.l1_partial) {
       a.l1_partial[tg] = 0.0f;
     }
     if (!next_tile(a, tg, &s_tg)) break;
@@ -322,14 +368,16 @@ __global__ __launch_bounds__(RT) void k_backward(RasterArgs a) {
 }
 
 // ---------------------------------------------------------------------
-// backward_tiles_local (backward-tiles-local.wgsl:185-382): reference
-// wavefront backward. 64-entry batches; per (pixel, splat) the 54 values
-// (logit, conic abc, mean xy, 48 SH) are converted with i32(v * 65536)
-// (truncating, saturating) and added with LDS integer atomics; the i32
-// sums are written per intersection. flags: SV_WF_REFERENCE_ORDER walks
-// batches front-to-back (:262), else back-to-front (order-correct);
-// SV_WF_SKIP_098 keeps the alpha > 0.98 skip (:329-331). Tile size 16.
+// backward_tiles_local (backward-tiles-local.wgsl:185-382): the reference
+// wavefront backward, one 256-thread workgroup per 16x16 tile as in the
+// shader. 64-entry batches; per (pixel, splat) the 54 values (logit, conic
+// abc, mean xy, 48 SH) are converted with i32(v * 65536) (truncating,
+// saturating) and added with LDS integer atomics; the i32 sums are
+// written per intersection. flags: WF_REFERENCE_ORDER walks batches
+// front-to-back (:262), else back-to-front (order-correct); WF_SKIP_098
+// keeps the alpha > 0.98 skip (:329-331).
 // ---------------------------------------------------------------------
+constexpr int RT = 256;
 constexpr int WF_B = 64;
 constexpr int WF_S = 54;
 
@@ -381,10 +429,10 @@ __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs a, const float
     for (int z = threadIdx.x; z < WF_B * WF_S; z += RT) s_acc[z] = 0;
     if (threadIdx.x < bsz) {
       const uint32_t g = min(a.vals[bs + threadIdx.x], a.N - 1u);
-      const size_t vg = vbase + g;
-      s_x0[threadIdx.x] = a.rec[vg];
-      s_x1[threadIdx.x] = a.rec[a.rs + vg];
-      s_x2[threadIdx.x] = a.rec[2 * a.rs + vg].x;
+      const float4 *q = a.rec + (vbase + g) * 4;
+      s_x0[threadIdx.x] = q[0];
+      s_x1[threadIdx.x] = q[1];
+      s_x2[threadIdx.x] = q[2].x;
       float wp[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -447,35 +495,23 @@ __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs a, const float
       }
     }
     __syncthreads();
-    for (uint32_t z = threadIdx.x; z < bsz * WF_S; z += RT)
-      fixed[(size_t)bs * WF_S + z] = s_acc[z];
+    for (uint32_t z = threadIdx.x; z < bsz * WF_S; z += RT) fixed[(size_t)bs * WF_S + z] = s_acc[z];
     __syncthreads();
   }
 }
 
-template <int B>
-static void launch_fwd_b(hipStream_t st, const RasterArgs &a, uint32_t grid) {
-  hipLaunchKernelGGL(k_forward<B>, dim3(grid), dim3(RT), 0, st, a);
-}
-template <int B>
-static void launch_bwd_b(hipStream_t st, const RasterArgs &a, uint32_t grid) {
-  hipLaunchKernelGGL(k_backward<B>, dim3(grid), dim3(RT), 0, st, a);
-}
-
-void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t raster_px,
-                    uint32_t grid) {
-  (void)raster_px;
+void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid) {
   if (a.n_tiles_total == 0 || grid == 0) return;
-  if (batch <= 64) launch_fwd_b<64>(st, a, grid);
-  else if (batch <= 128) launch_fwd_b<128>(st, a, grid);
-  else launch_fwd_b<256>(st, a, grid);
+  if (batch <= 64) hipLaunchKernelGGL(k_forward<64>, dim3(grid), dim3(WT), 0, st, a);
+  else if (batch <= 128) hipLaunchKernelGGL(k_forward<128>, dim3(grid), dim3(WT), 0, st, a);
+  else hipLaunchKernelGGL(k_forward<256>, dim3(grid), dim3(WT), 0, st, a);
 }
 
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid) {
   if (a.n_tiles_total == 0 || grid == 0) return;
-  if (batch <= 64) launch_bwd_b<64>(st, a, grid);
-  else if (batch <= 128) launch_bwd_b<128>(st, a, grid);
-  else launch_bwd_b<256>(st, a, grid);
+  if (batch <= 64) hipLaunchKernelGGL(k_backward<64>, dim3(grid), dim3(WT), 0, st, a);
+  else if (batch <= 128) hipLaunchKernelGGL(k_backward<128>, dim3(grid), dim3(WT), 0, st, a);
+  else hipLaunchKernelGGL(k_backward<256>, dim3(grid), dim3(WT), 0, st, a);
 }
 
 void launch_backward_fixed(hipStream_t st, const RasterArgs &a, const float *P, size_t cap,

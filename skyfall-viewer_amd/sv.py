@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsv.so")
 
 SV_OK, SV_E_ARG, SV_E_CAPACITY, SV_E_HIP, SV_E_STATE, SV_E_NOMEM = range(6)
-MODE_PLAIN, MODE_BATCHED = 0, 1
+MODE_PLAIN, MODE_BATCHED, MODE_SEQUENTIAL = 0, 1, 2
 WF_REFERENCE_ORDER, WF_SKIP_098 = 0x1, 0x2
 NO_LAST = 0xFFFFFFFF
 
@@ -37,6 +37,7 @@ class Config(C.Structure):
         ("sh_update_interval", C.c_uint32), ("pos_lr_schedule", C.c_uint32),
         ("render_f16", C.c_uint32), ("importance_decay", C.c_float),
         ("persistent", C.c_uint32), ("use_graph", C.c_uint32), ("keep_grads", C.c_uint32),
+        ("plain_wf_flags", C.c_uint32),
     ]
 
 
