@@ -99,6 +99,7 @@ struct sv_ctx {
   float *l1_partial = nullptr;
   size_t tiles_cap = 0;
   float4 *rec = nullptr;
+  float2 *aux = nullptr;
   uint32_t *counts = nullptr, *offs = nullptr;
   size_t vg_cap = 0;
   uint64_t *scan_tmp = nullptr;
@@ -330,10 +331,12 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
   const size_t M = (size_t)nv * c->N;
   if (M > c->vg_cap || !c->rec) {
     dfree(c->rec);
+    dfree(c->aux);
     dfree(c->counts);
     dfree(c->offs);
     c->vg_cap = 0;
     if (sv_status s = dalloc(c, &c->rec, 4 * M)) return s;
+    if (sv_status s = dalloc(c, &c->aux, M)) return s;
     if (sv_status s = dalloc(c, &c->counts, M)) return s;
     if (sv_status s = dalloc(c, &c->offs, M)) return s;
     c->vg_cap = M;
@@ -352,7 +355,7 @@ sv_status build_batch(sv_ctx *c) {
   launch_fill_u32(c->st, (uint32_t *)c->d_small, 0u, 6);
   {
     Scoped t(c, PH_PROJECT);
-    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->counts,
+    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts,
                    (uint32_t *)(c->d_small + 1));
   }
   if (sv_status s = check_launch(c, "project")) return s;
@@ -392,10 +395,10 @@ sv_status build_batch(sv_ctx *c) {
   {
     Scoped t(c, PH_EMIT);
     if (c->key64)
-      launch_emit<uint64_t>(c->st, c->rec, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+      launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
                             (uint64_t *)c->keys, c->vals);
     else
-      launch_emit<uint32_t>(c->st, c->rec, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+      launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
                             (uint32_t *)c->keys, c->vals);
   }
   if (sv_status s = check_launch(c, "emit")) return s;
@@ -522,6 +525,7 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.counts = c->counts;
   v.offs = c->offs;
   v.rec = c->rec;
+  v.aux = c->aux;
   v.isect_rec = c->isect_rec;
   v.importance = c->importance;
   v.G = c->G;
@@ -673,7 +677,7 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
     launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed);
     launch_reduce_fixed(c->st, c->fixed, c->vals, c->I, c->cap, c->G, c->geom);
     launch_geom(c->st, c->P, c->cap, c->N, c->d_views, dev_config(c->cfg), c->geom, c->G,
-                stats ? c->grad_accum : nullptr, c->rec, c->counts, stats ? c->importance : nullptr);
+                stats ? c->grad_accum : nullptr, c->aux, c->counts, stats ? c->importance : nullptr);
   }
   return check_launch(c, "backward_fixed");
 }
@@ -905,6 +909,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->ranges);
   dfree(c->l1_partial);
   dfree(c->rec);
+  dfree(c->aux);
   dfree(c->counts);
   dfree(c->offs);
   dfree(c->scan_tmp);
@@ -1272,24 +1277,26 @@ sv_status sv_read_projections(sv_ctx *c, int v, float *out) {
   HIP_OK(c, hipStreamSynchronize(c->st));
   const size_t N = c->N;
   std::vector<float4> r(4 * N);
+  std::vector<float2> ax(N);
   HIP_OK(c, hipMemcpy(r.data(), c->rec, 4 * N * sizeof(float4), hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(ax.data(), c->aux, N * sizeof(float2), hipMemcpyDeviceToHost));
   std::vector<float> pos(3 * N), lg(N);
   if (sv_status s = get_params(c, c->P, pos.data(), nullptr, nullptr, lg.data(), nullptr)) return s;
   for (size_t g = 0; g < N; ++g) {
     const float4 r0 = r[4 * g], r1 = r[4 * g + 1], r3 = r[4 * g + 3];
     float *o = out + 12 * g;
-    if (r3.w == 0.0f) {  // culled (radius 0): the record is all zero
+    if (ax[g].y == 0.0f) {  // culled (radius 0): the record is all zero
       for (int k = 0; k < 12; ++k) o[k] = 0.0f;
       continue;
     }
     o[0] = r0.x;
     o[1] = r0.y;
-    o[2] = r3.w;
-    o[3] = r3.z;
+    o[2] = ax[g].y;
+    o[3] = ax[g].x;
     o[4] = r0.z;
     o[5] = r0.w;
     o[6] = r1.x;
-    std::memcpy(&o[7], &r3.y, 4);
+    std::memcpy(&o[7], &r3.w, 4);
     o[8] = pos[3 * g];
     o[9] = pos[3 * g + 1];
     o[10] = pos[3 * g + 2];

@@ -110,6 +110,25 @@ __device__ __forceinline__ float sv_expf(float x) {
   return p * __uint_as_float((uint32_t)(ni + 127) << 23);
 }
 
+// sv_expf for x in [-87, 0] (the rasterizers' argument -power, with power
+// in [0, power_max] and power_max < 6): the same operations as sv_expf
+// without its range branches, so the result is the same float.
+__device__ __forceinline__ float sv_expf_neg(float x) {
+  const float n = rintf(x * 1.44269504088896341f);
+  float r = fmaf(-n, 0.693359375f, x);
+  r = fmaf(-n, -2.12194440e-4f, r);
+  float p = 1.9875691500e-4f;
+  p = fmaf(p, r, 1.3981999507e-3f);
+  p = fmaf(p, r, 8.3334519073e-3f);
+  p = fmaf(p, r, 4.1665795894e-2f);
+  p = fmaf(p, r, 1.6666665459e-1f);
+  p = fmaf(p, r, 5.0000001201e-1f);
+  const float r2 = r * r;
+  p = fmaf(p, r2, r);
+  p = p + 1.0f;
+  return p * __uint_as_float((uint32_t)((int)n + 127) << 23);
+}
+
 // Logistic function as written at training-forward-tiles.wgsl:188.
 __device__ __forceinline__ float sv_sigmoid(float logit) {
   return 1.0f / (1.0f + sv_expf(-logit));

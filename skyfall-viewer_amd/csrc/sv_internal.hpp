@@ -13,10 +13,11 @@ namespace sv {
 // Projection + SH colour + tile count for nv views per Gaussian
 // (project_gaussians_for_tiling, training-tiles.wgsl:39-180, with the SH
 // colour of training-forward-tiles.wgsl:65-105 evaluated once per view).
-// rec: 64-byte record (4 float4) per (slot, Gaussian); counts: u32 per
-// (slot, Gaussian).
+// rec: 64-byte record (4 float4) per (slot, Gaussian); aux: (depth,
+// radius); counts: u32 per (slot, Gaussian).
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
-                    int nv, const DevConfig &cfg, float4 *rec, uint32_t *counts, uint32_t *max_dq);
+                    int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
+                    uint32_t *max_dq);
 // Exclusive scan of u32 counts (M entries) into u32 offsets; total (u64)
 // written to *total. tmp must hold scan_tmp_words(M) u64 words.
 size_t scan_tmp_words(size_t M);
@@ -24,8 +25,9 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
                  uint64_t *total);
 // emit_intersections (training-tiles.wgsl:201-247) for nv slots.
 template <typename K>
-void launch_emit(hipStream_t st, float4 *rec, const uint32_t *counts, const uint32_t *offs, uint32_t N,
-                 int nv, const DevView *views, uint32_t dbits, K *keys, uint32_t *vals);
+void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits, K *keys,
+                 uint32_t *vals);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
 // host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
 template <typename K>
@@ -84,7 +86,8 @@ struct VjpArgs {
   DevConfig cfg;
   const uint32_t *counts;  // per (slot, Gaussian)
   const uint32_t *offs;
-  const float4 *rec;       // projection records (radius for the importance stats)
+  const float4 *rec;       // projection records
+  const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
   const float *isect_rec;  // 9 floats per intersection
   float4 *importance;      // importance statistics (may be null)
   float *G;                // gradient planes (accumulate), or null when fused
@@ -110,7 +113,7 @@ void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *v
 // from its records and counts.
 void launch_geom(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *view,
                  const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum,
-                 const float4 *rec, const uint32_t *counts, float4 *importance);
+                 const float2 *aux, const uint32_t *counts, float4 *importance);
 // Stable compaction of rows by keep flags (exclusive scan of flags).
 void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src, size_t cap_dst,
                     int planes, const uint32_t *keep_offs, const uint8_t *keep, uint32_t N);

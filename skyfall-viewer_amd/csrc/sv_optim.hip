@@ -260,7 +260,7 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     if (cnt == 0u) continue;
     imp.x = imp.x + (float)cnt;
     imp.y = imp.y + 1.0f;
-    imp.z = fmaxf(imp.z, a.rec[vg * 4 + 3].w);
+    imp.z = fmaxf(imp.z, a.aux[vg].y);
     any = true;
     const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
     float r[REC9];
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_
                                               const DevView *__restrict__ view, DevConfig cfg,
                                               const float *__restrict__ geom, float *__restrict__ G,
                                               float4 *__restrict__ grad_accum,
-                                              const float4 *__restrict__ rec,
+                                              const float2 *__restrict__ aux,
                                               const uint32_t *__restrict__ counts,
                                               float4 *__restrict__ importance) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_
     if (cnt > 0u) {
       imp.x = imp.x + (float)cnt;
       imp.y = imp.y + 1.0f;
-      imp.z = fmaxf(imp.z, rec[(size_t)g * 4 + 3].w);
+      imp.z = fmaxf(imp.z, aux[g].y);
     }
     importance[g] = imp;
   }
@@ -460,10 +460,10 @@ __global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_
 
 void launch_geom(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *view,
                  const DevConfig &cfg, const float *geom, float *G, float4 *grad_accum,
-                 const float4 *rec, const uint32_t *counts, float4 *importance) {
+                 const float2 *aux, const uint32_t *counts, float4 *importance) {
   if (N == 0) return;
   hipLaunchKernelGGL(k_geom, dim3((N + 255) / 256), dim3(256), 0, st, P, cap, N, view, cfg, geom, G,
-                     grad_accum, rec, counts, importance);
+                     grad_accum, aux, counts, importance);
 }
 
 // ---------------------------------------------------------------------

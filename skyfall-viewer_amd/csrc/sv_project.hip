@@ -16,16 +16,21 @@ namespace sv {
 // bytes (one memory burst) of four float4:
 //   q0 = (mean.x, mean.y, conic.a, conic.b)
 //   q1 = (conic.c, opacity, r, g)
-//   q2 = (b, power_max, rect x0|y0<<16, rect x1|y1<<16)
-//   q3 = (first intersection (set by emit), tiles_count, depth, radius)
-// power_max = ln(255 opacity) + 0.02: any pair with a larger power has
-// opacity * exp(-power) < 1/255 even with the ulp error of exp, so the
-// rasterizers may skip it without evaluating exp (the skip changes no
-// result).
+//   q2 = (b, power_max, ext_x, ext_y)
+//   q3 = (first intersection (set by emit), rect x0|y0<<16, rect x1|y1<<16,
+//         tiles_count)
+// and aux = (depth, radius). power_max = ln(255 opacity) + 0.02: a pair with
+// a larger power has opacity * exp(-power) < 1/255 even with the ulp error
+// of exp, so it would be skipped. ext_x, ext_y bound the region power <=
+// power_max: max |dx| = sqrt(2 power_max (Q^-1)_xx) for the conic Q, with a
+// 1% + 0.01 px margin for rounding. The rasterizers use both to skip pairs
+// without evaluating them; every skipped pair is one the reference skips,
+// so no result changes.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, size_t cap, uint32_t N,
                                                  const DevView *__restrict__ views, int nv,
                                                  DevConfig cfg, float4 *__restrict__ rec,
+                                                 float2 *__restrict__ aux,
                                                  uint32_t *__restrict__ counts,
                                                  uint32_t *__restrict__ max_dq) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,6 +74,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
       const float depth = pc[2];
       if (depth < cfg.near_plane || depth > cfg.far_plane) alive = false;
       float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
+      float2 ax = make_float2(0.f, 0.f);
       uint32_t count = 0;
       if (alive) {
         const float tx = pc[0], ty = pc[1], tz = pc[2];
@@ -137,10 +143,18 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           const uint32_t dq = f2u_sat(clampf(depth, 0.0f, 65535.0f)) & 0xffffu;
           if (count > 0u) local_max_dq = max(local_max_dq, dq);
           const float pmax = opac > 0.0f ? logf(255.0f * opac) + 0.02f : -1.0f;
+          const float qdet = aa * cc - bb * bb;
+          float ex = -1.0f, ey = -1.0f;
+          if (pmax > 0.0f && qdet > 0.0f) {
+            ex = sqrtf(2.0f * pmax * cc / qdet) * 1.01f + 0.01f;
+            ey = sqrtf(2.0f * pmax * aa / qdet) * 1.01f + 0.01f;
+          }
           r0 = make_float4(mx, my, aa, bb);
           r1 = make_float4(cc, opac, rgb[0], rgb[1]);
-          r2 = make_float4(rgb[2], pmax, __uint_as_float(x0 | (y0 << 16)), __uint_as_float(x1 | (y1 << 16)));
-          r3 = make_float4(0.0f, __uint_as_float(count), depth, radius);
+          r2 = make_float4(rgb[2], pmax, ex, ey);
+          r3 = make_float4(0.0f, __uint_as_float(x0 | (y0 << 16)), __uint_as_float(x1 | (y1 << 16)),
+                           __uint_as_float(count));
+          ax = make_float2(depth, radius);
         }
       }
       float4 *q = rec + vg * 4;
@@ -148,6 +162,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
       q[1] = r1;
       q[2] = r2;
       q[3] = r3;
+      aux[vg] = ax;
       counts[vg] = count;
     }
   }
@@ -166,10 +181,11 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
 }
 
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
-                    int nv, const DevConfig &cfg, float4 *rec, uint32_t *counts, uint32_t *max_dq) {
+                    int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
+                    uint32_t *max_dq) {
   if (N == 0) return;
   const uint32_t blocks = (N + 255u) / 256u;
-  hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec,
+  hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, aux,
                      counts, max_dq);
 }
 
@@ -286,6 +302,7 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // ---------------------------------------------------------------------
 template <typename K>
 __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
+                                              const float2 *__restrict__ aux,
                                               const uint32_t *__restrict__ counts,
                                               const uint32_t *__restrict__ offs, uint32_t N, int nv,
                                               const DevView *__restrict__ views, uint32_t dbits,
@@ -296,10 +313,9 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
   if (cnt == 0u) return;
   const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
   const DevView &V = views[s];
-  const float4 r2 = rec[vg * 4 + 2];
   float4 r3 = rec[vg * 4 + 3];
-  const uint32_t dq = f2u_sat(clampf(r3.z, 0.0f, 65535.0f)) & 0xffffu;
-  const uint32_t a = __float_as_uint(r2.z), b = __float_as_uint(r2.w);
+  const uint32_t dq = f2u_sat(clampf(aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
+  const uint32_t a = __float_as_uint(r3.y), b = __float_as_uint(r3.z);
   const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu, y1 = b >> 16;
   uint32_t p = offs[vg];
   r3.x = __uint_as_float(p);
@@ -314,17 +330,20 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
 }
 
 template <typename K>
-void launch_emit(hipStream_t st, float4 *rec, const uint32_t *counts, const uint32_t *offs, uint32_t N,
-                 int nv, const DevView *views, uint32_t dbits, K *keys, uint32_t *vals) {
+void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits, K *keys,
+                 uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, counts, offs,
-                     N, nv, views, dbits, keys, vals);
+  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, aux, counts,
+                     offs, N, nv, views, dbits, keys, vals);
 }
-template void launch_emit<uint32_t>(hipStream_t, float4 *, const uint32_t *, const uint32_t *, uint32_t,
-                                    int, const DevView *, uint32_t, uint32_t *, uint32_t *);
-template void launch_emit<uint64_t>(hipStream_t, float4 *, const uint32_t *, const uint32_t *, uint32_t,
-                                    int, const DevView *, uint32_t, uint64_t *, uint32_t *);
+template void launch_emit<uint32_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
+                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
+                                    uint32_t *, uint32_t *);
+template void launch_emit<uint64_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
+                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
+                                    uint64_t *, uint32_t *);
 
 // ---------------------------------------------------------------------
 // Tile ranges: ranges[t] = first sorted index whose tile is >= t, which is

@@ -35,6 +35,16 @@ __device__ __forceinline__ float dpp(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
 }
 
+// Sum over each row of 16 lanes (fixed association; every lane of the row
+// receives the row sum).
+__device__ __forceinline__ float row_total(float v) {
+  v = v + dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = v + dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = v + dpp<0x124>(v);  // row_ror:4
+  v = v + dpp<0x128>(v);  // row_ror:8
+  return v;
+}
+
 // Sum over the 64 lanes; the total is exact in lane 63 (fixed association
 // (R3 + R2) + (R1 + R0) of the row sums; deterministic).
 __device__ __forceinline__ float wave_total63(float v) {
@@ -74,19 +84,72 @@ __device__ __forceinline__ TileGeom tile_geom(const RasterArgs &a, uint32_t tg) 
   return t;
 }
 
+// Position of an intersection in emit order: the Gaussian's first
+// intersection plus its row-major index inside the Gaussian's tile
+// rectangle (training-tiles.wgsl:238-246).
+__device__ __forceinline__ uint32_t emit_pos(const float4 &q3, uint32_t tx, uint32_t ty) {
+  const uint32_t u = __float_as_uint(q3.y), w = __float_as_uint(q3.z);
+  const uint32_t x0 = u & 0xffffu, y0 = u >> 16, x1 = w & 0xffffu;
+  return __float_as_uint(q3.x) + (ty - y0) * (x1 - x0) + (tx - x0);
+}
+
+// Load up to B entries of a tile list from `first` (count cnt) into LDS,
+// keeping only those whose power <= power_max region (axis-aligned bounds
+// ext_x, ext_y around the mean) overlaps the tile's pixel centres; kept
+// entries stay in list order. Returns how many were kept.
+template <int B>
+__device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
+                                               uint32_t cnt, float tx0, float tx1, float ty0, float ty1,
+                                               float4 *s_q0, float4 *s_q1, float4 *s_q2, uint32_t *s_k,
+                                               uint32_t *s_p, uint32_t tx, uint32_t ty) {
+  const int lane = threadIdx.x;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t kept = 0;
+  for (uint32_t r = 0; r < cnt; r += WT) {
+    const uint32_t k = r + lane;
+    bool touch = false;
+    float4 q0, q1, q2, q3;
+    if (k < cnt) {
+      const float4 *q = a.rec + (vbase + min(a.vals[first + k], a.N - 1u)) * 4;
+      q0 = q[0];
+      q1 = q[1];
+      q2 = q[2];
+      if (s_p) {
+        q3 = q[3];
+        s_p[k] = emit_pos(q3, tx, ty);
+      }
+      const float ddx = fabsf(q0.x - fminf(fmaxf(q0.x, tx0), tx1));
+      const float ddy = fabsf(q0.y - fminf(fmaxf(q0.y, ty0), ty1));
+      touch = ddx <= q2.z && ddy <= q2.w;
+    }
+    const uint64_t m = __ballot(touch);
+    if (touch) {
+      const uint32_t pos = kept + (uint32_t)__popcll(m & lt_mask);
+      s_q0[pos] = q0;
+      s_q1[pos] = q1;
+      s_q2[pos] = q2;
+      s_k[pos] = k;
+    }
+    kept += (uint32_t)__popcll(m);
+  }
+  return kept;
+}
+
 // ---------------------------------------------------------------------
 // training_tiled_forward (training-forward-tiles.wgsl:107-215). Per pixel
 // (centre +0.5) over the tile list front to back: power = 0.5(a dx^2 +
 // c dy^2) + b dx dy, skipped if < 0; alpha = min(sigmoid(logit) e^-power,
 // 0.999), skipped if < 1/255; C += c alpha (1 - A); A += alpha (1 - A);
-// stop when 1 - A < 1e-4 after the add. last = last visited index. Output
-// C + (1 - A) bg. Empty tile: bg, A = 0, last = NO_LAST.
+// stop when 1 - A < 1e-4 after the add. last = last visited index (the
+// saturating entry, else the end of the list). Output C + (1 - A) bg.
+// Empty tile: bg, A = 0, last = NO_LAST.
 // ---------------------------------------------------------------------
 template <int B>
 __global__ __launch_bounds__(WT) void k_forward(RasterArgs a) {
   __shared__ float4 s_q0[B];
   __shared__ float4 s_q1[B];
-  __shared__ float2 s_q2[B];
+  __shared__ float4 s_q2[B];
+  __shared__ uint32_t s_k[B];
   __shared__ uint32_t s_tg;
   uint32_t tg = first_tile(a, &s_tg);
   while (tg < a.n_tiles_total) {
@@ -112,48 +175,50 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs a) {
     } else {
       const float fx = (float)t.px + 0.5f;
       const size_t vbase = (size_t)t.s * a.N;
+      const float tx0 = (float)(t.tx * 16u) + 0.5f, tx1 = (float)min(t.tx * 16u + 15u, V.W - 1u) + 0.5f;
+      const float ty0 = (float)(t.ty * 16u) + 0.5f, ty1 = (float)min(t.ty * 16u + 15u, V.H - 1u) + 0.5f;
       float c0[PPL], c1[PPL], c2[PPL], A[PPL], fy[PPL];
       uint32_t last[PPL];
       bool done[PPL];
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         c0[j] = c1[j] = c2[j] = A[j] = 0.0f;
-        last[j] = start;
+        last[j] = end - 1u;
         const uint32_t py = t.py0 + 4u * j;
         fy[j] = (float)py + 0.5f;
         done[j] = !(xin && py < V.H);
       }
       for (uint32_t b0 = start; b0 < end; b0 += B) {
         const uint32_t cnt = min((uint32_t)B, end - b0);
-        for (uint32_t k = threadIdx.x; k < cnt; k += WT) {
-          const float4 *q = a.rec + (vbase + min(a.vals[b0 + k], a.N - 1u)) * 4;
-          s_q0[k] = q[0];
-          s_q1[k] = q[1];
-          const float4 q2 = q[2];
-          s_q2[k] = make_float2(q2.x, q2.y);
-        }
+        const uint32_t kept =
+            load_batch<B>(a, vbase, b0, cnt, tx0, tx1, ty0, ty1, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0);
         __syncthreads();
-        for (uint32_t k = 0; k < cnt; ++k) {
-          const float4 x0 = s_q0[k];
-          const float4 x1 = s_q1[k];
-          const float2 x2 = s_q2[k];
+        for (uint32_t e = 0; e < kept; ++e) {
+          const float4 x0 = s_q0[e];
+          const float4 x1 = s_q1[e];
+          const float4 x2 = s_q2[e];
           const float dx = fx - x0.x;
+          if (fabsf(dx) <= x2.z) {
 #pragma unroll
-          for (int j = 0; j < PPL; ++j) {
-            if (done[j]) continue;
-            last[j] = b0 + k;
-            const float dy = fy[j] - x0.y;
-            const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-            if (sg < 0.0f || sg > x2.y) continue;
-            const float gauss = sv_expf(-sg);
-            const float as = fminf(x1.y * gauss, SV_MAX_ALPHA);
-            if (as < SV_MIN_ALPHA) continue;
-            const float contrib = as * (1.0f - A[j]);
-            c0[j] = c0[j] + x1.z * contrib;
-            c1[j] = c1[j] + x1.w * contrib;
-            c2[j] = c2[j] + x2.x * contrib;
-            A[j] = A[j] + contrib;
-            if (1.0f - A[j] < SV_T_EPS) done[j] = true;
+            for (int j = 0; j < PPL; ++j) {
+              if (done[j]) continue;
+              const float dy = fy[j] - x0.y;
+              if (fabsf(dy) > x2.w) continue;
+              const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+              if (sg < 0.0f || sg > x2.y) continue;
+              const float gauss = sv_expf_neg(-sg);
+              const float as = fminf(x1.y * gauss, SV_MAX_ALPHA);
+              if (as < SV_MIN_ALPHA) continue;
+              const float contrib = as * (1.0f - A[j]);
+              c0[j] = c0[j] + x1.z * contrib;
+              c1[j] = c1[j] + x1.w * contrib;
+              c2[j] = c2[j] + x2.x * contrib;
+              A[j] = A[j] + contrib;
+              if (1.0f - A[j] < SV_T_EPS) {
+                done[j] = true;
+                last[j] = b0 + s_k[e];
+              }
+            }
           }
           if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
         }
@@ -177,14 +242,6 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs a) {
   }
 }
 
-// Position of an intersection in emit order: the Gaussian's first
-// intersection plus its row-major index inside the Gaussian's tile
-// rectangle (training-tiles.wgsl:238-246).
-__device__ __forceinline__ uint32_t emit_pos(const float4 &q2, const float4 &q3, uint32_t tx, uint32_t ty) {
-  const uint32_t u = __float_as_uint(q2.z), w = __float_as_uint(q2.w);
-  const uint32_t x0 = u & 0xffffu, y0 = u >> 16, x1 = w & 0xffffu;
-  return __float_as_uint(q3.x) + (ty - y0) * (x1 - x0) + (tx - x0);
-}
 
 // ---------------------------------------------------------------------
 // Adjoint backward, order-correct. Math per (pixel, splat) is that of
@@ -205,9 +262,10 @@ template <int B>
 __global__ __launch_bounds__(WT) void k_backward(RasterArgs a) {
   __shared__ float4 s_q0[B];
   __shared__ float4 s_q1[B];
-  __shared__ float2 s_q2[B];
+  __shared__ float4 s_q2[B];
+  __shared__ uint32_t s_k[B];
   __shared__ uint32_t s_p[B];
-  __shared__ float s_out[B * REC9];
+  __shared__ float s_out[B * 4 * REC9];
   __shared__ uint32_t s_tg;
   const int lane = threadIdx.x;
   uint32_t tg = first_tile(a, &s_tg);
@@ -266,80 +324,81 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs a) {
       // entries no pixel reaches get zero records
       for (uint32_t i = tend + lane; i < end; i += WT) {
         const float4 *q = a.rec + (vbase + min(a.vals[i], a.N - 1u)) * 4;
-        const uint32_t p = emit_pos(q[2], q[3], t.tx, t.ty);
+        const uint32_t p = emit_pos(q[3], t.tx, t.ty);
         if ((uint64_t)p < a.n_isect) {
           float *o = a.isect_rec + (size_t)p * REC9;
 #pragma unroll
           for (int q9 = 0; q9 < REC9; ++q9) o[q9] = 0.0f;
         }
       }
+      const float fx = (float)t.px + 0.5f;
+      const float tx0 = (float)(t.tx * 16u) + 0.5f, tx1 = (float)min(t.tx * 16u + 15u, V.W - 1u) + 0.5f;
+      const float ty0 = (float)(t.ty * 16u) + 0.5f, ty1 = (float)min(t.ty * 16u + 15u, V.H - 1u) + 0.5f;
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        for (uint32_t k = lane; k < cnt; k += WT) {
-          const float4 *q = a.rec + (vbase + min(a.vals[lo + k], a.N - 1u)) * 4;
-          s_q0[k] = q[0];
-          s_q1[k] = q[1];
-          const float4 q2 = q[2];
-          s_q2[k] = make_float2(q2.x, q2.y);
-          s_p[k] = emit_pos(q2, q[3], t.tx, t.ty);
-        }
-        for (uint32_t z = lane; z < cnt * REC9; z += WT) s_out[z] = 0.0f;
+        for (uint32_t z = lane; z < cnt * 4 * REC9; z += WT) s_out[z] = 0.0f;
+        const uint32_t kept =
+            load_batch<B>(a, vbase, lo, cnt, tx0, tx1, ty0, ty1, s_q0, s_q1, s_q2, s_k, s_p, t.tx, t.ty);
         __syncthreads();
-        const float fx = (float)t.px + 0.5f;
-        for (int k = (int)cnt - 1; k >= 0; --k) {
-          const uint32_t i = lo + (uint32_t)k;
-          const float4 x0 = s_q0[k];
-          const float4 x1 = s_q1[k];
-          const float2 x2 = s_q2[k];
+        for (int e = (int)kept - 1; e >= 0; --e) {
+          const uint32_t k = s_k[e];
+          const uint32_t i = lo + k;
+          const float4 x0 = s_q0[e];
+          const float4 x1 = s_q1[e];
+          const float4 x2 = s_q2[e];
           const float dx = fx - x0.x;
           float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
           bool hit = false;
+          if (fabsf(dx) <= x2.z) {
 #pragma unroll
-          for (int j = 0; j < PPL; ++j) {
-            if (!(valid[j] && i <= lastp[j])) continue;
-            const float dy = fy[j] - x0.y;
-            const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-            if (sg < 0.0f || sg > x2.y) continue;
-            const float gauss = sv_expf(-sg);
-            const float op = x1.y;
-            const float as = fminf(op * gauss, SV_MAX_ALPHA);
-            if (as < SV_MIN_ALPHA) continue;
-            hit = true;
-            const float oma = fmaxf(1.0f - as, 0.0001f);
-            const float Tb = T[j] / oma;
-            const float contrib = as * Tb;
-            v0 = v0 + r0[j] * contrib;
-            v1 = v1 + r1[j] * contrib;
-            v2 = v2 + r2[j] * contrib;
-            const float cr = x1.z, cg = x1.w, cb = x2.x;
-            const float fg = (r0[j] * cr + r1[j] * cg) + r2[j] * cb;
-            const float bgt = (r0[j] * acc0[j] + r1[j] * acc1[j]) + r2[j] * acc2[j];
-            const float dL_dalpha = (fg - bgt) * Tb;
-            v3 = v3 + (dL_dalpha * as) * (1.0f - op);
-            const float dL_ds = (dL_dalpha * op) * (-gauss);
-            v4 = v4 + ((dL_ds * 0.5f) * dx) * dx;
-            v5 = v5 + (dL_ds * dx) * dy;
-            v6 = v6 + ((dL_ds * 0.5f) * dy) * dy;
-            v7 = v7 - dL_ds * (x0.z * dx + x0.w * dy);
-            v8 = v8 - dL_ds * (x0.w * dx + x1.x * dy);
-            acc0[j] = as * cr + (1.0f - as) * acc0[j];
-            acc1[j] = as * cg + (1.0f - as) * acc1[j];
-            acc2[j] = as * cb + (1.0f - as) * acc2[j];
-            T[j] = Tb;
+            for (int j = 0; j < PPL; ++j) {
+              if (!(valid[j] && i <= lastp[j])) continue;
+              const float dy = fy[j] - x0.y;
+              if (fabsf(dy) > x2.w) continue;
+              const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+              if (sg < 0.0f || sg > x2.y) continue;
+              const float gauss = sv_expf_neg(-sg);
+              const float op = x1.y;
+              const float as = fminf(op * gauss, SV_MAX_ALPHA);
+              if (as < SV_MIN_ALPHA) continue;
+              hit = true;
+              const float oma = fmaxf(1.0f - as, 0.0001f);
+              const float Tb = T[j] / oma;
+              const float contrib = as * Tb;
+              v0 = v0 + r0[j] * contrib;
+              v1 = v1 + r1[j] * contrib;
+              v2 = v2 + r2[j] * contrib;
+              const float cr = x1.z, cg = x1.w, cb = x2.x;
+              const float fg = (r0[j] * cr + r1[j] * cg) + r2[j] * cb;
+              const float bgt = (r0[j] * acc0[j] + r1[j] * acc1[j]) + r2[j] * acc2[j];
+              const float dL_dalpha = (fg - bgt) * Tb;
+              v3 = v3 + (dL_dalpha * as) * (1.0f - op);
+              const float dL_ds = (dL_dalpha * op) * (-gauss);
+              v4 = v4 + ((dL_ds * 0.5f) * dx) * dx;
+              v5 = v5 + (dL_ds * dx) * dy;
+              v6 = v6 + ((dL_ds * 0.5f) * dy) * dy;
+              v7 = v7 - dL_ds * (x0.z * dx + x0.w * dy);
+              v8 = v8 - dL_ds * (x0.w * dx + x1.x * dy);
+              acc0[j] = as * cr + (1.0f - as) * acc0[j];
+              acc1[j] = as * cg + (1.0f - as) * acc1[j];
+              acc2[j] = as * cb + (1.0f - as) * acc2[j];
+              T[j] = Tb;
+            }
           }
           if (!__any(hit)) continue;
-          v0 = wave_total63(v0);
-          v1 = wave_total63(v1);
-          v2 = wave_total63(v2);
-          v3 = wave_total63(v3);
-          v4 = wave_total63(v4);
-          v5 = wave_total63(v5);
-          v6 = wave_total63(v6);
-          v7 = wave_total63(v7);
-          v8 = wave_total63(v8);
-          if (lane == 63) {
-            float *o = &s_out[k * REC9];
+          v0 = row_total(v0);
+          v1 = row_total(v1);
+          v2 = row_total(v2);
+          v3 = row_total(v3);
+          v4 = row_total(v4);
+          v5 = row_total(v5);
+          v6 = row_total(v6);
+          v7 = row_total(v7);
+          v8 = row_total(v8);
+          if ((lane & 15) == 15) {
+            // row partials of entry k: s_out[(k * 4 + row) * 9 + q]
+            float *o = &s_out[(k * 4 + (lane >> 4)) * REC9];
             o[0] = v0;
             o[1] = v1;
             o[2] = v2;
@@ -355,7 +414,9 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs a) {
         for (uint32_t z = lane; z < cnt * REC9; z += WT) {
           const uint32_t k = z / REC9, q9 = z - k * REC9;
           const uint32_t p = s_p[k];
-          if ((uint64_t)p < a.n_isect) a.isect_rec[(size_t)p * REC9 + q9] = s_out[z];
+          const float *o = &s_out[k * 4 * REC9 + q9];
+          const float tot = ((o[3 * REC9] + o[2 * REC9]) + (o[REC9] + o[0]));
+          if ((uint64_t)p < a.n_isect) a.isect_rec[(size_t)p * REC9 + q9] = tot;
         }
         __syncthreads();
         hi = lo;

@@ -17,13 +17,13 @@ namespace sv {
 constexpr int SORT_T = 256;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_T * SORT_ITEMS;  // 4096 keys per workgroup
-constexpr int RADIX = 256;
 constexpr int NWAVE = SORT_T / 64;
 
-template <typename K>
+template <typename K, int RB>
 __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ keys, uint64_t n,
                                                       uint32_t shift, uint32_t nb,
                                                       uint32_t *__restrict__ hist) {
+  constexpr int RADIX = 1 << RB;
   __shared__ uint32_t s_h[NWAVE][RADIX];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_h[0][0])[i] = 0u;
@@ -73,7 +73,8 @@ __global__ __launch_bounds__(SORT_T) void k_sort_rowscan(uint32_t *__restrict__ 
                                                         const uint32_t *__restrict__ dtot) {
   __shared__ uint32_t s_w[NWAVE];
   const uint32_t d = blockIdx.x;
-  uint32_t part = (threadIdx.x < d) ? dtot[threadIdx.x] : 0u;
+  uint32_t part = 0;
+  for (uint32_t e = threadIdx.x; e < d; e += SORT_T) part += dtot[e];
   uint32_t carry = block_reduce_u32(part, s_w);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t base = 0; base < nb; base += SORT_T) {
@@ -99,13 +100,15 @@ __global__ __launch_bounds__(SORT_T) void k_sort_rowscan(uint32_t *__restrict__ 
   }
 }
 
-template <typename K>
+template <typename K, int RB>
 __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ kin,
                                                         const uint32_t *__restrict__ vin,
                                                         K *__restrict__ kout,
                                                         uint32_t *__restrict__ vout, uint64_t n,
                                                         uint32_t shift, uint32_t nb,
                                                         const uint32_t *__restrict__ hist) {
+  constexpr int RADIX = 1 << RB;
+  constexpr int DPT = RADIX / SORT_T;  // digits per thread in the block scan
   __shared__ uint32_t s_wcnt[NWAVE][RADIX];
   __shared__ uint32_t s_lbase[RADIX];
   __shared__ uint32_t s_gbase[RADIX];
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
     // lanes holding the same digit: AND of ballots over the 8 digit bits
     uint64_t m = __ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < 8; ++bit) {
+    for (int bit = 0; bit < RB; ++bit) {
       const bool on = (d >> bit) & 1u;
       const uint64_t bb = __ballot(on);
       m &= on ? bb : ~bb;
@@ -150,17 +153,25 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
     rank[j] = before + (uint32_t)__popcll(m & lt_mask);
   }
   __syncthreads();
-  // per digit: prefix over waves, block total, block exclusive scan
+  // per digit: prefix over waves, block total, block exclusive scan; each
+  // thread owns DPT consecutive digits
   {
-    const int d = threadIdx.x;  // SORT_T == RADIX
-    uint32_t tot = 0;
+    uint32_t dtot[DPT];
+    uint32_t tsum = 0;
 #pragma unroll
-    for (int w = 0; w < NWAVE; ++w) {
-      const uint32_t c = s_wcnt[w][d];
-      s_wcnt[w][d] = tot;
-      tot += c;
+    for (int u = 0; u < DPT; ++u) {
+      const int d = threadIdx.x * DPT + u;
+      uint32_t tot = 0;
+#pragma unroll
+      for (int w = 0; w < NWAVE; ++w) {
+        const uint32_t c = s_wcnt[w][d];
+        s_wcnt[w][d] = tot;
+        tot += c;
+      }
+      dtot[u] = tot;
+      tsum += tot;
     }
-    uint32_t x = tot;
+    uint32_t x = tsum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
@@ -168,11 +179,15 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
     }
     if (lane == 63) s_w[wave] = x;
     __syncthreads();
-    uint32_t wb = 0;
+    uint32_t run = x - tsum;
 #pragma unroll
     for (int w = 0; w < NWAVE; ++w)
-      if (w < wave) wb += s_w[w];
-    s_lbase[d] = wb + x - tot;
+      if (w < wave) run += s_w[w];
+#pragma unroll
+    for (int u = 0; u < DPT; ++u) {
+      s_lbase[threadIdx.x * DPT + u] = run;
+      run += dtot[u];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -200,33 +215,46 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
   }
 }
 
+// 8-bit digits: a 10-bit digit saves a pass for 17-20 bit keys but its
+// per-tile histogram (1024 x tiles) and 56 KB of LDS staging made each pass
+// about six times slower on gfx950, so every pass uses 8 bits.
+constexpr int DIGIT_BITS = 8;
+
 template <typename K>
 size_t sort_tmp_words(uint64_t n) {
   const uint64_t nb = (n + SORT_TILE - 1) / SORT_TILE;
-  return (size_t)(nb * RADIX + RADIX);
+  return (size_t)(nb * (1u << DIGIT_BITS) + (1u << DIGIT_BITS));
+}
+
+template <typename K, int RB>
+static void sort_pass(hipStream_t st, const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
+                      uint64_t n, uint32_t shift, uint32_t nb, uint32_t *hist, uint32_t *dtot) {
+  constexpr int RADIX = 1 << RB;
+  hipLaunchKernelGGL((k_sort_count<K, RB>), dim3(nb), dim3(SORT_T), 0, st, kin, n, shift, nb, hist);
+  hipLaunchKernelGGL(k_sort_rowsum, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
+  hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
+  hipLaunchKernelGGL((k_sort_scatter<K, RB>), dim3(nb), dim3(SORT_T), 0, st, kin, vin, kout, vout, n,
+                     shift, nb, hist);
 }
 
 template <typename K>
 void launch_sort(hipStream_t st, K *keys, uint32_t *vals, K *keys2, uint32_t *vals2, uint64_t n,
                  uint32_t bits, uint32_t *tmp, int *passes_out) {
   const uint32_t nb = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
-  const int passes = (int)((bits + 7) / 8);
+  const int rb = DIGIT_BITS;
+  const int passes = (int)((bits + rb - 1) / rb);
   *passes_out = passes;
   if (n <= 1 || passes == 0) {
     *passes_out = 0;
     return;
   }
   uint32_t *hist = tmp;
-  uint32_t *dtot = tmp + (size_t)nb * RADIX;
+  uint32_t *dtot = tmp + (size_t)nb * (1u << DIGIT_BITS);
   K *kin = keys, *kout = keys2;
   uint32_t *vin = vals, *vout = vals2;
   for (int p = 0; p < passes; ++p) {
-    const uint32_t shift = 8u * (uint32_t)p;
-    hipLaunchKernelGGL(k_sort_count<K>, dim3(nb), dim3(SORT_T), 0, st, kin, n, shift, nb, hist);
-    hipLaunchKernelGGL(k_sort_rowsum, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
-    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
-    hipLaunchKernelGGL(k_sort_scatter<K>, dim3(nb), dim3(SORT_T), 0, st, kin, vin, kout, vout, n,
-                       shift, nb, hist);
+    const uint32_t shift = (uint32_t)(rb * p);
+    sort_pass<K, DIGIT_BITS>(st, kin, vin, kout, vout, n, shift, nb, hist, dtot);
     K *tk = kin;
     kin = kout;
     kout = tk;
