@@ -71,9 +71,12 @@ def step_b_alg(N: int, V: int, I: int, P: int, d: int) -> float:
     return 1488.0 * N + 92.0 * N * V + I * (136.0 + 24.0 * d) + 40.0 * P
 
 
-def run_mode(mode, g, cams, targets, args):
+def run_mode(mode, g, cams, targets, args, graph=0):
+    """Returns (s/step of the timed loop, per-phase timings, phase steps,
+    stats of one more step, s/step of eager launches or None)."""
     ctx = sv.Context(capacity=g["pos"].shape[0])
     configure(ctx, args)
+    ctx.set_config(use_graph=graph)
     ctx.set_gaussians(g)
     ctx.set_views(cams)
     for v, t in enumerate(targets):
@@ -82,19 +85,33 @@ def run_mode(mode, g, cams, targets, args):
     for i in range(args.warmup):
         ctx.step_views(views, mode)
     ctx.sync()
-    ctx.reset_timings()
-    ctx.enable_timing(True)
-    ctx.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.step_views(views, mode)
     ctx.sync()
     t1 = time.perf_counter()
+    # per-phase kernel time: further steps with HIP events around each phase
+    # on the library stream (these launch eagerly)
+    phase_steps = max(1, min(args.steps, 5))
+    ctx.reset_timings()
+    ctx.enable_timing(True)
+    for i in range(phase_steps):
+        ctx.step_views(views, mode)
     timings = ctx.timings()
     ctx.enable_timing(False)
+    eager = None
+    if graph:
+        ctx.set_config(use_graph=0)
+        ctx.step_views(views, mode)
+        ctx.sync()
+        e0 = time.perf_counter()
+        for i in range(args.steps):
+            ctx.step_views(views, mode)
+        ctx.sync()
+        eager = (time.perf_counter() - e0) / args.steps
     st = ctx.step_views(views, mode, stats=True)
     ctx.close()
-    return (t1 - t0) / args.steps, timings, st
+    return (t1 - t0) / args.steps, timings, phase_steps, st, eager
 
 
 def cpu_baseline(g, cams, targets, args):
@@ -138,6 +155,7 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--persistent", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1, help="1: the batched step is one hipGraph launch")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -175,27 +193,29 @@ def main():
             targets.append(tc.image(v))
     log("targets rendered")
 
-    sec_b, tim_b, st_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args)
-    log(f"batched: {1e3 * sec_b:.2f} ms/step, I={st_b.n_isect}, key bits {st_b.key_bits}, passes {st_b.sort_passes}")
+    sec_b, tim_b, ps_b, st_b, eager_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args, graph=args.graph)
+    log(f"batched: {1e3 * sec_b:.2f} ms/step (graph {args.graph}), I={st_b.n_isect}, "
+        f"key bits {st_b.key_bits}, passes {st_b.sort_passes}"
+        + (f"; eager launches {1e3 * eager_b:.2f} ms/step" if eager_b else ""))
     plain = sequential = None
     if not args.no_plain:
-        sec_q, tim_q, _ = run_mode(sv.MODE_SEQUENTIAL, g, cams, targets, args)
+        sec_q, tim_q, ps_q, _, _ = run_mode(sv.MODE_SEQUENTIAL, g, cams, targets, args)
         sequential = {"ms_per_step": 1e3 * sec_q, "steps_per_s": 1.0 / sec_q,
-                      "phase_ms_per_step": {k: v[0] / args.steps for k, v in tim_q.items() if v[1]}}
+                      "phase_ms_per_step": {k: v[0] / ps_q for k, v in tim_q.items() if v[1]}}
         log(f"sequential adjoint: {1e3 * sec_q:.2f} ms/step")
-        sec_p, tim_p, _ = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
+        sec_p, tim_p, ps_p, _, _ = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
         plain = {"ms_per_step": 1e3 * sec_p, "steps_per_s": 1.0 / sec_p,
                  "method": "reference method per view: wavefront fixed-point backward, atomic reduce, "
                            "geometry backward, Adam",
-                 "phase_ms_per_step": {k: v[0] / args.steps for k, v in tim_p.items() if v[1]}}
+                 "phase_ms_per_step": {k: v[0] / ps_p for k, v in tim_p.items() if v[1]}}
         log(f"plain: {1e3 * sec_p:.2f} ms/step")
 
     I = int(st_b.n_isect)
     P = V * W * H
     key_bytes = 8 if st_b.key_bits > 32 else 4
     passes = int(st_b.sort_passes)
-    phase = {k: (v[0] / max(v[1], 1), v[1] / args.steps) for k, v in tim_b.items() if v[1]}
-    sort_ms_step = tim_b.get("sort", (0.0, 0))[0] / args.steps
+    phase = {k: (v[0] / max(v[1], 1), v[1] / ps_b) for k, v in tim_b.items() if v[1]}
+    sort_ms_step = tim_b.get("sort", (0.0, 0))[0] / ps_b
     keys_per_s = I / (sort_ms_step * 1e-3) if sort_ms_step > 0 else None
     # dominant single-kernel phase
     cands = {k: phase[k][0] for k in ("forward", "backward", "vjp_adam") if k in phase}
@@ -219,7 +239,8 @@ def main():
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
         "config": {"workload": f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step",
                    "splats": N, "views": V, "width": W, "height": H, "tile": 16, "lds_batch": args.batch,
-                   "persistent": args.persistent},
+                   "persistent": args.persistent, "graph": args.graph},
+        "eager_launch_ms_per_step": 1e3 * eager_b if eager_b else None,
         "keys_per_s": keys_per_s,
         "intersections_per_step": I,
         "sort_key_bits": int(st_b.key_bits),

@@ -48,7 +48,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [hipcc(), *HIP_FLAGS, *COMMON, "-c", path, "-o", out]
     else:
-        cmd = [hipcc(), *COMMON, "-pthread", "-c", path, "-o", out]
+        cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, "-pthread", "-c", path, "-o", out]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)

@@ -104,7 +104,12 @@ struct sv_ctx {
   size_t vg_cap = 0;
   uint64_t *scan_tmp = nullptr;
   size_t scan_tmp_cap = 0;
-  uint64_t *d_small = nullptr;  // [0] total, [1] max depth quantum, [2] work counter
+  uint64_t *d_small = nullptr;  // scan totals of prune / densify
+  StepMeta *d_meta = nullptr;   // device-side sizes of the current tile build
+  StepMeta *h_meta = nullptr;   // pinned copy read after a build
+  AdamParams *d_adam = nullptr;
+  AdamParams *h_adam = nullptr;      // pinned, mapped: written before a graph replay
+  AdamParams *h_adam_dev = nullptr;  // its device address
   void *keys = nullptr, *keys2 = nullptr;
   uint32_t *vals = nullptr, *vals2 = nullptr;
   uint64_t isect_cap = 0;
@@ -115,13 +120,22 @@ struct sv_ctx {
   uint64_t fixed_cap = 0;
   uint64_t *h_small = nullptr;  // pinned
 
-  // last tile build
+  // key plan of the batch (host bounds) and the last tile build (read back)
+  uint32_t tile_bits = 0, max_key_bits = 0;
+  bool key64 = false;
+  int max_passes = 0;
   bool built = false;
   bool forward_done = false;
   uint64_t I = 0;
   uint32_t dbits = 0, key_bits = 0;
-  bool key64 = false;
   int passes = 0;
+  int sel = 0;  // sorted pairs in (keys, vals) = 0 or (keys2, vals2) = 1
+
+  // captured batched step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<uint64_t> gsig;
+  bool capturing = false;
 
   // timing
   bool timing = false;
@@ -196,7 +210,7 @@ sv_status grow(sv_ctx *c, T **p, size_t &capn, size_t n) {
 
 void ph_begin(sv_ctx *c, int id, TimedPhase *tp) {
   tp->id = -1;
-  if (!c->timing) return;
+  if (!c->timing || c->capturing) return;
   if (c->event_pool.size() < 2) {
     for (int i = 0; i < 16; ++i) {
       hipEvent_t e;
@@ -276,6 +290,9 @@ sv_status check_view(sv_ctx *c, int v) {
 // ---------------------------------------------------------------------
 // Batch: a list of views rendered through one pipeline.
 // ---------------------------------------------------------------------
+void key_plan(sv_ctx *c);
+sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n);
+
 sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
   if (nv <= 0) return fail(c, SV_E_ARG, "no views");
   if (c->N == 0) return fail(c, SV_E_STATE, "no Gaussians");
@@ -342,99 +359,150 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     c->vg_cap = M;
   }
   if (sv_status s = grow(c, &c->scan_tmp, c->scan_tmp_cap, scan_tmp_words(M))) return s;
-  return SV_OK;
+  key_plan(c);
+  return ensure_isect_capacity(c, 1);
 }
 
 size_t rec_stride(const sv_ctx *c) { return (size_t)c->slot_views.size() * c->N; }
 
-// buildTileIntersections (gaussian-renderer.ts:1780-2023) for the batch.
-sv_status build_batch(sv_ctx *c) {
+// Key type and the most sort passes the batch can need, from bounds the
+// host knows: tile bits of the batch and the depth quantum bound. A listed
+// splat has depth <= far_plane, so depth_q <= min(65535, far_plane).
+void key_plan(sv_ctx *c) {
+  c->tile_bits = bitlen(c->tiles_total > 0 ? c->tiles_total - 1 : 0);
+  const double far = (double)c->cfg.far_plane;
+  const uint64_t dq_max = (far >= 0.0) ? (uint64_t)std::floor(std::min(far, 65535.0)) : 65535u;
+  c->max_key_bits = c->tile_bits + bitlen(dq_max);
+  c->key64 = c->max_key_bits > 32;
+  c->max_passes = (int)((c->max_key_bits + 7) / 8);
+}
+
+// Pair buffers for at least n intersections (contents are not kept).
+sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
+  if (n <= c->isect_cap && c->keys) return SV_OK;
+  const uint64_t want = std::max<uint64_t>(n + n / 4, 1 << 16);
+  dfree(c->keys);
+  dfree(c->keys2);
+  dfree(c->vals);
+  dfree(c->vals2);
+  dfree(c->isect_rec);
+  dfree(c->sort_tmp);
+  c->isect_cap = 0;
+  c->sort_tmp_cap = 0;
+  if (sv_status s = dalloc(c, (uint64_t **)&c->keys, want)) return s;
+  if (sv_status s = dalloc(c, (uint64_t **)&c->keys2, want)) return s;
+  if (sv_status s = dalloc(c, &c->vals, want)) return s;
+  if (sv_status s = dalloc(c, &c->vals2, want)) return s;
+  if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC9)) return s;
+  const size_t stw = sort_tmp_words<uint64_t>(want);
+  if (sv_status s = dalloc(c, &c->sort_tmp, stw)) return s;
+  c->sort_tmp_cap = stw;
+  c->isect_cap = want;
+  return SV_OK;
+}
+
+// buildTileIntersections (gaussian-renderer.ts:1780-2023) for the batch,
+// enqueued without any host readback: the pair count, key layout, sort
+// passes and result buffer stay on the device (StepMeta), and every launch
+// is sized by capacities, so the same launches serve any step.
+sv_status enqueue_build(sv_ctx *c) {
   const int nv = (int)c->slot_views.size();
   const DevConfig dc = dev_config(c->cfg);
   const size_t M = (size_t)nv * c->N;
-  launch_fill_u32(c->st, (uint32_t *)c->d_small, 0u, 6);
+  launch_fill_u32(c->st, (uint32_t *)c->d_meta, 0u, sizeof(StepMeta) / 4);
   {
     Scoped t(c, PH_PROJECT);
     launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts,
-                   (uint32_t *)(c->d_small + 1));
+                   &c->d_meta->max_dq);
   }
-  if (sv_status s = check_launch(c, "project")) return s;
   {
     Scoped t(c, PH_SCAN);
-    launch_scan(c->st, c->counts, c->offs, M, c->scan_tmp, c->d_small);
+    launch_scan(c->st, c->counts, c->offs, M, c->scan_tmp, &c->d_meta->n_isect);
+    launch_keyinfo(c->st, c->d_meta, c->tile_bits, c->max_key_bits, c->isect_cap);
   }
-  if (sv_status s = check_launch(c, "scan")) return s;
-  HIP_OK(c, hipMemcpyAsync(c->h_small, c->d_small, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
-  if (sv_status s = sync(c)) return s;
-  const uint64_t I = c->h_small[0];
-  const uint32_t maxdq = (uint32_t)(c->h_small[1] & 0xffffffffu);
-  if (I >= 0xffffffffull)
-    return fail(c, SV_E_CAPACITY, "%llu intersections exceed the 32-bit index range", (unsigned long long)I);
-  if (I > c->isect_cap) {
-    const uint64_t want = std::max<uint64_t>(I + I / 4, 1024);
-    dfree(c->keys);
-    dfree(c->keys2);
-    dfree(c->vals);
-    dfree(c->vals2);
-    dfree(c->isect_rec);
-    c->isect_cap = 0;
-    if (sv_status s = dalloc(c, (uint64_t **)&c->keys, want)) return s;
-    if (sv_status s = dalloc(c, (uint64_t **)&c->keys2, want)) return s;
-    if (sv_status s = dalloc(c, &c->vals, want)) return s;
-    if (sv_status s = dalloc(c, &c->vals2, want)) return s;
-    if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC9)) return s;
-    c->isect_cap = want;
-  }
-  c->I = I;
-  c->dbits = bitlen(maxdq);
-  const uint32_t tile_bits = bitlen(c->tiles_total > 0 ? c->tiles_total - 1 : 0);
-  c->key_bits = tile_bits + c->dbits;
-  c->key64 = c->key_bits > 32;
-  const size_t stw = c->key64 ? sort_tmp_words<uint64_t>(I) : sort_tmp_words<uint32_t>(I);
-  if (sv_status s = grow(c, &c->sort_tmp, c->sort_tmp_cap, stw)) return s;
   {
     Scoped t(c, PH_EMIT);
     if (c->key64)
-      launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+      launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
                             (uint64_t *)c->keys, c->vals);
     else
-      launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->dbits,
+      launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
                             (uint32_t *)c->keys, c->vals);
   }
-  if (sv_status s = check_launch(c, "emit")) return s;
   {
     Scoped t(c, PH_SORT);
     if (c->key64)
-      launch_sort<uint64_t>(c->st, (uint64_t *)c->keys, c->vals, (uint64_t *)c->keys2, c->vals2, I,
-                            c->key_bits, c->sort_tmp, &c->passes);
+      launch_sort<uint64_t>(c->st, (uint64_t *)c->keys, c->vals, (uint64_t *)c->keys2, c->vals2,
+                            c->isect_cap, c->max_passes, c->sort_tmp, c->d_meta);
     else
-      launch_sort<uint32_t>(c->st, (uint32_t *)c->keys, c->vals, (uint32_t *)c->keys2, c->vals2, I,
-                            c->key_bits, c->sort_tmp, &c->passes);
-  }
-  if (sv_status s = check_launch(c, "sort")) return s;
-  if (c->passes & 1) {
-    std::swap(c->keys, c->keys2);
-    std::swap(c->vals, c->vals2);
+      launch_sort<uint32_t>(c->st, (uint32_t *)c->keys, c->vals, (uint32_t *)c->keys2, c->vals2,
+                            c->isect_cap, c->max_passes, c->sort_tmp, c->d_meta);
   }
   {
     Scoped t(c, PH_RANGES);
     if (c->key64)
-      launch_ranges<uint64_t>(c->st, (const uint64_t *)c->keys, I, c->dbits, c->tiles_total, c->ranges);
+      launch_ranges<uint64_t>(c->st, (const uint64_t *)c->keys, (const uint64_t *)c->keys2, c->isect_cap,
+                              c->tiles_total, c->d_meta, c->ranges);
     else
-      launch_ranges<uint32_t>(c->st, (const uint32_t *)c->keys, I, c->dbits, c->tiles_total, c->ranges);
+      launch_ranges<uint32_t>(c->st, (const uint32_t *)c->keys, (const uint32_t *)c->keys2, c->isect_cap,
+                              c->tiles_total, c->d_meta, c->ranges);
   }
-  if (sv_status s = check_launch(c, "ranges")) return s;
-  c->built = true;
+  return check_launch(c, "tile build");
+}
+
+// Wait for the stream and read the build's metadata. *grow_to > 0: the
+// buffers were too small and nothing after the scan ran; grow and redo.
+sv_status read_meta(sv_ctx *c, uint64_t *grow_to) {
+  *grow_to = 0;
+  HIP_OK(c, hipMemcpyAsync(c->h_meta, c->d_meta, sizeof(StepMeta), hipMemcpyDeviceToHost, c->st));
+  if (sv_status s = sync(c)) return s;
+  const StepMeta &m = *c->h_meta;
+  if (m.overflow == 2)
+    return fail(c, SV_E_CAPACITY, "sort key of %u bits exceeds the planned %u bits", m.key_bits, c->max_key_bits);
+  if (m.overflow == 1) {
+    if (m.n_isect >= 0xffffffffull)
+      return fail(c, SV_E_CAPACITY, "%llu intersections exceed the 32-bit index range",
+                  (unsigned long long)m.n_isect);
+    *grow_to = m.n_isect;
+    return SV_OK;
+  }
+  c->I = m.n_isect;
+  c->dbits = m.dbits;
+  c->key_bits = m.key_bits;
+  c->passes = (int)m.passes;
+  c->sel = (int)m.sel;
   return SV_OK;
 }
+
+// Build and wait (the per-view entry points and the plain / sequential
+// steps read the pair count on the host).
+sv_status build_batch(sv_ctx *c) {
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (sv_status s = enqueue_build(c)) return s;
+    uint64_t grow_to = 0;
+    if (sv_status s = read_meta(c, &grow_to)) return s;
+    if (!grow_to) {
+      c->built = true;
+      return SV_OK;
+    }
+    if (sv_status s = ensure_isect_capacity(c, grow_to)) return s;
+  }
+  return fail(c, SV_E_STATE, "tile build did not fit after growing the buffers");
+}
+
+const uint32_t *cur_vals(const sv_ctx *c) { return c->sel ? c->vals2 : c->vals; }
+const void *cur_keys(const sv_ctx *c) { return c->sel ? c->keys2 : c->keys; }
 
 RasterArgs raster_args(sv_ctx *c) {
   RasterArgs a{};
   a.views = c->d_views;
   a.tile_view = c->tile_view;
   a.ranges = c->ranges;
-  a.vals = c->vals;
-  a.n_isect = c->I;
+  a.vals0 = c->vals;
+  a.vals1 = c->vals2;
+  a.meta = c->d_meta;
+  a.vals = nullptr;
+  a.n_isect = 0;
   a.rec = c->rec;
   a.N = c->N;
   a.n_tiles_total = c->tiles_total;
@@ -453,31 +521,32 @@ RasterArgs raster_args(sv_ctx *c) {
 
 uint32_t raster_grid(sv_ctx *c, RasterArgs &a) {
   if (c->cfg.persistent) {
-    a.work_counter = (uint32_t *)(c->d_small + 2);
+    a.work_counter = &c->d_meta->work;
     launch_fill_u32(c->st, a.work_counter, 0u, 1);
     return std::min<uint32_t>(c->tiles_total, 256u * 8u);
   }
   return c->tiles_total;
 }
 
-sv_status forward_batch(sv_ctx *c) {
-  if (!c->built) return fail(c, SV_E_STATE, "tiles not built");
+sv_status enqueue_forward(sv_ctx *c) {
   RasterArgs a = raster_args(c);
   const uint32_t grid = raster_grid(c, a);
   {
     Scoped t(c, PH_FORWARD);
     launch_forward(c->st, a, c->cfg.batch, grid);
   }
-  if (sv_status s = check_launch(c, "forward")) return s;
+  return check_launch(c, "forward");
+}
+
+sv_status forward_batch(sv_ctx *c) {
+  if (!c->built) return fail(c, SV_E_STATE, "tiles not built");
+  if (sv_status s = enqueue_forward(c)) return s;
   c->forward_done = true;
   return SV_OK;
 }
 
 // use_residual: read the residual arena instead of the fused loss.
-sv_status backward_batch(sv_ctx *c, bool use_residual) {
-  if (!c->forward_done) return fail(c, SV_E_STATE, "forward not run for this batch");
-  for (int v : c->slot_views)
-    if (!use_residual && !c->views[v].has_target) return fail(c, SV_E_STATE, "view %d has no target", v);
+sv_status enqueue_backward(sv_ctx *c, bool use_residual) {
   RasterArgs a = raster_args(c);
   if (use_residual) a.residual = c->residual;
   const uint32_t grid = raster_grid(c, a);
@@ -488,8 +557,18 @@ sv_status backward_batch(sv_ctx *c, bool use_residual) {
   return check_launch(c, "backward");
 }
 
-void adam_hparams(sv_ctx *c, uint32_t t, float lr[5], float *bc1, float *bc2s, int *update_sh) {
+sv_status backward_batch(sv_ctx *c, bool use_residual) {
+  if (!c->forward_done) return fail(c, SV_E_STATE, "forward not run for this batch");
+  for (int v : c->slot_views)
+    if (!use_residual && !c->views[v].has_target) return fail(c, SV_E_STATE, "view %d has no target", v);
+  return enqueue_backward(c, use_residual);
+}
+
+// Adam hyper-parameters of step t (run_adam_on_buffer,
+// gaussian-renderer.ts:2409-2452).
+AdamParams adam_params(const sv_ctx *c, uint32_t t) {
   const sv_config &k = c->cfg;
+  AdamParams h{};
   double lr_pos = k.lr_pos;
   if (k.pos_lr_schedule && k.lr_pos > 0.0f) {
     // computePositionLearningRate (gaussian-renderer.ts:284-298)
@@ -501,17 +580,20 @@ void adam_hparams(sv_ctx *c, uint32_t t, float lr[5], float *bc1, float *bc2s, i
     const double delay = delay_mult + (1.0 - delay_mult) * std::sin(0.5 * M_PI * phase);
     lr_pos = loglerp * delay;
   }
-  lr[0] = (float)lr_pos;
-  lr[1] = k.lr_scale;
-  lr[2] = k.lr_rot;
-  lr[3] = k.lr_opacity;
-  lr[4] = k.lr_sh;
-  // run_adam_on_buffer (gaussian-renderer.ts:2425-2440): bias terms in
-  // double, stored as f32.
+  h.lr[0] = (float)lr_pos;
+  h.lr[1] = k.lr_scale;
+  h.lr[2] = k.lr_rot;
+  h.lr[3] = k.lr_opacity;
+  h.lr[4] = k.lr_sh;
+  h.beta1 = k.beta1;
+  h.beta2 = k.beta2;
+  h.eps = k.eps;
+  // bias terms in double, stored as f32 (gaussian-renderer.ts:2425-2440)
   const double tt = t < 1 ? 1.0 : (double)t;
-  *bc1 = (float)(1.0 / (1.0 - std::pow((double)k.beta1, tt)));
-  *bc2s = (float)(1.0 / std::sqrt(1.0 - std::pow((double)k.beta2, tt)));
-  *update_sh = (k.sh_update_interval == 0) ? 1 : ((t % k.sh_update_interval) == 0);
+  h.bc1_rcp = (float)(1.0 / (1.0 - std::pow((double)k.beta1, tt)));
+  h.bc2_sqrt_rcp = (float)(1.0 / std::sqrt(1.0 - std::pow((double)k.beta2, tt)));
+  h.update_sh = (k.sh_update_interval == 0) ? 1u : ((t % k.sh_update_interval) == 0 ? 1u : 0u);
+  return h;
 }
 
 VjpArgs vjp_args(sv_ctx *c) {
@@ -531,9 +613,11 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.G = c->G;
   v.geom = c->geom;
   v.grad_accum = c->grad_accum;
+  v.meta = c->d_meta;
   v.Pw = c->P;
   v.M1 = c->M1;
   v.M2 = c->M2;
+  v.adam = c->d_adam;
   return v;
 }
 
@@ -543,17 +627,19 @@ sv_status zero_grads(sv_ctx *c) {
   return check_launch(c, "zero grads");
 }
 
-sv_status adam_planes(sv_ctx *c) {
-  c->step_t += 1;
-  float lr[5], bc1, bc2s;
-  int upd;
-  adam_hparams(c, c->step_t, lr, &bc1, &bc2s, &upd);
+// Adam over the gradient planes with parameters already in d_adam.
+sv_status enqueue_adam_planes(sv_ctx *c) {
   {
     Scoped t(c, PH_ADAM);
-    launch_adam(c->st, c->P, c->M1, c->M2, c->G, c->cap, c->N, lr, c->cfg.beta1, c->cfg.beta2, c->cfg.eps,
-                bc1, bc2s, upd);
+    launch_adam(c->st, c->P, c->M1, c->M2, c->G, c->cap, c->N, c->d_adam);
   }
   return check_launch(c, "adam");
+}
+
+sv_status adam_planes(sv_ctx *c) {
+  c->step_t += 1;
+  launch_load_adam(c->st, adam_params(c, c->step_t), nullptr, c->d_adam);
+  return enqueue_adam_planes(c);
 }
 
 sv_status upload_planar(sv_ctx *c, float *dst, size_t cap, const float *src, size_t n, int comps,
@@ -675,21 +761,27 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
   {
     Scoped t(c, PH_FIXED);
     launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed);
-    launch_reduce_fixed(c->st, c->fixed, c->vals, c->I, c->cap, c->G, c->geom);
+    launch_reduce_fixed(c->st, c->fixed, c->vals, c->vals2, c->d_meta, c->I, c->cap, c->G, c->geom);
     launch_geom(c->st, c->P, c->cap, c->N, c->d_views, dev_config(c->cfg), c->geom, c->G,
                 stats ? c->grad_accum : nullptr, c->aux, c->counts, stats ? c->importance : nullptr);
   }
   return check_launch(c, "backward_fixed");
 }
 
-sv_status step_batched(sv_ctx *c, const int *vs, int nv) {
-  if (sv_status s = prepare_batch(c, vs, nv)) return s;
-  if (sv_status s = build_batch(c)) return s;
-  if (sv_status s = forward_batch(c)) return s;
+// The batched step after prepare_batch, enqueued with no host readback
+// and no allocation, so that it can be captured: build, forward,
+// [loss], adjoint backward, per-Gaussian VJP with fused Adam (or VJP into
+// the gradient planes + Adam when keep_grads). Adam parameters come from
+// `hp` or, when hp is null, from the mapped host block h_adam (re-read on
+// every replay).
+sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
+  if (sv_status s = enqueue_build(c)) return s;
+  if (sv_status s = enqueue_forward(c)) return s;
   const bool use_res = batch_needs_residual_buffer(c);
   if (use_res)
     if (sv_status s = loss_views(c)) return s;
-  if (sv_status s = backward_batch(c, use_res)) return s;
+  if (sv_status s = enqueue_backward(c, use_res)) return s;
+  launch_load_adam(c->st, hp ? *hp : AdamParams{}, hp ? nullptr : c->h_adam_dev, c->d_adam);
   VjpArgs va = vjp_args(c);
   if (c->cfg.keep_grads) {
     if (sv_status s = zero_grads(c)) return s;
@@ -698,26 +790,101 @@ sv_status step_batched(sv_ctx *c, const int *vs, int nv) {
       launch_vjp(c->st, va);
     }
     if (sv_status s = check_launch(c, "vjp")) return s;
-    return adam_planes(c);
+    return enqueue_adam_planes(c);
   }
-  c->step_t += 1;
-  float lr[5], bc1, bc2s;
-  int upd;
-  adam_hparams(c, c->step_t, lr, &bc1, &bc2s, &upd);
   va.G = nullptr;
   va.geom = nullptr;
-  for (int k = 0; k < 5; ++k) va.lr[k] = lr[k];
-  va.beta1 = c->cfg.beta1;
-  va.beta2 = c->cfg.beta2;
-  va.eps = c->cfg.eps;
-  va.bc1_rcp = bc1;
-  va.bc2_sqrt_rcp = bc2s;
-  va.update_sh = upd;
   {
     Scoped t(c, PH_VJP);
     launch_vjp(c->st, va);
   }
   return check_launch(c, "vjp+adam");
+}
+
+// Everything a captured step depends on: buffers, sizes, the batch's
+// views and the configuration (passed by value into kernels).
+std::vector<uint64_t> graph_signature(const sv_ctx *c) {
+  std::vector<uint64_t> g;
+  const void *ptrs[] = {c->P, c->M1, c->M2, c->G, c->geom, c->importance, c->grad_accum, c->color,
+                        c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
+                        c->l1_partial, c->rec, c->aux, c->counts, c->offs, c->scan_tmp, c->keys,
+                        c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
+                        c->h_adam_dev};
+  for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
+  g.push_back(c->N);
+  g.push_back(c->cap);
+  g.push_back(c->isect_cap);
+  g.push_back(c->tiles_total);
+  for (int v : c->slot_views) g.push_back((uint64_t)(uint32_t)v);
+  uint64_t w[(sizeof(sv_config) + 7) / 8] = {0};
+  std::memcpy(w, &c->cfg, sizeof(sv_config));
+  for (uint64_t x : w) g.push_back(x);
+  return g;
+}
+
+void drop_graph(sv_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->gsig.clear();
+}
+
+// Capture the batched step into a graph unless the current one matches.
+sv_status ensure_graph(sv_ctx *c) {
+  std::vector<uint64_t> sig = graph_signature(c);
+  if (c->gexec && sig == c->gsig) return SV_OK;
+  drop_graph(c);
+  HIP_OK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  sv_status s = enqueue_batched_step(c, nullptr);
+  c->capturing = false;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->st, &g);
+  if (s) {
+    if (g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  if (e != hipSuccess || !g) return fail(c, SV_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(c, SV_E_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  c->graph = g;
+  c->gexec = x;
+  c->gsig.swap(sig);
+  return SV_OK;
+}
+
+// One batched step. With use_graph (and timing off) the step is one graph
+// launch; the Adam step counter only advances when the step ran. If the
+// pair buffers were too small, nothing past the scan wrote any state: the
+// buffers grow and the step runs again.
+sv_status step_batched(sv_ctx *c, const int *vs, int nv) {
+  if (sv_status s = prepare_batch(c, vs, nv)) return s;
+  const uint32_t t = c->step_t + 1;
+  const AdamParams hp = adam_params(c, t);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    if (c->cfg.use_graph && !c->timing) {
+      if (sv_status s = ensure_graph(c)) return s;
+      *c->h_adam = hp;
+      HIP_OK(c, hipGraphLaunch(c->gexec, c->st));
+    } else {
+      if (sv_status s = enqueue_batched_step(c, &hp)) return s;
+    }
+    uint64_t grow_to = 0;
+    if (sv_status s = read_meta(c, &grow_to)) return s;
+    if (!grow_to) {
+      c->step_t = t;
+      c->built = true;
+      c->forward_done = true;
+      return SV_OK;
+    }
+    if (sv_status s = ensure_isect_capacity(c, grow_to)) return s;
+  }
+  return fail(c, SV_E_STATE, "batched step did not fit after growing the buffers");
 }
 
 double sum_l1(sv_ctx *c) {
@@ -865,8 +1032,15 @@ sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
   c->cap = capacity;
   sv_status s = alloc_gaussian_arrays(c, capacity);
   if (!s) s = dalloc(c, &c->d_small, 8);
+  if (!s) s = dalloc(c, &c->d_meta, 1);
+  if (!s) s = dalloc(c, &c->d_adam, 1);
   if (!s && hipHostMalloc((void **)&c->h_small, 8 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
     s = fail(c, SV_E_NOMEM, "hipHostMalloc failed");
+  if (!s && hipHostMalloc((void **)&c->h_meta, sizeof(StepMeta), hipHostMallocDefault) != hipSuccess)
+    s = fail(c, SV_E_NOMEM, "hipHostMalloc failed");
+  if (!s && (hipHostMalloc((void **)&c->h_adam, sizeof(AdamParams), hipHostMallocMapped) != hipSuccess ||
+             hipHostGetDevicePointer((void **)&c->h_adam_dev, c->h_adam, 0) != hipSuccess))
+    s = fail(c, SV_E_NOMEM, "mapped hipHostMalloc failed");
   if (s) {
     g_create_err = c->err;
     sv_destroy(c);
@@ -913,7 +1087,12 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->counts);
   dfree(c->offs);
   dfree(c->scan_tmp);
+  drop_graph(c);
   dfree(c->d_small);
+  dfree(c->d_meta);
+  dfree(c->d_adam);
+  if (c->h_meta) (void)hipHostFree(c->h_meta);
+  if (c->h_adam) (void)hipHostFree(c->h_adam);
   dfree(c->keys);
   dfree(c->keys2);
   dfree(c->vals);
@@ -935,7 +1114,6 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->raster_px != 16) return fail(c, SV_E_ARG, "raster_px must be 16 (got %u)", k->raster_px);
   if (k->sh_deg > 3) return fail(c, SV_E_ARG, "sh_deg must be <= 3");
   if (k->plain_wf_flags > 3) return fail(c, SV_E_ARG, "plain_wf_flags must be a combination of SV_WF_*");
-  if (k->use_graph) return fail(c, SV_E_ARG, "use_graph is not available in this build");
   c->cfg = *k;
   return SV_OK;
 }
@@ -1239,17 +1417,17 @@ sv_status sv_read_keys(sv_ctx *c, int v, uint32_t *keys, uint32_t *vals) {
     return fail(c, SV_E_STATE, "tiles of view %d are not the last build", v);
   HIP_OK(c, hipStreamSynchronize(c->st));
   const uint64_t I = c->I;
-  if (vals && I) HIP_OK(c, hipMemcpy(vals, c->vals, I * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (vals && I) HIP_OK(c, hipMemcpy(vals, cur_vals(c), I * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (keys && I) {
     const uint64_t dmask = c->dbits ? ((1ull << c->dbits) - 1ull) : 0ull;
     if (c->key64) {
       std::vector<uint64_t> k(I);
-      HIP_OK(c, hipMemcpy(k.data(), c->keys, I * sizeof(uint64_t), hipMemcpyDeviceToHost));
+      HIP_OK(c, hipMemcpy(k.data(), cur_keys(c), I * sizeof(uint64_t), hipMemcpyDeviceToHost));
       for (uint64_t i = 0; i < I; ++i)
         keys[i] = (uint32_t)((((k[i] >> c->dbits) & 0xfffu) << 16) | (k[i] & dmask));
     } else {
       std::vector<uint32_t> k(I);
-      HIP_OK(c, hipMemcpy(k.data(), c->keys, I * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      HIP_OK(c, hipMemcpy(k.data(), cur_keys(c), I * sizeof(uint32_t), hipMemcpyDeviceToHost));
       for (uint64_t i = 0; i < I; ++i)
         keys[i] = (uint32_t)(((((uint64_t)k[i] >> c->dbits) & 0xfffu) << 16) | (k[i] & dmask));
     }

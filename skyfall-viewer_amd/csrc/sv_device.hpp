@@ -43,6 +43,30 @@ struct DevView {
   uint64_t pix_off;    // first pixel of this view in the pixel arrays
 };
 
+// Values of the current tile build that only the device knows; kernels
+// read them so that no launch depends on a host readback (the batched step
+// is captured in a hipGraph).
+struct StepMeta {
+  uint64_t n_isect;            // intersections (written by the scan)
+  uint32_t max_dq;             // largest depth quantum of a listed splat
+  uint32_t work;               // persistent raster work counter
+  uint32_t dbits;              // key bits of the depth quantum
+  uint32_t key_bits;           // tile bits + dbits
+  uint32_t passes;             // 8-bit sort passes needed
+  uint32_t sel;                // ping-pong buffer holding the sorted pairs
+  uint32_t overflow;           // n_isect > capacity: nothing past emit runs
+  uint32_t pad[7];
+};
+
+// Adam hyper-parameters of the current step (run_adam_on_buffer,
+// gaussian-renderer.ts:2409-2452), loaded on the device each step.
+struct AdamParams {
+  float lr[5];  // position, log-scale, quaternion, logit, SH
+  float beta1, beta2, eps, bc1_rcp, bc2_sqrt_rcp;
+  uint32_t update_sh;
+  uint32_t pad;
+};
+
 struct DevConfig {
   float bg[3];
   float near_plane, far_plane, radius_clip, eps2d;

@@ -23,36 +23,47 @@ void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, cons
 size_t scan_tmp_words(size_t M);
 void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t M, uint64_t *tmp,
                  uint64_t *total);
+// Derive dbits, key bits, sort passes, result buffer and overflow from
+// the scan total and the largest depth quantum.
+void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap);
 // emit_intersections (training-tiles.wgsl:201-247) for nv slots.
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
-                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits, K *keys,
-                 uint32_t *vals);
+                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
+                 K *keys, uint32_t *vals);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
 // host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
+// Grid sized for the capacity; n, dbits and the result buffer from m.
 template <typename K>
-void launch_ranges(hipStream_t st, const K *keys, uint64_t I, uint32_t dbits, uint32_t n_tiles,
-                   uint32_t *ranges);
+void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap, uint32_t n_tiles,
+                   const StepMeta *m, uint32_t *ranges);
 // Residual of training_loss (training-loss.wgsl:86-126) for one slot.
 void launch_loss(hipStream_t st, const DevView &hv, const DevConfig &cfg, const float *color,
                  const float *target, float *residual);
 
 // sv_sort.hip -----------------------------------------------------------
-// Stable LSD radix sort of (key, value) pairs over bits [0, bits).
-// Result ends in (keys, vals); (keys2, vals2) is scratch of the same size.
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits. Launches
+// max_passes passes sized for `cap` pairs; each reads n, the key width and
+// the number of passes from m and exits when not needed. Pass p reads
+// buffer p % 2 and writes buffer (p + 1) % 2; the result is in buffer
+// m->sel.
 template <typename K>
 size_t sort_tmp_words(uint64_t n);
 template <typename K>
-void launch_sort(hipStream_t st, K *keys, uint32_t *vals, K *keys2, uint32_t *vals2, uint64_t n,
-                 uint32_t bits, uint32_t *tmp, int *passes_out);
+void launch_sort(hipStream_t st, K *keys0, uint32_t *vals0, K *keys1, uint32_t *vals1, uint64_t cap,
+                 int max_passes, uint32_t *tmp, const StepMeta *m);
 
 // sv_raster.hip ---------------------------------------------------------
 struct RasterArgs {
   const DevView *views;
   const uint32_t *tile_view;  // global tile -> slot
   const uint32_t *ranges;     // n_tiles_total + 1
-  const uint32_t *vals;       // sorted Gaussian ids
-  uint64_t n_isect;           // intersections (bound of vals / adjoint records)
+  const uint32_t *vals0;      // sorted Gaussian ids: buffer 0 ...
+  const uint32_t *vals1;      // ... or buffer 1, per meta->sel
+  const StepMeta *meta;       // intersections, result buffer, overflow
+  // set on the device from meta at kernel entry (resolve_meta)
+  const uint32_t *vals;
+  uint64_t n_isect;
   const float4 *rec;          // 64-byte record per (slot, Gaussian)
   uint32_t N;                 // Gaussians (record stride per slot)
   uint32_t n_tiles_total;
@@ -93,20 +104,22 @@ struct VjpArgs {
   float *G;                // gradient planes (accumulate), or null when fused
   float *geom;             // 5 planes: grad_geom of the last slot (may be null)
   float4 *grad_accum;      // densify statistics (may be null)
+  const StepMeta *meta;    // overflow: no update
   // fused Adam (when G == null)
   float *Pw;               // parameters to update (== P)
   float *M1, *M2;
-  float lr[5];             // pos, scale, quat, logit, sh
-  float beta1, beta2, eps, bc1_rcp, bc2_sqrt_rcp;
-  int update_sh;
+  const AdamParams *adam;
 };
 void launch_vjp(hipStream_t st, const VjpArgs &a);
 void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G, size_t cap,
-                 uint32_t N, const float lr[5], float beta1, float beta2, float eps, float bc1_rcp,
-                 float bc2_sqrt_rcp, int update_sh);
+                 uint32_t N, const AdamParams *adam);
+// Copy the step's Adam parameters to the device: from mapped host memory
+// (src, re-read on every graph replay) or, with src null, from `value`.
+void launch_load_adam(hipStream_t st, const AdamParams &value, const AdamParams *src, AdamParams *dst);
 // backward_reduce (backward-reduce.wgsl:35-71) + geometry backward input.
-void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals, uint64_t I,
-                         size_t cap, float *G, float *geom);
+void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals0,
+                         const uint32_t *vals1, const StepMeta *m, uint64_t I, size_t cap, float *G,
+                         float *geom);
 // training_backward_geom (training-backward-geom.wgsl:36-299) from the
 // per-Gaussian grad_geom planes of one view.
 // Also accumulate_importance (importance-accum.wgsl:19-40) for that view

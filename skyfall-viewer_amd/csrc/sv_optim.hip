@@ -231,7 +231,7 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.N) return;
+  if (g >= a.N || (a.meta && a.meta->overflow)) return;
   float prm[11];
 #pragma unroll
   for (int k = 0; k < 11; ++k) {
@@ -324,15 +324,16 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     }
     return;
   }
+  const AdamParams &H = *a.adam;
 #pragma unroll
   for (int k = 0; k < P_NUM; ++k) {
     const int t = tensor_of_plane(k);
-    if (t == 4 && !a.update_sh) continue;
-    const float lr = a.lr[t];
+    if (t == 4 && !H.update_sh) continue;
+    const float lr = H.lr[t];
     if (t == 0 && !(lr > 0.0f)) continue;
     const size_t o = (size_t)k * a.cap + g;
     float p = a.Pw[o], m1 = a.M1[o], m2 = a.M2[o];
-    adam_elem(p, m1, m2, grad[k], lr, a.beta1, a.beta2, a.eps, a.bc1_rcp, a.bc2_sqrt_rcp);
+    adam_elem(p, m1, m2, grad[k], lr, H.beta1, H.beta2, H.eps, H.bc1_rcp, H.bc2_sqrt_rcp);
     a.Pw[o] = p;
     a.M1[o] = m1;
     a.M2[o] = m2;
@@ -348,31 +349,42 @@ void launch_vjp(hipStream_t st, const VjpArgs &a) {
 // planes: one thread per (plane, Gaussian), learning rate per tensor.
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ P, float *__restrict__ M1,
                                               float *__restrict__ M2, const float *__restrict__ G,
-                                              size_t cap, uint32_t N, float lr0, float lr1, float lr2,
-                                              float lr3, float lr4, float b1, float b2, float eps,
-                                              float bc1, float bc2s, int update_sh) {
+                                              size_t cap, uint32_t N, const AdamParams *__restrict__ hp) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
   if (g >= N) return;
+  const AdamParams &H = *hp;
   const int t = tensor_of_plane(k);
-  if (t == 4 && !update_sh) return;
-  const float lr = t == 0 ? lr0 : (t == 1 ? lr1 : (t == 2 ? lr2 : (t == 3 ? lr3 : lr4)));
+  if (t == 4 && !H.update_sh) return;
+  const float lr = H.lr[t];
   if (t == 0 && !(lr > 0.0f)) return;
   const size_t o = (size_t)k * cap + g;
   float p = P[o], m1 = M1[o], m2 = M2[o];
-  adam_elem(p, m1, m2, G[o], lr, b1, b2, eps, bc1, bc2s);
+  adam_elem(p, m1, m2, G[o], lr, H.beta1, H.beta2, H.eps, H.bc1_rcp, H.bc2_sqrt_rcp);
   P[o] = p;
   M1[o] = m1;
   M2[o] = m2;
 }
 
 void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G, size_t cap,
-                 uint32_t N, const float lr[5], float beta1, float beta2, float eps, float bc1_rcp,
-                 float bc2_sqrt_rcp, int update_sh) {
+                 uint32_t N, const AdamParams *adam) {
   if (N == 0) return;
-  hipLaunchKernelGGL(k_adam, dim3((N + 255) / 256, P_NUM), dim3(256), 0, st, P, M1, M2, G, cap, N,
-                     lr[0], lr[1], lr[2], lr[3], lr[4], beta1, beta2, eps, bc1_rcp, bc2_sqrt_rcp,
-                     update_sh);
+  hipLaunchKernelGGL(k_adam, dim3((N + 255) / 256, P_NUM), dim3(256), 0, st, P, M1, M2, G, cap, N, adam);
+}
+
+// One wave copies the step's Adam parameters into device memory: from
+// `src` (host memory mapped into the device address space, rewritten by
+// the host before each step) or, when src is null, from the value passed.
+__global__ void k_load_adam(AdamParams v, const AdamParams *src, AdamParams *dst) {
+  constexpr int W = sizeof(AdamParams) / 4;
+  const int i = threadIdx.x;
+  if (i >= W) return;
+  const uint32_t *s = src ? (const uint32_t *)src : (const uint32_t *)&v;
+  ((uint32_t *)dst)[i] = s[i];
+}
+
+void launch_load_adam(hipStream_t st, const AdamParams &value, const AdamParams *src, AdamParams *dst) {
+  hipLaunchKernelGGL(k_load_adam, dim3(1), dim3(64), 0, st, value, src, dst);
 }
 
 // backward_reduce (backward-reduce.wgsl:35-71): per intersection, the 54
@@ -380,12 +392,13 @@ void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G,
 // to grad_opacity, grad_geom and grad_sh (order not deterministic, as in
 // the reference).
 __global__ __launch_bounds__(256) void k_reduce_fixed(const int32_t *__restrict__ fixed,
-                                                      const uint32_t *__restrict__ vals, uint64_t I,
-                                                      size_t cap, float *__restrict__ G,
-                                                      float *__restrict__ geom) {
+                                                      const uint32_t *__restrict__ vals0,
+                                                      const uint32_t *__restrict__ vals1,
+                                                      const StepMeta *m, size_t cap,
+                                                      float *__restrict__ G, float *__restrict__ geom) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= I) return;
-  const uint32_t g = vals[i];
+  if (m->overflow || i >= m->n_isect) return;
+  const uint32_t g = (m->sel ? vals1 : vals0)[i];
   if ((size_t)g >= cap) return;
   const int32_t *s = fixed + i * 54;
   const float inv = 1.0f / 65536.0f;
@@ -401,11 +414,12 @@ __global__ __launch_bounds__(256) void k_reduce_fixed(const int32_t *__restrict_
   }
 }
 
-void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals, uint64_t I,
-                         size_t cap, float *G, float *geom) {
+void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals0,
+                         const uint32_t *vals1, const StepMeta *m, uint64_t I, size_t cap, float *G,
+                         float *geom) {
   if (I == 0) return;
-  hipLaunchKernelGGL(k_reduce_fixed, dim3((uint32_t)((I + 255) / 256)), dim3(256), 0, st, fixed, vals,
-                     I, cap, G, geom);
+  hipLaunchKernelGGL(k_reduce_fixed, dim3((uint32_t)((I + 255) / 256)), dim3(256), 0, st, fixed, vals0,
+                     vals1, m, cap, G, geom);
 }
 
 // training_backward_geom from the grad_geom planes (wavefront path) +

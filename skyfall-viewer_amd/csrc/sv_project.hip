@@ -2,6 +2,8 @@
 // tile ranges and the loss residual (gfx950).
 #include "sv_internal.hpp"
 
+#include <algorithm>
+
 namespace sv {
 
 // ---------------------------------------------------------------------
@@ -300,15 +302,41 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // (camera, tile, depth_q) restricted to the keys present. Value =
 // Gaussian index. Order within a Gaussian: ty-major, then tx (:238-246).
 // ---------------------------------------------------------------------
+// Key layout of the step from the scan total and the largest depth
+// quantum: dbits = bit length of max depth_q, key bits = tile bits + dbits,
+// 8-bit sort passes; overflow 1 = more pairs than the buffers hold (the
+// host grows them and runs the step again), 2 = key wider than the key
+// type. With an overflow nothing after this point writes any state.
+__global__ void k_keyinfo(StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const uint64_t n = m->n_isect;
+  const uint32_t dbits = 32u - (uint32_t)__clz((int)m->max_dq);
+  const uint32_t kb = tile_bits + dbits;
+  uint32_t of = 0;
+  if (n > cap || n >= 0xffffffffull) of = 1;
+  else if (kb > key_type_bits) of = 2;
+  const uint32_t passes = (of || n <= 1) ? 0u : (kb + 7u) / 8u;
+  m->dbits = dbits;
+  m->key_bits = kb;
+  m->passes = passes;
+  m->sel = passes & 1u;
+  m->overflow = of;
+}
+
+void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap) {
+  hipLaunchKernelGGL(k_keyinfo, dim3(1), dim3(64), 0, st, m, tile_bits, key_type_bits, cap);
+}
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
                                               const float2 *__restrict__ aux,
                                               const uint32_t *__restrict__ counts,
                                               const uint32_t *__restrict__ offs, uint32_t N, int nv,
-                                              const DevView *__restrict__ views, uint32_t dbits,
+                                              const DevView *__restrict__ views, const StepMeta *m,
                                               K *__restrict__ keys, uint32_t *__restrict__ vals) {
   const size_t vg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (vg >= (size_t)nv * N) return;
+  if (vg >= (size_t)nv * N || m->overflow) return;
+  const uint32_t dbits = m->dbits;
   const uint32_t cnt = counts[vg];
   if (cnt == 0u) return;
   const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
@@ -331,18 +359,18 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
 
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
-                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, uint32_t dbits, K *keys,
-                 uint32_t *vals) {
+                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
+                 K *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
   hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, aux, counts,
-                     offs, N, nv, views, dbits, keys, vals);
+                     offs, N, nv, views, m, keys, vals);
 }
 template void launch_emit<uint32_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
-                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
+                                    const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
                                     uint32_t *, uint32_t *);
 template void launch_emit<uint64_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
-                                    const uint32_t *, uint32_t, int, const DevView *, uint32_t,
+                                    const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
                                     uint64_t *, uint32_t *);
 
 // ---------------------------------------------------------------------
@@ -351,11 +379,22 @@ template void launch_emit<uint64_t>(hipStream_t, float4 *, const float2 *, const
 // computes (training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
 // Every entry is written by exactly one thread; no atomics.
 // ---------------------------------------------------------------------
+// Launched for max(cap, n_tiles + 1) threads; the live count, key layout
+// and sorted buffer come from the step metadata.
 template <typename K>
-__global__ __launch_bounds__(256) void k_ranges(const K *__restrict__ keys, uint64_t I, uint32_t dbits,
-                                                uint32_t n_tiles, uint32_t *__restrict__ ranges) {
+__global__ __launch_bounds__(256) void k_ranges(const K *__restrict__ keys0, const K *__restrict__ keys1,
+                                                const StepMeta *m, uint32_t n_tiles,
+                                                uint32_t *__restrict__ ranges) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m->overflow) return;
+  const uint64_t I = m->n_isect;
+  if (I == 0) {
+    if (i <= n_tiles) ranges[i] = 0u;
+    return;
+  }
   if (i >= I) return;
+  const K *__restrict__ keys = m->sel ? keys1 : keys0;
+  const uint32_t dbits = m->dbits;
   const uint32_t t = (uint32_t)(keys[i] >> dbits);
   const uint32_t lo = (i == 0) ? 0u : (uint32_t)(keys[i - 1] >> dbits) + 1u;
   for (uint32_t u = lo; u <= t && u <= n_tiles; ++u) ranges[u] = (uint32_t)i;
@@ -381,19 +420,16 @@ void launch_fill_f32(hipStream_t st, float *p, float v, size_t n) {
 }
 
 template <typename K>
-void launch_ranges(hipStream_t st, const K *keys, uint64_t I, uint32_t dbits, uint32_t n_tiles,
-                   uint32_t *ranges) {
-  if (I == 0) {
-    launch_fill_u32(st, ranges, 0u, (size_t)n_tiles + 1);
-    return;
-  }
-  hipLaunchKernelGGL(k_ranges<K>, dim3((uint32_t)((I + 255) / 256)), dim3(256), 0, st, keys, I, dbits,
-                     n_tiles, ranges);
+void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap, uint32_t n_tiles,
+                   const StepMeta *m, uint32_t *ranges) {
+  const uint64_t threads = std::max<uint64_t>(cap, (uint64_t)n_tiles + 1);
+  hipLaunchKernelGGL(k_ranges<K>, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st, keys0, keys1,
+                     m, n_tiles, ranges);
 }
-template void launch_ranges<uint32_t>(hipStream_t, const uint32_t *, uint64_t, uint32_t, uint32_t,
-                                      uint32_t *);
-template void launch_ranges<uint64_t>(hipStream_t, const uint64_t *, uint64_t, uint32_t, uint32_t,
-                                      uint32_t *);
+template void launch_ranges<uint32_t>(hipStream_t, const uint32_t *, const uint32_t *, uint64_t, uint32_t,
+                                      const StepMeta *, uint32_t *);
+template void launch_ranges<uint64_t>(hipStream_t, const uint64_t *, const uint64_t *, uint64_t, uint32_t,
+                                      const StepMeta *, uint32_t *);
 
 // ---------------------------------------------------------------------
 // training_loss (training-loss.wgsl:86-126) for one view: residual =

@@ -13,6 +13,16 @@ namespace sv {
 constexpr int WT = 64;  // threads per tile (one wave)
 constexpr int PPL = 4;  // pixels per lane
 
+// Sorted id buffer and pair count of the step from its metadata; false
+// when the step overflowed (nothing is rendered or written).
+__device__ __forceinline__ bool resolve_meta(RasterArgs &a) {
+  const StepMeta *m = a.meta;
+  if (m->overflow) return false;
+  a.vals = m->sel ? a.vals1 : a.vals0;
+  a.n_isect = m->n_isect;
+  return true;
+}
+
 __device__ __forceinline__ bool next_tile(const RasterArgs &a, uint32_t &tg, uint32_t *s_tg) {
   if (a.work_counter == nullptr) return false;
   __syncthreads();
@@ -145,7 +155,9 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
 // Empty tile: bg, A = 0, last = NO_LAST.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(WT) void k_forward(RasterArgs a) {
+__global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
+  RasterArgs a = ain;
+  if (!resolve_meta(a)) return;
   __shared__ float4 s_q0[B];
   __shared__ float4 s_q1[B];
   __shared__ float4 s_q2[B];
@@ -259,7 +271,9 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs a) {
 // a.residual.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(WT) void k_backward(RasterArgs a) {
+__global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
+  RasterArgs a = ain;
+  if (!resolve_meta(a)) return;
   __shared__ float4 s_q0[B];
   __shared__ float4 s_q1[B];
   __shared__ float4 s_q2[B];
@@ -442,9 +456,11 @@ constexpr int RT = 256;
 constexpr int WF_B = 64;
 constexpr int WF_S = 54;
 
-__global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs a, const float *__restrict__ P,
+__global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs ain, const float *__restrict__ P,
                                                        size_t cap, uint32_t flags,
                                                        int32_t *__restrict__ fixed) {
+  RasterArgs a = ain;
+  if (!resolve_meta(a)) return;
   __shared__ int32_t s_acc[WF_B * WF_S];
   __shared__ float4 s_x0[WF_B];
   __shared__ float4 s_x1[WF_B];

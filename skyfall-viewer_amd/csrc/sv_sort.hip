@@ -19,11 +19,29 @@ constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_T * SORT_ITEMS;  // 4096 keys per workgroup
 constexpr int NWAVE = SORT_T / 64;
 
+// Pairs, tiles and whether pass `pass` runs, from the step metadata.
+struct PassInfo {
+  uint64_t n;
+  uint32_t nb, shift;
+  bool run;
+};
+__device__ __forceinline__ PassInfo pass_info(const StepMeta *m, uint32_t pass) {
+  PassInfo p;
+  p.n = m->n_isect;
+  p.run = pass < m->passes;
+  p.nb = (uint32_t)((p.n + SORT_TILE - 1) / SORT_TILE);
+  p.shift = 8u * pass;
+  return p;
+}
+
 template <typename K, int RB>
-__global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ keys, uint64_t n,
-                                                      uint32_t shift, uint32_t nb,
-                                                      uint32_t *__restrict__ hist) {
+__global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ keys, const StepMeta *m,
+                                                      uint32_t pass, uint32_t *__restrict__ hist) {
   constexpr int RADIX = 1 << RB;
+  const PassInfo pi = pass_info(m, pass);
+  if (!pi.run || blockIdx.x >= pi.nb) return;
+  const uint64_t n = pi.n;
+  const uint32_t shift = pi.shift, nb = pi.nb;
   __shared__ uint32_t s_h[NWAVE][RADIX];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_h[0][0])[i] = 0u;
@@ -58,8 +76,12 @@ __device__ __forceinline__ uint32_t block_reduce_u32(uint32_t v, uint32_t *s_w) 
 }
 
 // One workgroup per digit: total of the digit over all tiles.
-__global__ __launch_bounds__(SORT_T) void k_sort_rowsum(const uint32_t *__restrict__ hist, uint32_t nb,
+__global__ __launch_bounds__(SORT_T) void k_sort_rowsum(const uint32_t *__restrict__ hist,
+                                                       const StepMeta *m, uint32_t pass,
                                                        uint32_t *__restrict__ dtot) {
+  const PassInfo pi = pass_info(m, pass);
+  if (!pi.run) return;
+  const uint32_t nb = pi.nb;
   __shared__ uint32_t s_w[NWAVE];
   const uint32_t d = blockIdx.x;
   uint32_t s = 0;
@@ -69,8 +91,12 @@ __global__ __launch_bounds__(SORT_T) void k_sort_rowsum(const uint32_t *__restri
 }
 
 // One workgroup per digit: hist row -> exclusive scan + digit base.
-__global__ __launch_bounds__(SORT_T) void k_sort_rowscan(uint32_t *__restrict__ hist, uint32_t nb,
+__global__ __launch_bounds__(SORT_T) void k_sort_rowscan(uint32_t *__restrict__ hist,
+                                                        const StepMeta *m, uint32_t pass,
                                                         const uint32_t *__restrict__ dtot) {
+  const PassInfo pi = pass_info(m, pass);
+  if (!pi.run) return;
+  const uint32_t nb = pi.nb;
   __shared__ uint32_t s_w[NWAVE];
   const uint32_t d = blockIdx.x;
   uint32_t part = 0;
@@ -104,11 +130,15 @@ template <typename K, int RB>
 __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ kin,
                                                         const uint32_t *__restrict__ vin,
                                                         K *__restrict__ kout,
-                                                        uint32_t *__restrict__ vout, uint64_t n,
-                                                        uint32_t shift, uint32_t nb,
+                                                        uint32_t *__restrict__ vout,
+                                                        const StepMeta *m, uint32_t pass,
                                                         const uint32_t *__restrict__ hist) {
   constexpr int RADIX = 1 << RB;
   constexpr int DPT = RADIX / SORT_T;  // digits per thread in the block scan
+  const PassInfo pi = pass_info(m, pass);
+  if (!pi.run || blockIdx.x >= pi.nb) return;
+  const uint64_t n = pi.n;
+  const uint32_t shift = pi.shift, nb = pi.nb;
   __shared__ uint32_t s_wcnt[NWAVE][RADIX];
   __shared__ uint32_t s_lbase[RADIX];
   __shared__ uint32_t s_gbase[RADIX];
@@ -226,49 +256,34 @@ size_t sort_tmp_words(uint64_t n) {
   return (size_t)(nb * (1u << DIGIT_BITS) + (1u << DIGIT_BITS));
 }
 
-template <typename K, int RB>
-static void sort_pass(hipStream_t st, const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
-                      uint64_t n, uint32_t shift, uint32_t nb, uint32_t *hist, uint32_t *dtot) {
-  constexpr int RADIX = 1 << RB;
-  hipLaunchKernelGGL((k_sort_count<K, RB>), dim3(nb), dim3(SORT_T), 0, st, kin, n, shift, nb, hist);
-  hipLaunchKernelGGL(k_sort_rowsum, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
-  hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(SORT_T), 0, st, hist, nb, dtot);
-  hipLaunchKernelGGL((k_sort_scatter<K, RB>), dim3(nb), dim3(SORT_T), 0, st, kin, vin, kout, vout, n,
-                     shift, nb, hist);
-}
-
+// Passes are launched for `cap` pairs; blocks past the live pair count and
+// passes past m->passes return at once, so the same launches serve any
+// pair count up to cap (and can be captured in a graph).
 template <typename K>
-void launch_sort(hipStream_t st, K *keys, uint32_t *vals, K *keys2, uint32_t *vals2, uint64_t n,
-                 uint32_t bits, uint32_t *tmp, int *passes_out) {
-  const uint32_t nb = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
-  const int rb = DIGIT_BITS;
-  const int passes = (int)((bits + rb - 1) / rb);
-  *passes_out = passes;
-  if (n <= 1 || passes == 0) {
-    *passes_out = 0;
-    return;
-  }
+void launch_sort(hipStream_t st, K *keys0, uint32_t *vals0, K *keys1, uint32_t *vals1, uint64_t cap,
+                 int max_passes, uint32_t *tmp, const StepMeta *m) {
+  const uint32_t nb = (uint32_t)((cap + SORT_TILE - 1) / SORT_TILE);
+  if (nb == 0) return;
+  constexpr int RADIX = 1 << DIGIT_BITS;
   uint32_t *hist = tmp;
-  uint32_t *dtot = tmp + (size_t)nb * (1u << DIGIT_BITS);
-  K *kin = keys, *kout = keys2;
-  uint32_t *vin = vals, *vout = vals2;
-  for (int p = 0; p < passes; ++p) {
-    const uint32_t shift = (uint32_t)(rb * p);
-    sort_pass<K, DIGIT_BITS>(st, kin, vin, kout, vout, n, shift, nb, hist, dtot);
-    K *tk = kin;
-    kin = kout;
-    kout = tk;
-    uint32_t *tv = vin;
-    vin = vout;
-    vout = tv;
+  uint32_t *dtot = tmp + (size_t)nb * RADIX;
+  for (int p = 0; p < max_passes; ++p) {
+    K *kin = (p & 1) ? keys1 : keys0, *kout = (p & 1) ? keys0 : keys1;
+    uint32_t *vin = (p & 1) ? vals1 : vals0, *vout = (p & 1) ? vals0 : vals1;
+    hipLaunchKernelGGL((k_sort_count<K, DIGIT_BITS>), dim3(nb), dim3(SORT_T), 0, st, kin, m, (uint32_t)p,
+                       hist);
+    hipLaunchKernelGGL(k_sort_rowsum, dim3(RADIX), dim3(SORT_T), 0, st, hist, m, (uint32_t)p, dtot);
+    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(SORT_T), 0, st, hist, m, (uint32_t)p, dtot);
+    hipLaunchKernelGGL((k_sort_scatter<K, DIGIT_BITS>), dim3(nb), dim3(SORT_T), 0, st, kin, vin, kout,
+                       vout, m, (uint32_t)p, hist);
   }
 }
 
 template size_t sort_tmp_words<uint32_t>(uint64_t);
 template size_t sort_tmp_words<uint64_t>(uint64_t);
 template void launch_sort<uint32_t>(hipStream_t, uint32_t *, uint32_t *, uint32_t *, uint32_t *,
-                                    uint64_t, uint32_t, uint32_t *, int *);
+                                    uint64_t, int, uint32_t *, const StepMeta *);
 template void launch_sort<uint64_t>(hipStream_t, uint64_t *, uint32_t *, uint64_t *, uint32_t *,
-                                    uint64_t, uint32_t, uint32_t *, int *);
+                                    uint64_t, int, uint32_t *, const StepMeta *);
 
 }  // namespace sv

@@ -256,7 +256,10 @@ class Context:
 
     def set_config(self, cfg: Config = None, **kw):
         c = cfg if cfg is not None else self.config()
+        names = {f[0] for f in Config._fields_}
         for k, v in kw.items():
+            if k not in names:
+                raise AttributeError(f"sv_config has no field {k!r}")
             if k == "bg":
                 for i in range(3):
                     c.bg[i] = v[i]
