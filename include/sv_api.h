@@ -236,6 +236,20 @@ sv_status sv_prune_opacity(sv_ctx *ctx, float min_opacity, size_t *n_removed);
  * get zero moments. */
 sv_status sv_densify(sv_ctx *ctx, const uint8_t *actions, uint64_t seed,
                      size_t *n_added);
+/* Densify decision on the device, replacing the candidate selection of
+ * performDensification (gaussian-renderer.ts:3249-3326) and the per-view
+ * scoring of getMultiViewScores (:2640-2798): the views are rendered, each
+ * tile's L1 error against its target is scored, each view's threshold is
+ * its tile errors' `percentile` (value at floor(p (T - 1)) of the sorted
+ * errors); a Gaussian listed on a tile above its view's threshold is
+ * split (largest projected radius > split_px) or cloned. actions: N bytes
+ * (0 none, 1 clone, 2 split), ready for sv_densify. */
+sv_status sv_densify_decide(sv_ctx *ctx, const int *views, int nviews, float percentile,
+                            float split_px, uint8_t *actions, size_t *n_clone,
+                            size_t *n_split);
+/* Per-tile L1 error of view `view` from the last sv_densify_decide
+ * (tiles_x * tiles_y floats, row-major tiles). */
+sv_status sv_read_tile_l1(sv_ctx *ctx, int view, float *out);
 
 /* --- timing --------------------------------------------------------- */
 /* Enable per-kernel hipEvent timing on the context stream. */

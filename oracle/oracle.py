@@ -256,3 +256,71 @@ def compact_rows(src: np.ndarray, keep: np.ndarray) -> np.ndarray:
     n = lib().orc_compact_rows(C.c_size_t(s.shape[0]), C.c_size_t(stride), _fp(s), k.ctypes.data_as(C.POINTER(C.c_uint8)),
                                _fp(d))
     return d[:n]
+
+
+PARAM_KEYS = ("pos", "logscale", "quat", "logit", "sh")
+PARAM_WIDTH = {"pos": 3, "logscale": 3, "quat": 4, "logit": 1, "sh": 48}
+
+
+def to_rows(g: dict) -> np.ndarray:
+    """Parameters (or moments) as rows of 59 floats: pos, log-scale, quat, logit, SH."""
+    n = g["pos"].shape[0]
+    return np.ascontiguousarray(np.concatenate([np.asarray(g[k], np.float32).reshape(n, -1) for k in PARAM_KEYS], 1))
+
+
+def from_rows(rows: np.ndarray) -> dict:
+    out, c = {}, 0
+    for k in PARAM_KEYS:
+        w = PARAM_WIDTH[k]
+        out[k] = np.ascontiguousarray(rows[:, c:c + w] if w > 1 else rows[:, c])
+        c += w
+    return out
+
+
+def prune_keep(logit: np.ndarray, min_opacity: float) -> np.ndarray:
+    lg = np.ascontiguousarray(logit, np.float32)
+    keep = np.zeros(lg.size, np.uint8)
+    lib().orc_prune_keep(C.c_size_t(lg.size), _fp(lg), C.c_float(min_opacity), keep.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return keep
+
+
+def densify(g: dict, m1: dict, m2: dict, actions: np.ndarray, seed: int):
+    """Returns (params, m1, m2) dicts after clone / split."""
+    r, a1, a2 = to_rows(g), to_rows(m1), to_rows(m2)
+    act = np.ascontiguousarray(actions, np.uint8)
+    n = r.shape[0]
+    n_out = int(n + np.count_nonzero(act == 1) + np.count_nonzero(act == 2))
+    o, o1, o2 = (np.zeros((max(n_out, 1), 59), np.float32) for _ in range(3))
+    lib().orc_densify.restype = C.c_size_t
+    w = lib().orc_densify(C.c_size_t(n), _fp(r), _fp(a1), _fp(a2), act.ctypes.data_as(C.POINTER(C.c_uint8)),
+                          C.c_uint64(seed), _fp(o), _fp(o1), _fp(o2))
+    assert w == n_out
+    return from_rows(o[:w]), from_rows(o1[:w]), from_rows(o2[:w])
+
+
+def tile_l1(view: OrcView, color: np.ndarray, target: np.ndarray) -> np.ndarray:
+    c = np.ascontiguousarray(color, np.float32)
+    t = np.ascontiguousarray(target, np.float32)
+    out = np.zeros(view.tiles_x * view.tiles_y, np.float32)
+    lib().orc_tile_l1(C.byref(view), _fp(c), _fp(t), _fp(out))
+    return out
+
+
+def densify_decide(projs, views, tile_l1s, thr, split_px: float) -> np.ndarray:
+    nv = len(projs)
+    n = projs[0].shape[0]
+    parr = (C.c_void_p * nv)(*[p.ctypes.data_as(C.c_void_p) for p in projs])
+    varr = (OrcView * nv)(*views)
+    tl = [np.ascontiguousarray(t, np.float32) for t in tile_l1s]
+    tarr = (C.POINTER(C.c_float) * nv)(*[_fp(t) for t in tl])
+    th = np.ascontiguousarray(thr, np.float32)
+    act = np.zeros(n, np.uint8)
+    lib().orc_densify_decide(C.c_size_t(n), C.c_size_t(nv), parr, varr, tarr, _fp(th), C.c_float(split_px),
+                             act.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return act
+
+
+def percentile_lower(x: np.ndarray, p: float) -> float:
+    """The value at index floor(p (n - 1)) of the sorted values."""
+    s = np.sort(np.asarray(x, np.float32))
+    return float(s[int(np.floor(p * (s.size - 1)))])

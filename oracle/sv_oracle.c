@@ -36,6 +36,11 @@
  *                       -lr*g/(|g|+eps)
  *   orc_accumulate_*    closed forms
  *   orc_compact_rows    numpy boolean indexing
+ *   orc_prune_keep, orc_densify, orc_tile_l1, orc_densify_decide
+ *                       parity unpinned against the reference (its rule is
+ *                       weighted random, its draws Math.random); pinned by
+ *                       numpy restatements and invariants (counts, order,
+ *                       zero child moments, split offsets ~ N(0, sigma))
  */
 #include "sv_oracle.h"
 
@@ -1302,4 +1307,132 @@ void orc_f16_round_v(size_t n, const float *x, float *y) {
 }
 void orc_sh_color_v(size_t n, const float *dirs, const float *sh48, uint32_t deg, float *rgb) {
   for (size_t i = 0; i < n; ++i) orc_sh_color(&dirs[3 * i], &sh48[48 * i], deg, &rgb[3 * i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* a18: prune / densify compaction                                      */
+/* ------------------------------------------------------------------ */
+void orc_prune_keep(size_t n, const float *logit, float min_opacity, uint8_t *keep) {
+  for (size_t i = 0; i < n; ++i) keep[i] = sigmoid_ref(logit[i]) < min_opacity ? 0u : 1u;
+}
+
+static uint64_t splitmix_at(uint64_t seed, uint64_t ctr) {
+  uint64_t z = seed + (ctr + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static const double ORC_PI = 3.14159265358979323846;
+static double unif_at(uint64_t seed, uint64_t ctr) {
+  return (double)(splitmix_at(seed, ctr) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+size_t orc_densify(size_t n, const float *rows, const float *m1, const float *m2,
+                   const uint8_t *actions, uint64_t seed, float *out, float *out_m1,
+                   float *out_m2) {
+  enum { S = 59 };
+  size_t w = 0;
+  /* survivors (gaussian-renderer.ts:3344-3370) */
+  for (size_t i = 0; i < n; ++i) {
+    if (actions[i] == 2) continue;
+    memcpy(&out[w * S], &rows[i * S], S * sizeof(float));
+    memcpy(&out_m1[w * S], &m1[i * S], S * sizeof(float));
+    memcpy(&out_m2[w * S], &m2[i * S], S * sizeof(float));
+    ++w;
+  }
+  /* children (:3372-3433) */
+  for (size_t i = 0; i < n; ++i) {
+    const uint8_t a = actions[i];
+    if (a == 0) continue;
+    const float *r = &rows[i * S];
+    const double ls[3] = {r[3], r[4], r[5]};
+    const double p[3] = {r[0], r[1], r[2]};
+    const uint64_t base = 16ull * (uint64_t)i;
+    const int nc = a == 1 ? 1 : 2;
+    for (int c = 0; c < nc; ++c) {
+      float *o = &out[w * S];
+      memcpy(o, r, S * sizeof(float));
+      memset(&out_m1[w * S], 0, S * sizeof(float));
+      memset(&out_m2[w * S], 0, S * sizeof(float));
+      double off[3];
+      if (a == 1) {
+        /* clone: max sigma * 0.25 * (0.5 + 0.5 u) along a random direction */
+        const double js = fmax(fmax(exp(ls[0]), exp(ls[1])), exp(ls[2]));
+        const double mag = js * 0.25 * (0.5 + unif_at(seed, base + 0) * 0.5);
+        double d[3] = {unif_at(seed, base + 1) * 2.0 - 1.0, unif_at(seed, base + 2) * 2.0 - 1.0,
+                       unif_at(seed, base + 3) * 2.0 - 1.0};
+        double nrm = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (nrm == 0.0) nrm = 1.0;
+        for (int k = 0; k < 3; ++k) off[k] = d[k] / nrm * mag;
+      } else {
+        /* split: R * N(0, sigma) (sampleGaussian3D, Box-Muller on four
+         * draws, :3103-3120; quatToMat3 of the stored quaternion,
+         * :3086-3098), log-scale - ln(1.6) */
+        const double sg[3] = {exp(ls[0]), exp(ls[1]), exp(ls[2])};
+        const uint64_t cb = base + 4ull * (uint64_t)c;
+        const double u1 = unif_at(seed, cb + 0), u2 = unif_at(seed, cb + 1);
+        const double u3 = unif_at(seed, cb + 2), u4 = unif_at(seed, cb + 3);
+        const double r1 = sqrt(-2.0 * log(1.0 - u1)), r3 = sqrt(-2.0 * log(1.0 - u3));
+        const double z[3] = {r1 * cos(2.0 * ORC_PI * u2) * sg[0], r1 * sin(2.0 * ORC_PI * u2) * sg[1],
+                             r3 * cos(2.0 * ORC_PI * u4) * sg[2]};
+        const double x = r[6], y = r[7], zq = r[8], wq = r[9];
+        const double x2 = x + x, y2 = y + y, z2 = zq + zq;
+        const double xx = x * x2, xy = x * y2, xz = x * z2, yy = y * y2, yz = y * z2, zz = zq * z2;
+        const double wx = wq * x2, wy = wq * y2, wz = wq * z2;
+        const double R[3][3] = {{1 - (yy + zz), xy - wz, xz + wy},
+                                {xy + wz, 1 - (xx + zz), yz - wx},
+                                {xz - wy, yz + wx, 1 - (xx + yy)}};
+        for (int k = 0; k < 3; ++k) off[k] = R[k][0] * z[0] + R[k][1] * z[1] + R[k][2] * z[2];
+        const double red = log(0.8 * 2.0);
+        for (int k = 0; k < 3; ++k) o[3 + k] = (float)(ls[k] - red);
+      }
+      for (int k = 0; k < 3; ++k) o[k] = (float)(p[k] + off[k]);
+      ++w;
+    }
+  }
+  return w;
+}
+
+void orc_tile_l1(const orc_view *v, const float *color, const float *target, float *tile_l1) {
+  const uint32_t W = v->width, H = v->height, ts = v->tile_size;
+  const uint32_t nt = v->tiles_x * v->tiles_y;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const uint32_t tx = t % v->tiles_x, ty = t / v->tiles_x;
+    float slot[256];
+    for (uint32_t k = 0; k < 256; ++k) {
+      const uint32_t x = tx * ts + (k & 15u), y = ty * ts + (k >> 4);
+      slot[k] = 0.0f;
+      if (x < W && y < H) {
+        const size_t q = ((size_t)y * W + x) * 3;
+        slot[k] = (fabsf(color[q] - target[q]) + fabsf(color[q + 1] - target[q + 1])) +
+                  fabsf(color[q + 2] - target[q + 2]);
+      }
+    }
+    for (uint32_t s = 128; s > 0; s >>= 1)
+      for (uint32_t k = 0; k < s; ++k) slot[k] = slot[k] + slot[k + s];
+    tile_l1[t] = slot[0];
+  }
+}
+
+void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
+                        const orc_view *views, const float *const *tile_l1,
+                        const float *thr, float split_px, uint8_t *actions) {
+  for (size_t i = 0; i < n; ++i) {
+    int cand = 0;
+    float rmax = 0.0f;
+    for (size_t s = 0; s < nviews; ++s) {
+      const orc_proj *gp = &proj[s][i];
+      if (gp->tiles_count == 0u) continue;
+      rmax = fmaxf(rmax, gp->radius);
+      uint32_t r[4];
+      tile_rect(&views[s], gp->mean2d[0], gp->mean2d[1], gp->radius, r);
+      for (uint32_t ty = r[1]; ty < r[3] && !cand; ++ty)
+        for (uint32_t tx = r[0]; tx < r[2]; ++tx)
+          if (tile_l1[s][ty * views[s].tiles_x + tx] > thr[s]) {
+            cand = 1;
+            break;
+          }
+    }
+    actions[i] = !cand ? 0u : (rmax > split_px ? 2u : 1u);
+  }
 }

@@ -159,6 +159,34 @@ double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
 size_t orc_compact_rows(size_t n, size_t stride, const float *src,
                         const uint8_t *keep, float *dst);
 
+/* Prune rule: keep[i] = !(sigmoid(logit_i) < min_opacity), sigmoid as
+ * training-forward-tiles.wgsl:188. */
+void orc_prune_keep(size_t n, const float *logit, float min_opacity, uint8_t *keep);
+
+/* performDensification (gaussian-renderer.ts:3211-3466) on rows of 59
+ * floats (position 3, log-scale 3, quaternion xyzw 4, logit 1, SH 48) with
+ * their two moment rows. actions: 0 keep, 1 clone, 2 split. Output: every
+ * row that is not split, in order, then the children in parent order (clone:
+ * 1, split: 2) with zero moments. The reference's Math.random draws come
+ * from the counter stream u(seed, 16 g + k). Returns the output count. */
+size_t orc_densify(size_t n, const float *rows, const float *m1, const float *m2,
+                   const uint8_t *actions, uint64_t seed, float *out, float *out_m1,
+                   float *out_m2);
+
+/* Per-tile L1 error of one view: e_p = (|dr| + |dg|) + |db| per pixel, the
+ * 256 pixel slots of the tile in row-major order (0 outside the image),
+ * summed by a fixed pairwise tree (slot i += slot i + s, s = 128 .. 1).
+ * color / target: H x W x 3. */
+void orc_tile_l1(const orc_view *v, const float *color, const float *target, float *tile_l1);
+
+/* Densify rule (config 1): a Gaussian is a candidate when a tile of its
+ * rectangle in a view where it is listed has L1 error above that view's
+ * threshold; candidates split when their largest projected radius over
+ * those views exceeds split_px, else clone. proj[v] / tile_l1[v] per view. */
+void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
+                        const orc_view *views, const float *const *tile_l1,
+                        const float *thr, float split_px, uint8_t *actions);
+
 /* Vector helpers for the tests. */
 void orc_expf_v(size_t n, const float *x, float *y);
 void orc_f16_round_v(size_t n, const float *x, float *y);

@@ -97,7 +97,10 @@ struct sv_ctx {
   uint32_t tiles_total = 0;
   uint32_t *tile_view = nullptr, *ranges = nullptr;
   float *l1_partial = nullptr;
+  float *tile_l1 = nullptr;  // densify scoring, per global tile
   size_t tiles_cap = 0;
+  std::vector<int> scored_views;  // batch of the last sv_densify_decide
+  std::vector<uint32_t> scored_base;
   float4 *rec = nullptr;
   float2 *aux = nullptr;
   uint32_t *counts = nullptr, *offs = nullptr;
@@ -331,11 +334,14 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       dfree(c->tile_view);
       dfree(c->ranges);
       dfree(c->l1_partial);
+      dfree(c->tile_l1);
       c->tiles_cap = 0;
+      c->scored_views.clear();
       size_t want = (size_t)tb + 1;
       if (sv_status s = dalloc(c, &c->tile_view, want)) return s;
       if (sv_status s = dalloc(c, &c->ranges, want)) return s;
       if (sv_status s = dalloc(c, &c->l1_partial, want)) return s;
+      if (sv_status s = dalloc(c, &c->tile_l1, want)) return s;
       c->tiles_cap = want;
     }
     std::vector<uint32_t> tv(tb);
@@ -1082,6 +1088,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->tile_view);
   dfree(c->ranges);
   dfree(c->l1_partial);
+  dfree(c->tile_l1);
   dfree(c->rec);
   dfree(c->aux);
   dfree(c->counts);
@@ -1609,6 +1616,76 @@ sv_status sv_densify(sv_ctx *c, const uint8_t *actions, uint64_t seed, size_t *n
   dfree(child_offs);
   dfree(tmp);
   return swap_in(c, Pn, M1n, M2n, cap_n, (uint32_t)n_new);
+}
+
+sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, float split_px,
+                            uint8_t *actions, size_t *n_clone, size_t *n_split) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!vs || nv <= 0 || !actions) return fail(c, SV_E_ARG, "views and actions are required");
+  if (!(percentile >= 0.0f && percentile <= 1.0f)) return fail(c, SV_E_ARG, "percentile must be in [0, 1]");
+  for (int i = 0; i < nv; ++i) {
+    if (sv_status s = check_view(c, vs[i])) return s;
+    if (!c->views[vs[i]].has_target) return fail(c, SV_E_STATE, "view %d has no target", vs[i]);
+  }
+  if (sv_status s = prepare_batch(c, vs, nv)) return s;
+  if (sv_status s = build_batch(c)) return s;
+  if (sv_status s = forward_batch(c)) return s;
+  launch_tile_l1(c->st, c->d_views, c->tile_view, c->tiles_total, c->color, c->target, c->tile_l1);
+  if (sv_status s = check_launch(c, "tile_l1")) return s;
+  std::vector<float> l1(c->tiles_total);
+  HIP_OK(c, hipMemcpyAsync(l1.data(), c->tile_l1, l1.size() * sizeof(float), hipMemcpyDeviceToHost, c->st));
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  std::vector<float> thr(nv);
+  c->scored_views.assign(vs, vs + nv);
+  c->scored_base.resize(nv);
+  for (int s = 0; s < nv; ++s) {
+    const DevView &d = c->h_slots[s];
+    c->scored_base[s] = d.tile_base;
+    std::vector<float> v(l1.begin() + d.tile_base, l1.begin() + d.tile_base + d.n_tiles);
+    std::sort(v.begin(), v.end());
+    thr[s] = v.empty() ? 0.0f : v[(size_t)std::floor((double)percentile * (double)(v.size() - 1))];
+  }
+  float *d_thr = nullptr;
+  uint8_t *d_act = nullptr;
+  if (sv_status s = dalloc(c, &d_thr, (size_t)nv)) return s;
+  if (sv_status s = dalloc(c, &d_act, (size_t)c->N)) {
+    dfree(d_thr);
+    return s;
+  }
+  sv_status st = SV_OK;
+  if (hipMemcpy(d_thr, thr.data(), nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    st = fail(c, SV_E_HIP, "threshold upload failed");
+  if (!st) {
+    launch_densify_decide(c->st, c->rec, c->aux, c->N, c->d_views, nv, c->tile_l1, d_thr, split_px, d_act);
+    st = check_launch(c, "densify_decide");
+  }
+  if (!st && hipMemcpy(actions, d_act, c->N, hipMemcpyDeviceToHost) != hipSuccess)
+    st = fail(c, SV_E_HIP, "actions readback failed");
+  dfree(d_thr);
+  dfree(d_act);
+  if (st) return st;
+  size_t nc = 0, ns = 0;
+  for (uint32_t i = 0; i < c->N; ++i) {
+    nc += actions[i] == 1;
+    ns += actions[i] == 2;
+  }
+  if (n_clone) *n_clone = nc;
+  if (n_split) *n_split = ns;
+  return SV_OK;
+}
+
+sv_status sv_read_tile_l1(sv_ctx *c, int v, float *out) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (sv_status s = check_view(c, v)) return s;
+  if (!out) return fail(c, SV_E_ARG, "null output");
+  for (size_t s = 0; s < c->scored_views.size(); ++s)
+    if (c->scored_views[s] == v) {
+      HIP_OK(c, hipStreamSynchronize(c->st));
+      HIP_OK(c, hipMemcpy(out, c->tile_l1 + c->scored_base[s], c->views[v].n_tiles * sizeof(float),
+                          hipMemcpyDeviceToHost));
+      return SV_OK;
+    }
+  return fail(c, SV_E_STATE, "view %d was not scored by the last sv_densify_decide", v);
 }
 
 sv_status sv_enable_timing(sv_ctx *c, int enable) {

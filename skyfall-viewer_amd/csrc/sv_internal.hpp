@@ -132,6 +132,13 @@ void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src
                     int planes, const uint32_t *keep_offs, const uint8_t *keep, uint32_t N);
 void launch_prune_flags(hipStream_t st, const float *P, size_t cap, uint32_t N, float min_opacity,
                         uint8_t *keep, uint32_t *keep_u32);
+// Per-tile L1 error of every tile of the batch (scoring for densify).
+void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_view, uint32_t n_tiles,
+                    const float *color, const float *target, float *tile_l1);
+// Densify rule from the batch's records, tile L1 and per-view thresholds.
+void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, uint32_t N,
+                           const DevView *views, int nv, const float *tile_l1, const float *thr,
+                           float split_px, uint8_t *actions);
 void launch_densify(hipStream_t st, const float *P, const float *M1, const float *M2, size_t cap,
                     uint32_t N, const uint8_t *actions, const uint32_t *keep_offs,
                     const uint32_t *child_offs, uint32_t n_keep, uint64_t seed, float *Pn,

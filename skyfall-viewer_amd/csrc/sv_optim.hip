@@ -538,6 +538,87 @@ void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src
                      planes, keep_offs, keep, N);
 }
 
+// Per-tile L1 error of the batch (scoring for the densify rule): one
+// 256-thread workgroup per global tile; slot k = pixel (k & 15, k >> 4) of
+// the tile holds (|dr| + |dg|) + |db| (0 outside the view), summed by the
+// fixed pairwise tree slot[k] += slot[k + s], s = 128 .. 1.
+__global__ __launch_bounds__(256) void k_tile_l1(const DevView *__restrict__ views,
+                                                 const uint32_t *__restrict__ tile_view,
+                                                 uint32_t n_tiles, const float *__restrict__ color,
+                                                 const float *__restrict__ target,
+                                                 float *__restrict__ tile_l1) {
+  __shared__ float s_v[256];
+  const uint32_t tg = blockIdx.x;
+  if (tg >= n_tiles) return;
+  const DevView &V = views[tile_view[tg]];
+  const uint32_t t = tg - V.tile_base;
+  const uint32_t x = (t % V.tiles_x) * 16u + (threadIdx.x & 15u);
+  const uint32_t y = (t / V.tiles_x) * 16u + (threadIdx.x >> 4);
+  float e = 0.0f;
+  if (x < V.W && y < V.H) {
+    const size_t P = (size_t)V.W * V.H, lp = (size_t)y * V.W + x;
+    const float *c = color + 3 * V.pix_off, *g = target + 3 * V.pix_off;
+    e = (fabsf(c[lp] - g[lp]) + fabsf(c[P + lp] - g[P + lp])) + fabsf(c[2 * P + lp] - g[2 * P + lp]);
+  }
+  s_v[threadIdx.x] = e;
+  __syncthreads();
+  for (uint32_t st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s_v[threadIdx.x] = s_v[threadIdx.x] + s_v[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tile_l1[tg] = s_v[0];
+}
+
+void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_view, uint32_t n_tiles,
+                    const float *color, const float *target, float *tile_l1) {
+  if (n_tiles == 0) return;
+  hipLaunchKernelGGL(k_tile_l1, dim3(n_tiles), dim3(256), 0, st, views, tile_view, n_tiles, color, target,
+                     tile_l1);
+}
+
+// Densify rule: candidate when a tile of the Gaussian's rectangle in a view
+// where it is listed has L1 error above that view's threshold; a candidate
+// splits when its largest projected radius over those views exceeds
+// split_px, else it is cloned. Replaces the decision block of
+// performDensification (gaussian-renderer.ts:3249-3326).
+__global__ __launch_bounds__(256) void k_densify_decide(const float4 *__restrict__ rec,
+                                                        const float2 *__restrict__ aux, uint32_t N,
+                                                        const DevView *__restrict__ views, int nv,
+                                                        const float *__restrict__ tile_l1,
+                                                        const float *__restrict__ thr, float split_px,
+                                                        uint8_t *__restrict__ actions) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  bool cand = false;
+  float rmax = 0.0f;
+  for (int s = 0; s < nv; ++s) {
+    const size_t vg = (size_t)s * N + g;
+    const float4 q3 = rec[vg * 4 + 3];
+    if (__float_as_uint(q3.w) == 0u) continue;
+    rmax = fmaxf(rmax, aux[vg].y);
+    if (cand) continue;
+    const DevView &V = views[s];
+    const uint32_t a = __float_as_uint(q3.y), b = __float_as_uint(q3.z);
+    const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu, y1 = b >> 16;
+    const float th = thr[s];
+    for (uint32_t ty = y0; ty < y1 && !cand; ++ty)
+      for (uint32_t tx = x0; tx < x1; ++tx)
+        if (tile_l1[V.tile_base + ty * V.tiles_x + tx] > th) {
+          cand = true;
+          break;
+        }
+  }
+  actions[g] = !cand ? 0u : (rmax > split_px ? 2u : 1u);
+}
+
+void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, uint32_t N,
+                           const DevView *views, int nv, const float *tile_l1, const float *thr,
+                           float split_px, uint8_t *actions) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_densify_decide, dim3((N + 255) / 256), dim3(256), 0, st, rec, aux, N, views, nv,
+                     tile_l1, thr, split_px, actions);
+}
+
 // Counter-based SplitMix64 stream: value `ctr` of stream `seed`.
 __device__ __forceinline__ uint64_t rng_u64(uint64_t seed, uint64_t ctr) {
   uint64_t z = seed + (ctr + 1ull) * 0x9E3779B97F4A7C15ull;

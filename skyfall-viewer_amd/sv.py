@@ -97,6 +97,9 @@ _SIGS = {
     "sv_prune_mask": (C.c_int, [C.c_void_p, _U8, C.POINTER(C.c_size_t)]),
     "sv_prune_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
     "sv_densify": (C.c_int, [C.c_void_p, _U8, C.c_uint64, C.POINTER(C.c_size_t)]),
+    "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, _U8,
+                                    C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sv_timings": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.POINTER(C.c_int)]),
     "sv_reset_timings": (C.c_int, [C.c_void_p]),
@@ -431,6 +434,21 @@ class Context:
         n = C.c_size_t()
         self._ck(lib().sv_densify(self.h, a.ctypes.data_as(_U8), seed, C.byref(n)), "densify")
         return n.value
+
+    def densify_decide(self, views, percentile: float = 0.9, split_px: float = 8.0):
+        """Returns (actions uint8[N], n_clone, n_split)."""
+        vs = (C.c_int * len(views))(*views)
+        act = np.zeros(self.count, np.uint8)
+        nc, ns = C.c_size_t(), C.c_size_t()
+        self._ck(lib().sv_densify_decide(self.h, vs, len(views), percentile, split_px, act.ctypes.data_as(_U8),
+                                         C.byref(nc), C.byref(ns)), "densify_decide")
+        return act, nc.value, ns.value
+
+    def tile_l1(self, v: int) -> np.ndarray:
+        w, h = self.views[v]
+        out = np.zeros(((w + 15) // 16) * ((h + 15) // 16), np.float32)
+        self._ck(lib().sv_read_tile_l1(self.h, v, _fp(out)), "read_tile_l1")
+        return out
 
     # timing -------------------------------------------------------------------
     def enable_timing(self, on: bool = True):

@@ -1,0 +1,123 @@
+"""Prune / densify on the device against the oracle (a18): survivor sets,
+compacted parameters and moments are bit-exact; children of clone / split
+are bit-exact except where the device's double exp / log / cos differ from
+the host's in the last place (one float ulp at most). The densify rule's
+tile scores and decisions are bit-exact against the oracle on the same
+state.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("pos", "logscale", "quat", "logit", "sh")
+
+
+def _ctx(sv, g, cap, cams=None, targets=None):
+    c = sv.Context(capacity=cap)
+    c.set_config(sh_deg=3, w_l2=0.8, w_l1=0.2, w_ssim=0.0)
+    c.set_gaussians(g)
+    if cams is not None:
+        c.set_views(cams)
+        for k, t in enumerate(targets):
+            c.set_target(k, t)
+    return c
+
+
+def _moments(rng, n):
+    return {"pos": rng.normal(size=(n, 3)).astype(np.float32),
+            "logscale": rng.normal(size=(n, 3)).astype(np.float32),
+            "quat": rng.normal(size=(n, 4)).astype(np.float32),
+            "logit": rng.normal(size=n).astype(np.float32),
+            "sh": rng.normal(size=(n, 48)).astype(np.float32)}
+
+
+def _set_moments(c, m1, m2):
+    c.set_moments(1, m1)
+    c.set_moments(2, m2)
+
+
+def test_prune_opacity_survivors(sv, orc):
+    g = sv.scene_gaussians(0, 20000, 1.0)
+    g["logit"] = (g["logit"] - np.float32(4.5)).astype(np.float32)
+    rng = np.random.default_rng(0)
+    m1, m2 = _moments(rng, 20000), _moments(rng, 20000)
+    c = _ctx(sv, g, 20000)
+    _set_moments(c, m1, m2)
+    removed = c.prune_opacity(0.005)
+    keep = orc.prune_keep(g["logit"], 0.005).astype(bool)
+    print(f"[prune] removed {removed} of 20000", flush=True)
+    assert removed == int((~keep).sum()) and 0 < removed < 20000
+    got, g1, g2 = c.get_gaussians(), c.get_moments(1), c.get_moments(2)
+    for k in KEYS:
+        assert np.array_equal(got[k], g[k][keep]), k
+        assert np.array_equal(g1[k], m1[k][keep]) and np.array_equal(g2[k], m2[k][keep]), k
+    c.close()
+
+
+def test_prune_mask_survivors(sv):
+    g = sv.scene_gaussians(1, 5000, 1.0)
+    rng = np.random.default_rng(1)
+    rm = (rng.uniform(size=5000) < 0.3).astype(np.uint8)
+    c = _ctx(sv, g, 5000)
+    assert c.prune_mask(rm) == int(rm.sum())
+    got = c.get_gaussians()
+    for k in KEYS:
+        assert np.array_equal(got[k], g[k][rm == 0]), k
+    # nothing removed / everything removed: no change (applyPruneMask early out)
+    assert c.prune_mask(np.zeros(c.count, np.uint8)) == 0
+    assert c.prune_mask(np.ones(c.count, np.uint8)) == 0 and c.count == int((rm == 0).sum())
+    c.close()
+
+
+@pytest.mark.parametrize("grow", [False, True])
+def test_densify_matches_oracle(sv, orc, grow):
+    n = 6000
+    g = sv.scene_gaussians(2, n, 1.0)
+    rng = np.random.default_rng(2)
+    m1, m2 = _moments(rng, n), _moments(rng, n)
+    act = rng.choice([0, 0, 0, 1, 2], n).astype(np.uint8)
+    cap = n if grow else 4 * n  # grow: the arrays must be reallocated (next power of two)
+    c = _ctx(sv, g, cap)
+    _set_moments(c, m1, m2)
+    added = c.densify(act, 4242)
+    want, w1, w2 = orc.densify(g, m1, m2, act, 4242)
+    assert added == want["pos"].shape[0] - n and c.count == want["pos"].shape[0]
+    got, g1, g2 = c.get_gaussians(), c.get_moments(1), c.get_moments(2)
+    ns = int((act != 2).sum())
+    for k in KEYS:
+        assert np.array_equal(got[k][:ns], want[k][:ns]), k
+        assert np.array_equal(g1[k], w1[k]) and np.array_equal(g2[k], w2[k]), k
+    for k in ("quat", "logit", "sh"):
+        assert np.array_equal(got[k], want[k]), k
+    for k in ("pos", "logscale"):
+        ulp = np.abs(got[k][ns:].view(np.int32).astype(np.int64) - want[k][ns:].view(np.int32).astype(np.int64))
+        print(f"[densify] children {k}: {int((ulp > 0).sum())} values off by <= {int(ulp.max())} ulp", flush=True)
+        assert ulp.max() <= 1, k
+    c.close()
+
+
+def test_densify_decide_matches_oracle(sv, orc):
+    n = 30000
+    g = sv.scene_gaussians(3, n, 1.0)
+    g["logscale"] = (g["logscale"] + np.float32(1.5)).astype(np.float32)
+    cams = sv.ring_cameras(3, 3.0, 200, 136)
+    tg = dict(g)
+    tg["pos"] = sv.scene_jitter(4, g["pos"], 0.02)
+    views = [orc.view_params(200, 136, sh_deg=3) for _ in cams]
+    targets = [orc.render_and_grad(tg, cam, v)["color"] for cam, v in zip(cams, views)]
+    c = _ctx(sv, g, n, cams, targets)
+    act, nc, ns = c.densify_decide([0, 1, 2], 0.9, 8.0)
+    projs, l1s = [], []
+    for k, (cam, v) in enumerate(zip(cams, views)):
+        r = orc.render_and_grad(g, cam, v)
+        l1 = orc.tile_l1(v, r["color"], targets[k])
+        assert np.array_equal(c.tile_l1(k), l1), k
+        projs.append(r["proj"])
+        l1s.append(l1)
+    thr = np.array([orc.percentile_lower(x, 0.9) for x in l1s], np.float32)
+    want = orc.densify_decide(projs, views, l1s, thr, 8.0)
+    print(f"[decide] clone {nc} split {ns} of {n}", flush=True)
+    assert np.array_equal(act, want)
+    assert nc == int((want == 1).sum()) and ns == int((want == 2).sum()) and nc > 0 and ns > 0
+    c.close()
