@@ -50,6 +50,7 @@ def configure(ctx: sv.Context, args, keep_grads=0):
     c.batch = args.batch
     c.persistent = args.persistent
     c.keep_grads = keep_grads
+    c.xcd_remap = args.xcd_remap
     ctx.set_config(c)
 
 
@@ -156,6 +157,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1, help="1: the batched step is one hipGraph launch")
+    ap.add_argument("--xcd-remap", type=int, default=1, help="1: contiguous tile ranges per XCD")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -239,7 +241,7 @@ def main():
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
         "config": {"workload": f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step",
                    "splats": N, "views": V, "width": W, "height": H, "tile": 16, "lds_batch": args.batch,
-                   "persistent": args.persistent, "graph": args.graph},
+                   "persistent": args.persistent, "graph": args.graph, "xcd_remap": args.xcd_remap},
         "eager_launch_ms_per_step": 1e3 * eager_b if eager_b else None,
         "keys_per_s": keys_per_s,
         "intersections_per_step": I,

@@ -77,6 +77,9 @@ typedef struct sv_config {
   uint32_t keep_grads;      /* 1: batched steps also write the gradient
                                buffers (sv_read_grads) before Adam */
   uint32_t plain_wf_flags;  /* SV_WF_* flags of the SV_MODE_PLAIN backward */
+  uint32_t xcd_remap;       /* 1: raster workgroups on one XCD take a
+                               contiguous range of tiles (L2 locality of
+                               neighbouring tiles' records) */
 } sv_config;
 
 /* Step modes for sv_step. */
@@ -230,6 +233,9 @@ sv_status sv_prune_mask(sv_ctx *ctx, const uint8_t *remove, size_t *n_removed);
 /* Prune rule: drop Gaussians with sigmoid(logit) < min_opacity
  * (prune-decide.wgsl:178-180), decided and compacted on the device. */
 sv_status sv_prune_opacity(sv_ctx *ctx, float min_opacity, size_t *n_removed);
+/* resetOpacity (gaussian-renderer.ts:3867-3920): logits above
+ * logit(max_opacity) are set to it, on the device; moments are kept. */
+sv_status sv_reset_opacity(sv_ctx *ctx, float max_opacity, size_t *n_clamped);
 /* performDensification (gaussian-renderer.ts:3211-3466) with a seeded RNG:
  * actions[i] = 0 none, 1 clone, 2 split. Output order: non-split originals
  * in order, then children in parent order (clone: 1, split: 2); children

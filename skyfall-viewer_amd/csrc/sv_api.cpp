@@ -275,6 +275,7 @@ DevConfig dev_config(const sv_config &k) {
   d.sh_deg = k.sh_deg;
   d.render_f16 = k.render_f16;
   d.importance_decay = k.importance_decay;
+  d.xcd_remap = k.xcd_remap;
   return d;
 }
 
@@ -1013,6 +1014,7 @@ sv_status sv_default_config(sv_config *o) {
   o->use_graph = 0;
   o->keep_grads = 0;
   o->plain_wf_flags = 0;
+  o->xcd_remap = 1;
   return SV_OK;
 }
 
@@ -1121,6 +1123,7 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->raster_px != 16) return fail(c, SV_E_ARG, "raster_px must be 16 (got %u)", k->raster_px);
   if (k->sh_deg > 3) return fail(c, SV_E_ARG, "sh_deg must be <= 3");
   if (k->plain_wf_flags > 3) return fail(c, SV_E_ARG, "plain_wf_flags must be a combination of SV_WF_*");
+  if (k->xcd_remap > 1) return fail(c, SV_E_ARG, "xcd_remap must be 0 or 1");
   c->cfg = *k;
   return SV_OK;
 }
@@ -1616,6 +1619,21 @@ sv_status sv_densify(sv_ctx *c, const uint8_t *actions, uint64_t seed, size_t *n
   dfree(child_offs);
   dfree(tmp);
   return swap_in(c, Pn, M1n, M2n, cap_n, (uint32_t)n_new);
+}
+
+sv_status sv_reset_opacity(sv_ctx *c, float max_opacity, size_t *n_clamped) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!(max_opacity > 0.0f && max_opacity < 1.0f)) return fail(c, SV_E_ARG, "max_opacity must be in (0, 1)");
+  const float max_logit = (float)std::log((double)max_opacity / (1.0 - (double)max_opacity));
+  launch_fill_u32(c->st, (uint32_t *)c->d_small, 0u, 2);
+  launch_clamp_logit(c->st, c->P, c->cap, c->N, max_logit, (uint32_t *)c->d_small);
+  if (sv_status s = check_launch(c, "reset_opacity")) return s;
+  HIP_OK(c, hipMemcpyAsync(c->h_small, c->d_small, sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
+  if (sv_status s = sync(c)) return s;
+  if (n_clamped) *n_clamped = (size_t)(c->h_small[0] & 0xffffffffu);
+  c->built = false;
+  c->forward_done = false;
+  return SV_OK;
 }
 
 sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, float split_px,

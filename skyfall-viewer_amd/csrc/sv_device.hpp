@@ -75,6 +75,7 @@ struct DevConfig {
   uint32_t sh_deg;
   uint32_t render_f16;
   float importance_decay;
+  uint32_t xcd_remap;
 };
 
 // ---------------------------------------------------------------------

@@ -132,6 +132,8 @@ void launch_compact(hipStream_t st, const float *src, float *dst, size_t cap_src
                     int planes, const uint32_t *keep_offs, const uint8_t *keep, uint32_t N);
 void launch_prune_flags(hipStream_t st, const float *P, size_t cap, uint32_t N, float min_opacity,
                         uint8_t *keep, uint32_t *keep_u32);
+// Clamp logits above max_logit; adds the number clamped to *n_clamped.
+void launch_clamp_logit(hipStream_t st, float *P, size_t cap, uint32_t N, float max_logit, uint32_t *n_clamped);
 // Per-tile L1 error of every tile of the batch (scoring for densify).
 void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_view, uint32_t n_tiles,
                     const float *color, const float *target, float *tile_l1);

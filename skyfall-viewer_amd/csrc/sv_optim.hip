@@ -520,6 +520,27 @@ void launch_prune_flags(hipStream_t st, const float *P, size_t cap, uint32_t N, 
                      keep, keep_u32);
 }
 
+// resetOpacity (gaussian-renderer.ts:3867-3920): clamp logits to
+// max_logit; one counter add per wave of clamped entries.
+__global__ void k_clamp_logit(float *__restrict__ P, size_t cap, uint32_t N, float max_logit,
+                              uint32_t *__restrict__ n_clamped) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool hit = false;
+  if (g < N) {
+    float *p = P + (size_t)P_LOGIT * cap + g;
+    if (*p > max_logit) {
+      *p = max_logit;
+      hit = true;
+    }
+  }
+  const uint64_t m = __ballot(hit);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(n_clamped, (uint32_t)__popcll(m));
+}
+void launch_clamp_logit(hipStream_t st, float *P, size_t cap, uint32_t N, float max_logit, uint32_t *n_clamped) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_clamp_logit, dim3((N + 255) / 256), dim3(256), 0, st, P, cap, N, max_logit, n_clamped);
+}
+
 // Order-preserving compaction of `planes` planes (applyPruneMask,
 // gaussian-renderer.ts:2917-3000): row g goes to keep_offs[g].
 __global__ __launch_bounds__(256) void k_compact(const float *__restrict__ src, float *__restrict__ dst,

@@ -32,8 +32,19 @@ __device__ __forceinline__ bool next_tile(const RasterArgs &a, uint32_t &tg, uin
   return tg < a.n_tiles_total;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b mod 8). With xcd_remap the blocks of one XCD take one contiguous range
+// of work items (a bijection on [0, n)), so neighbouring tiles, which read
+// the same splats' records and write adjacent adjoint records, run on the
+// same XCD and share its L2.
+constexpr uint32_t NUM_XCD = 8;
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+  const uint32_t q = n / NUM_XCD, r = n % NUM_XCD, x = b % NUM_XCD;
+  return x * q + min(x, r) + b / NUM_XCD;
+}
+
 __device__ __forceinline__ uint32_t first_tile(const RasterArgs &a, uint32_t *s_tg) {
-  if (a.work_counter == nullptr) return blockIdx.x;
+  if (a.work_counter == nullptr) return a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   if (threadIdx.x == 0) *s_tg = atomicAdd(a.work_counter, 1u);
   __syncthreads();
   return *s_tg;

@@ -1,0 +1,196 @@
+'use strict';
+// JS host side of the MI355X training step: a drop-in for the training
+// part of the reference's GaussianRenderer (src/renderers/gaussian-renderer.ts:30-58,
+// returned at :4110-4153) and a Function-style differentiable rasterizer,
+// both over the N-API addon (sv_napi.cpp) and libsv.so.
+//
+// Differences from the reference renderer, by design:
+//  - images are Float32Array (h * w * 3) instead of GPU textures
+//    (setTrainingImages takes {data, width, height});
+//  - the camera per iteration is chosen by the caller's view order
+//    (option viewOrder: 'cycle' | 'batched') instead of Math.random;
+//  - prune uses the opacity rule (minOpacity) decided on the device;
+//  - errors of the device path throw; missing state warns and skips as the
+//    reference does (console.warn + return).
+
+const native = require('./skyfall_sv.node');
+
+// Reference defaults (gaussian-renderer.ts:88-96, 506-522, 2421-2423).
+const DEFAULT_PARAMS = {
+  lr_means: 0.00016, lr_sh: 0.0025, lr_opacity: 0.025, lr_scale: 0.005, lr_rotation: 0.001,
+  sh_update_interval: 16,
+};
+const DEFAULT_LOSS = { w_l2: 0.0, w_l1: 0.8, w_ssim: 0.2 };
+
+function cameraToNative(cam) {
+  const pick = (k) => Array.from(cam[k]);
+  const out = { view: pick('view'), proj: pick('proj'), viewport: pick('viewport'), focal: pick('focal') };
+  if (cam.view_inv) out.view_inv = pick('view_inv');
+  if (cam.proj_inv) out.proj_inv = pick('proj_inv');
+  return out;
+}
+
+// torch.autograd.Function shape without torch: forward renders the views
+// and saves only what the backward needs (final alpha and last index per
+// pixel stay on the device, 8 bytes a pixel); backward takes dL/dimage per
+// view and returns dL/dparameters (transmittance is recomputed by reverse
+// traversal, no per-pixel lists are stored).
+class RasterizeFunction {
+  static forward(ctx, views) {
+    const images = [];
+    for (const v of views) {
+      native.forward(ctx.handle, v);
+      const [w, h] = ctx.size(v);
+      images.push(native.readImage(ctx.handle, v, w, h));
+    }
+    return { images, saved: { views: views.slice() } };
+  }
+
+  static backward(ctx, saved, dLdImages) {
+    native.zeroGrad(ctx.handle);
+    saved.views.forEach((v, i) => {
+      native.forward(ctx.handle, v);
+      native.setResidual(ctx.handle, v, dLdImages[i]);
+      native.backward(ctx.handle, v);
+    });
+    return native.readGrads(ctx.handle);
+  }
+}
+
+class Context {
+  constructor(capacity, device = 0) {
+    this.handle = native.create(capacity, device);
+    this.sizes = [];
+  }
+  setViews(cameras) {
+    native.setViews(this.handle, cameras.map(cameraToNative));
+    this.sizes = cameras.map((c) => [Math.round(c.viewport[0]), Math.round(c.viewport[1])]);
+  }
+  size(v) { return this.sizes[v]; }
+  close() { native.destroy(this.handle); }
+}
+
+// pointCloud: {pos, logscale, quat (x,y,z,w), logit, sh} Float32Arrays.
+function createGaussianRenderer(pointCloud, options = {}) {
+  const n = pointCloud.logit.length;
+  const capacity = options.capacity || Math.max(1, 2 * n);
+  const ctx = new Context(capacity, options.device || 0);
+  native.setGaussians(ctx.handle, pointCloud);
+  let params = { ...DEFAULT_PARAMS };
+  let loss = { ...DEFAULT_LOSS };
+  let cameras = [];
+  let imagesSet = false;
+  let training = false;
+  let stopRequested = false;
+  let iteration = 0;
+  let target = 0;
+  let ips = 0;
+  let cursor = 0;
+  let profiling = false;
+  const viewOrder = options.viewOrder || 'cycle';
+
+  function applyConfig(extra = {}) {
+    native.setConfig(ctx.handle, {
+      lr_pos: params.lr_means, lr_sh: params.lr_sh, lr_opacity: params.lr_opacity,
+      lr_scale: params.lr_scale, lr_rot: params.lr_rotation,
+      sh_update_interval: params.sh_update_interval,
+      w_l2: loss.w_l2, w_l1: loss.w_l1, w_ssim: loss.w_ssim, ...(options.config || {}), ...extra,
+    });
+  }
+  applyConfig();
+
+  function nextViews() {
+    if (viewOrder === 'batched') return cameras.map((_, i) => i);
+    const v = cursor % cameras.length;
+    cursor += 1;
+    return [v];
+  }
+
+  const renderer = {
+    context: ctx,
+    getGaussianCount: () => native.getCount(ctx.handle),
+    setTrainingCameras(cams) {
+      cameras = cams.slice();
+      ctx.setViews(cameras);
+      imagesSet = false;
+    },
+    setTrainingImages(images) {
+      if (images.length !== cameras.length) {
+        console.warn('[train] image count does not match camera count; skipping');
+        return;
+      }
+      images.forEach((img, v) => native.setTarget(ctx.handle, v, img.data));
+      imagesSet = true;
+    },
+    getTrainingParams: () => ({ ...params }),
+    setTrainingParams(p) { params = { ...params, ...p }; applyConfig(); },
+    getLossWeights: () => ({ ...loss }),
+    setLossWeights(w) { loss = { ...loss, ...w }; applyConfig(); },
+    async buildTrainingTiles() {
+      if (!cameras.length) { console.warn('[tiles] no training cameras'); return; }
+      native.buildTiles(ctx.handle, nextViews()[0]);
+    },
+    // One training iteration (trainStep, gaussian-renderer.ts:3638-3660).
+    async trainStep() {
+      const mode = viewOrder === 'batched' ? native.MODE_BATCHED : native.MODE_SEQUENTIAL;
+      const st = await native.stepAsync(ctx.handle, nextViews(), mode, 1);
+      iteration += 1;
+      return st;
+    },
+    async train(numFrames) {
+      if (training) { console.warn('[train] already training'); return; }
+      if (!cameras.length || !imagesSet) { console.warn('[train] cameras and images are required'); return; }
+      training = true;
+      stopRequested = false;
+      target = iteration + numFrames;
+      const t0 = Date.now();
+      const start = iteration;
+      try {
+        while (iteration < target && !stopRequested) await renderer.trainStep();
+      } finally {
+        training = false;
+        const dt = (Date.now() - t0) / 1000;
+        ips = dt > 0 ? (iteration - start) / dt : 0;
+      }
+    },
+    stopTraining() { stopRequested = true; },
+    getTrainingStatus: () => ({
+      isTraining: training, currentIteration: iteration, targetIterations: target,
+      iterationsPerSecond: ips, gaussianCount: native.getCount(ctx.handle),
+    }),
+    async prune(customParams) {
+      const minOpacity = customParams && customParams.minOpacity !== undefined ? customParams.minOpacity : 0.005;
+      return native.pruneOpacity(ctx.handle, minOpacity);
+    },
+    async resetOpacity(maxOpacity = 0.1) { native.resetOpacity(ctx.handle, maxOpacity); },
+    async densify({ views, percentile = 0.9, splitPx = 8, seed = iteration } = {}) {
+      const d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx);
+      native.densify(ctx.handle, d.actions, seed);
+      return { cloned: d.clones, split: d.splits };
+    },
+    enableProfiling(enabled) { profiling = !!enabled; native.enableTiming(ctx.handle, profiling); },
+    printProfilingReport() {
+      for (const t of native.timings(ctx.handle)) {
+        console.log(`${t.name.padEnd(16)} ${(t.totalMs / t.launches).toFixed(3)} ms x ${t.launches}`);
+      }
+    },
+    async profiledTrainStep() {
+      native.enableTiming(ctx.handle, true);
+      const st = await renderer.trainStep();
+      renderer.printProfilingReport();
+      native.enableTiming(ctx.handle, profiling);
+      return st;
+    },
+    getTrainingController: () => null,
+    async dumpTrainingDiagnostics() {
+      const s = renderer.getTrainingStatus();
+      console.log(`[diag] iteration ${s.currentIteration}, ${s.gaussianCount} Gaussians, ${cameras.length} views`);
+    },
+    readImage(v) { const [w, h] = ctx.size(v); return native.readImage(ctx.handle, v, w, h); },
+    getGaussians: () => native.getGaussians(ctx.handle),
+    destroy() { ctx.close(); },
+  };
+  return renderer;
+}
+
+module.exports = { native, Context, RasterizeFunction, createGaussianRenderer, cameraToNative };

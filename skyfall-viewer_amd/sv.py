@@ -37,7 +37,7 @@ class Config(C.Structure):
         ("sh_update_interval", C.c_uint32), ("pos_lr_schedule", C.c_uint32),
         ("render_f16", C.c_uint32), ("importance_decay", C.c_float),
         ("persistent", C.c_uint32), ("use_graph", C.c_uint32), ("keep_grads", C.c_uint32),
-        ("plain_wf_flags", C.c_uint32),
+        ("plain_wf_flags", C.c_uint32), ("xcd_remap", C.c_uint32),
     ]
 
 
@@ -97,6 +97,7 @@ _SIGS = {
     "sv_prune_mask": (C.c_int, [C.c_void_p, _U8, C.POINTER(C.c_size_t)]),
     "sv_prune_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
     "sv_densify": (C.c_int, [C.c_void_p, _U8, C.c_uint64, C.POINTER(C.c_size_t)]),
+    "sv_reset_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
     "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, _U8,
                                     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
@@ -433,6 +434,11 @@ class Context:
         a = np.ascontiguousarray(actions, dtype=np.uint8)
         n = C.c_size_t()
         self._ck(lib().sv_densify(self.h, a.ctypes.data_as(_U8), seed, C.byref(n)), "densify")
+        return n.value
+
+    def reset_opacity(self, max_opacity: float = 0.1) -> int:
+        n = C.c_size_t()
+        self._ck(lib().sv_reset_opacity(self.h, max_opacity, C.byref(n)), "reset_opacity")
         return n.value
 
     def densify_decide(self, views, percentile: float = 0.9, split_px: float = 8.0):
