@@ -76,7 +76,7 @@ struct RasterArgs {
   const float *residual;  // 3 planes per view, or null for the fused loss
   // backward outputs
   const uint32_t *offs;   // per (slot, Gaussian) first intersection (emit order)
-  float *isect_rec;       // 9 floats per intersection (emit order)
+  float *isect_rec;       // 64-byte adjoint record per intersection (emit order)
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent queue counter (null: one block per tile)
 };
@@ -99,7 +99,7 @@ struct VjpArgs {
   const uint32_t *offs;
   const float4 *rec;       // projection records
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
-  const float *isect_rec;  // 9 floats per intersection
+  const float *isect_rec;  // 64-byte adjoint record per intersection
   float4 *importance;      // importance statistics (may be null)
   float *G;                // gradient planes (accumulate), or null when fused
   float *geom;             // 5 planes: grad_geom of the last slot (may be null)

@@ -262,13 +262,19 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     imp.y = imp.y + 1.0f;
     imp.z = fmaxf(imp.z, a.aux[vg].y);
     any = true;
-    const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
+    const float4 *rp = reinterpret_cast<const float4 *>(a.isect_rec) + (size_t)a.offs[vg] * 4;
     float r[REC9];
-#pragma unroll
-    for (int q = 0; q < REC9; ++q) r[q] = rp[q];
+    {
+      const float4 x = rp[0], y = rp[1], z = rp[2];
+      r[0] = x.x; r[1] = x.y; r[2] = x.z; r[3] = x.w;
+      r[4] = y.x; r[5] = y.y; r[6] = y.z; r[7] = y.w;
+      r[8] = z.x;
+    }
     for (uint32_t j = 1; j < cnt; ++j) {
-#pragma unroll
-      for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[(size_t)j * REC9 + q];
+      const float4 x = rp[4 * j], y = rp[4 * j + 1], z = rp[4 * j + 2];
+      r[0] = r[0] + x.x; r[1] = r[1] + x.y; r[2] = r[2] + x.z; r[3] = r[3] + x.w;
+      r[4] = r[4] + y.x; r[5] = r[5] + y.y; r[6] = r[6] + y.z; r[7] = r[7] + y.w;
+      r[8] = r[8] + z.x;
     }
     const DevView &V = a.views[s];
     float pc[4];

@@ -35,13 +35,19 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
                                                  float2 *__restrict__ aux,
                                                  uint32_t *__restrict__ counts,
                                                  uint32_t *__restrict__ max_dq) {
+  // Records of one view for the block's 256 Gaussians are staged in LDS
+  // and stored as whole contiguous lines (each store instruction of a wave
+  // writes 1 KB), instead of 64-byte pieces 64 bytes apart per lane.
+  __shared__ float4 s_rec[256 * 4];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = g < N;
+  const uint32_t gl = live ? g : N - 1u;  // lanes past N compute on a valid Gaussian, store nothing
   uint32_t local_max_dq = 0;
-  if (g < N) {
+  {
     float prm[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
-      const float v = P[(size_t)k * cap + g];
+      const float v = P[(size_t)k * cap + gl];
       prm[k] = cfg.render_f16 ? f16_round(v) : v;
     }
     const float wx = prm[P_POS], wy = prm[P_POS + 1], wz = prm[P_POS + 2];
@@ -137,13 +143,13 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
             float coef[16];
 #pragma unroll
             for (int k = 0; k < 16; ++k) {
-              const float v = P[(size_t)(P_SH + 3 * k + c) * cap + g];
+              const float v = P[(size_t)(P_SH + 3 * k + c) * cap + gl];
               coef[k] = cfg.render_f16 ? f16_round(v) : v;
             }
             rgb[c] = sh_channel(b, coef, 1, cfg.sh_deg);
           }
           const uint32_t dq = f2u_sat(clampf(depth, 0.0f, 65535.0f)) & 0xffffu;
-          if (count > 0u) local_max_dq = max(local_max_dq, dq);
+          if (count > 0u && live) local_max_dq = max(local_max_dq, dq);
           const float pmax = opac > 0.0f ? logf(255.0f * opac) + 0.02f : -1.0f;
           const float qdet = aa * cc - bb * bb;
           float ex = -1.0f, ey = -1.0f;
@@ -159,13 +165,25 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           ax = make_float2(depth, radius);
         }
       }
-      float4 *q = rec + vg * 4;
-      q[0] = r0;
-      q[1] = r1;
-      q[2] = r2;
-      q[3] = r3;
-      aux[vg] = ax;
-      counts[vg] = count;
+      s_rec[threadIdx.x * 4 + 0] = r0;
+      s_rec[threadIdx.x * 4 + 1] = r1;
+      s_rec[threadIdx.x * 4 + 2] = r2;
+      s_rec[threadIdx.x * 4 + 3] = r3;
+      if (live) {
+        aux[vg] = ax;
+        counts[vg] = count;
+      }
+      __syncthreads();
+      // piece j of the block's 1024 float4 belongs to Gaussian
+      // blockIdx.x * 256 + j / 4 of this view
+      const uint32_t g0 = blockIdx.x * blockDim.x;
+      float4 *dst = rec + ((size_t)s * N + g0) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t j = (uint32_t)u * 256u + threadIdx.x;
+        if (g0 + j / 4u < N) dst[j] = s_rec[j];
+      }
+      __syncthreads();
     }
   }
   // block max of the depth quantum, then one atomic per block
@@ -327,6 +345,12 @@ void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t ke
   hipLaunchKernelGGL(k_keyinfo, dim3(1), dim3(64), 0, st, m, tile_bits, key_type_bits, cap);
 }
 
+// One workgroup emits the pairs of 256 consecutive (view, Gaussian) items.
+// Their output ranges are adjacent (exclusive scan), so the workgroup
+// writes one contiguous span: thread t takes span positions t, t + 256, ...
+// and finds the owning item by binary search over the items' first
+// positions in LDS. Stores are coalesced whole lines instead of one
+// sequential run per thread.
 template <typename K>
 __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
                                               const float2 *__restrict__ aux,
@@ -334,27 +358,60 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
                                               const uint32_t *__restrict__ offs, uint32_t N, int nv,
                                               const DevView *__restrict__ views, const StepMeta *m,
                                               K *__restrict__ keys, uint32_t *__restrict__ vals) {
-  const size_t vg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (vg >= (size_t)nv * N || m->overflow) return;
+  __shared__ uint32_t s_lo[256];   // first output position (0xffffffff past the items)
+  __shared__ uint32_t s_rect[256]; // x0 | width << 16
+  __shared__ uint32_t s_y0[256];
+  __shared__ K s_kb[256];          // key of the item's tile (tx = x0, ty = y0) without depth bits
+  __shared__ uint32_t s_dq[256];
+  __shared__ uint32_t s_g[256];
+  __shared__ uint32_t s_tx[256];   // tiles_x of the item's view
+  if (m->overflow) return;
   const uint32_t dbits = m->dbits;
-  const uint32_t cnt = counts[vg];
-  if (cnt == 0u) return;
-  const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
-  const DevView &V = views[s];
-  float4 r3 = rec[vg * 4 + 3];
-  const uint32_t dq = f2u_sat(clampf(aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
-  const uint32_t a = __float_as_uint(r3.y), b = __float_as_uint(r3.z);
-  const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu, y1 = b >> 16;
-  uint32_t p = offs[vg];
-  r3.x = __uint_as_float(p);
-  rec[vg * 4 + 3] = r3;
-  for (uint32_t ty = y0; ty < y1; ++ty)
-    for (uint32_t tx = x0; tx < x1; ++tx) {
-      const K tile = (K)(V.tile_base + ty * V.tiles_x + tx);
-      keys[p] = (tile << dbits) | (K)dq;
-      vals[p] = g;
-      ++p;
+  const size_t M = (size_t)nv * N;
+  const uint32_t t = threadIdx.x;
+  const size_t vg = (size_t)blockIdx.x * 256u + t;
+  uint32_t lo = 0xffffffffu, hi = 0;
+  if (vg < M) {
+    const uint32_t cnt = counts[vg];
+    lo = offs[vg];
+    hi = lo + cnt;
+    if (cnt) {
+      const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
+      const DevView &V = views[s];
+      float4 r3 = rec[vg * 4 + 3];
+      const uint32_t a = __float_as_uint(r3.y), b = __float_as_uint(r3.z);
+      const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu;
+      r3.x = __uint_as_float(lo);
+      rec[vg * 4 + 3] = r3;
+      s_rect[t] = x0 | ((x1 - x0) << 16);
+      s_y0[t] = y0;
+      s_kb[t] = (K)(V.tile_base + y0 * V.tiles_x + x0);
+      s_dq[t] = f2u_sat(clampf(aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
+      s_g[t] = g;
+      s_tx[t] = V.tiles_x;
     }
+  }
+  s_lo[t] = lo;
+  // span of the block: from the first item's position to the end of the last
+  __shared__ uint32_t s_span[2];
+  if (t == 0) s_span[0] = lo;
+  const size_t last = min((size_t)blockIdx.x * 256u + 255u, M - 1);
+  if (vg == last) s_span[1] = hi;
+  __syncthreads();
+  const uint32_t span_lo = s_span[0], span_hi = s_span[1];
+  for (uint32_t k = span_lo + t; k < span_hi; k += 256u) {
+    // largest j with s_lo[j] <= k: that item's range contains k
+    uint32_t j = 0;
+#pragma unroll
+    for (uint32_t step = 128; step > 0; step >>= 1)
+      if (s_lo[j + step] <= k) j += step;
+    const uint32_t local = k - s_lo[j];
+    const uint32_t rect = s_rect[j], w = rect >> 16;
+    const uint32_t dy = local / w, dx = local - dy * w;
+    const K tile = s_kb[j] + (K)(dy * s_tx[j] + dx);
+    keys[k] = (tile << dbits) | (K)s_dq[j];
+    vals[k] = s_g[j];
+  }
 }
 
 template <typename K>
