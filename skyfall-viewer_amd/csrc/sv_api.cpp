@@ -400,7 +400,7 @@ sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
   if (sv_status s = dalloc(c, (uint64_t **)&c->keys2, want)) return s;
   if (sv_status s = dalloc(c, &c->vals, want)) return s;
   if (sv_status s = dalloc(c, &c->vals2, want)) return s;
-  if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC_STRIDE)) return s;
+  if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC9)) return s;
   const size_t stw = sort_tmp_words<uint64_t>(want);
   if (sv_status s = dalloc(c, &c->sort_tmp, stw)) return s;
   c->sort_tmp_cap = stw;

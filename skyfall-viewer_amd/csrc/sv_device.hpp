@@ -30,10 +30,6 @@ constexpr uint32_t NO_LAST = 0xffffffffu;
 constexpr uint32_t WF_REFERENCE_ORDER = 0x1u;
 constexpr uint32_t WF_SKIP_098 = 0x2u;
 constexpr int REC9 = 9;  // per-(splat, tile) adjoint record: rgb3, logit, conic3, mean2
-// Records are stored 64-byte aligned (16 floats, 7 unused) so that each is
-// written as four 16-byte pieces covering one whole 64-byte memory sector:
-// a partial-sector write makes the L2 fetch the rest of the line first.
-constexpr int REC_STRIDE = 16;
 
 // Per-view constants resident in device memory.
 struct DevView {

@@ -220,6 +220,54 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
   return k < P_SCALE ? 0 : (k < P_QUAT ? 1 : (k < P_LOGIT ? 2 : (k < P_SH ? 3 : 4)));
 }
 
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr uint32_t VJP_CHUNK = 160;  // records staged per wave (5.6 KB of LDS)
+
+// Sum of the lane's adjoint records [off, off + cnt) of one view (fixed
+// order: the first record, then + each following one). The 64 lanes' ranges
+// are adjacent (emit order), so the wave loads the whole span with
+// coalesced dword loads into LDS and each lane sums its part from there.
+// All lanes of the wave must call this together.
+__device__ __forceinline__ void wave_sum_records(const float *__restrict__ recs, uint32_t off, uint32_t cnt,
+                                                 float r[REC9], float *s_buf) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t lo = cnt ? off : 0xffffffffu, hi = cnt ? off + cnt : 0u;
+  const uint32_t span_lo = wave_min_u32(lo), span_hi = wave_max_u32(hi);
+  for (uint32_t base = span_lo; base < span_hi; base += VJP_CHUNK) {
+    const uint32_t n = min(VJP_CHUNK, span_hi - base);
+    const float *src = recs + (size_t)base * REC9;
+    for (uint32_t z = (uint32_t)lane; z < REC9 * n; z += 64u) s_buf[z] = __builtin_nontemporal_load(src + z);
+    wave_lds_sync();
+    const uint32_t j0 = max(lo, base), j1 = min(hi, base + n);
+    for (uint32_t j = j0; j < j1; ++j) {
+      const float *x = s_buf + REC9 * (j - base);
+      if (j == off) {
+#pragma unroll
+        for (int q = 0; q < REC9; ++q) r[q] = x[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < REC9; ++q) r[q] = r[q] + x[q];
+      }
+    }
+    wave_lds_sync();
+  }
+}
+
 // ---------------------------------------------------------------------
 // Per-Gaussian backward over the views of a step: sum the Gaussian's
 // per-tile adjoint records of each view (emit order, fixed), apply the SH
@@ -230,12 +278,16 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 // cover all active elements).
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
+  __shared__ float s_recs[4][REC9 * VJP_CHUNK];
+  if (a.meta && a.meta->overflow) return;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.N || (a.meta && a.meta->overflow)) return;
+  // lanes past N take part in the wave's record loads and store nothing
+  const bool live = g < a.N;
+  const uint32_t gl = live ? g : a.N - 1u;
   float prm[11];
 #pragma unroll
   for (int k = 0; k < 11; ++k) {
-    const float v = a.P[(size_t)k * a.cap + g];
+    const float v = a.P[(size_t)k * a.cap + gl];
     prm[k] = a.cfg.render_f16 ? f16_round(v) : v;
   }
   const GeomPre pre = geom_pre(prm);
@@ -245,12 +297,14 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
   float glogit = 0.0f;
   float gp[3] = {0.f, 0.f, 0.f}, gs[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f};
   bool any = false;
-  float4 dacc = a.grad_accum ? a.grad_accum[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 dacc = a.grad_accum ? a.grad_accum[gl] : make_float4(0.f, 0.f, 0.f, 0.f);
   float geo[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-  float4 imp = a.importance ? a.importance[g] : make_float4(0.f, 0.f, 0.f, 0.f);
+  float4 imp = a.importance ? a.importance[gl] : make_float4(0.f, 0.f, 0.f, 0.f);
   for (int s = 0; s < a.nv; ++s) {
-    const size_t vg = (size_t)s * a.N + g;
-    const uint32_t cnt = a.counts[vg];
+    const size_t vg = (size_t)s * a.N + gl;
+    const uint32_t cnt = live ? a.counts[vg] : 0u;
+    float r[REC9];
+    wave_sum_records(a.isect_rec, cnt ? a.offs[vg] : 0u, cnt, r, s_recs[threadIdx.x >> 6]);
     // accumulate_importance (importance-accum.wgsl:19-40), once per view
     imp.x = imp.x * a.cfg.importance_decay;
     imp.y = imp.y * a.cfg.importance_decay;
@@ -262,20 +316,6 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     imp.y = imp.y + 1.0f;
     imp.z = fmaxf(imp.z, a.aux[vg].y);
     any = true;
-    const float4 *rp = reinterpret_cast<const float4 *>(a.isect_rec) + (size_t)a.offs[vg] * 4;
-    float r[REC9];
-    {
-      const float4 x = rp[0], y = rp[1], z = rp[2];
-      r[0] = x.x; r[1] = x.y; r[2] = x.z; r[3] = x.w;
-      r[4] = y.x; r[5] = y.y; r[6] = y.z; r[7] = y.w;
-      r[8] = z.x;
-    }
-    for (uint32_t j = 1; j < cnt; ++j) {
-      const float4 x = rp[4 * j], y = rp[4 * j + 1], z = rp[4 * j + 2];
-      r[0] = r[0] + x.x; r[1] = r[1] + x.y; r[2] = r[2] + x.z; r[3] = r[3] + x.w;
-      r[4] = r[4] + y.x; r[5] = r[5] + y.y; r[6] = r[6] + y.z; r[7] = r[7] + y.w;
-      r[8] = r[8] + z.x;
-    }
     const DevView &V = a.views[s];
     float pc[4];
     m4_mul_pos(V.view, pre.wx, pre.wy, pre.wz, pc);
@@ -304,6 +344,7 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) geo[q] = r[4 + q];
   }
+  if (!live) return;
   if (a.grad_accum) a.grad_accum[g] = dacc;
   if (a.importance) a.importance[g] = imp;
   if (a.geom) {

@@ -69,6 +69,14 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
     const M3 C3D = mul(mul(transpose(mul(S, R)), S), R);
     const float logit = prm[P_LOGIT];
     const float opac = sv_sigmoid(logit);
+    // SH coefficients once per Gaussian (re-reading them per view made the
+    // kernel fetch 12x its parameters from memory)
+    float shc[48];
+#pragma unroll
+    for (int k = 0; k < 48; ++k) {
+      const float v = P[(size_t)(P_SH + k) * cap + gl];
+      shc[k] = cfg.render_f16 ? f16_round(v) : v;
+    }
 
     for (int s = 0; s < nv; ++s) {
       const DevView &V = views[s];
@@ -142,10 +150,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           for (int c = 0; c < 3; ++c) {
             float coef[16];
 #pragma unroll
-            for (int k = 0; k < 16; ++k) {
-              const float v = P[(size_t)(P_SH + 3 * k + c) * cap + gl];
-              coef[k] = cfg.render_f16 ? f16_round(v) : v;
-            }
+            for (int k = 0; k < 16; ++k) coef[k] = shc[3 * k + c];
             rgb[c] = sh_channel(b, coef, 1, cfg.sh_deg);
           }
           const uint32_t dq = f2u_sat(clampf(depth, 0.0f, 65535.0f)) & 0xffffu;

@@ -346,13 +346,15 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
       if (lane == 63 && a.l1_partial) a.l1_partial[tg] = l1t;
       uint32_t tend = wave_max_u32(m);
       if (tend < start) tend = start;
-      // entries no pixel reaches get zero records (four lanes per record)
-      for (uint64_t z = 4ull * tend + lane; z < 4ull * end; z += WT) {
-        const uint32_t i = (uint32_t)(z >> 2);
+      // entries no pixel reaches get zero records
+      for (uint32_t i = tend + lane; i < end; i += WT) {
         const float4 *q = a.rec + (vbase + min(a.vals[i], a.N - 1u)) * 4;
         const uint32_t p = emit_pos(q[3], t.tx, t.ty);
-        if ((uint64_t)p < a.n_isect)
-          reinterpret_cast<float4 *>(a.isect_rec)[(size_t)p * 4 + (uint32_t)(z & 3u)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if ((uint64_t)p < a.n_isect) {
+          float *o = a.isect_rec + (size_t)p * REC9;
+#pragma unroll
+          for (int q9 = 0; q9 < REC9; ++q9) __builtin_nontemporal_store(0.0f, o + q9);
+        }
       }
       const float fx = (float)t.px + 0.5f;
       const float tx0 = (float)(t.tx * 16u) + 0.5f, tx1 = (float)min(t.tx * 16u + 15u, V.W - 1u) + 0.5f;
@@ -434,22 +436,14 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
           }
         }
         __syncthreads();
-        // four lanes write one record's 64 bytes (values 4c .. 4c + 3)
-        for (uint32_t z = lane; z < cnt * 4u; z += WT) {
-          const uint32_t k = z >> 2, c4 = z & 3u;
+        // adjoint records are written once and read once by the VJP: stream
+        // them past the caches, which hold the splat records the tiles share
+        for (uint32_t z = lane; z < cnt * REC9; z += WT) {
+          const uint32_t k = z / REC9, q9 = z - k * REC9;
           const uint32_t p = s_p[k];
-          float v[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const int q9 = (int)c4 * 4 + u;
-            v[u] = 0.0f;
-            if (q9 < REC9) {
-              const float *o = &s_out[k * 4 * REC9 + q9];
-              v[u] = ((o[3 * REC9] + o[2 * REC9]) + (o[REC9] + o[0]));
-            }
-          }
-          if ((uint64_t)p < a.n_isect)
-            reinterpret_cast<float4 *>(a.isect_rec)[(size_t)p * 4 + c4] = make_float4(v[0], v[1], v[2], v[3]);
+          const float *o = &s_out[k * 4 * REC9 + q9];
+          const float tot = ((o[3 * REC9] + o[2 * REC9]) + (o[REC9] + o[0]));
+          if ((uint64_t)p < a.n_isect) __builtin_nontemporal_store(tot, a.isect_rec + (size_t)p * REC9 + q9);
         }
         __syncthreads();
         hi = lo;
