@@ -83,6 +83,12 @@ def run_mode(mode, g, cams, targets, args, graph=0):
     for v, t in enumerate(targets):
         ctx.set_target(v, t)
     views = list(range(len(cams)))
+    tune = None
+    if mode == sv.MODE_BATCHED and args.autotune:
+        tune = ctx.autotune(views, 3)
+        best = min(tune, key=lambda e: e["ms_per_step"])
+        log("autotune: " + ", ".join(f"b{e['batch']}/p{e['persistent']}/x{e['xcd_remap']} {e['ms_per_step']:.2f}"
+                                      for e in tune) + f" -> {best}")
     for i in range(args.warmup):
         ctx.step_views(views, mode)
     ctx.sync()
@@ -111,8 +117,10 @@ def run_mode(mode, g, cams, targets, args, graph=0):
         ctx.sync()
         eager = (time.perf_counter() - e0) / args.steps
     st = ctx.step_views(views, mode, stats=True)
+    cfg = ctx.config()
+    chosen = {"batch": cfg.batch, "persistent": cfg.persistent, "xcd_remap": cfg.xcd_remap}
     ctx.close()
-    return (t1 - t0) / args.steps, timings, phase_steps, st, eager
+    return (t1 - t0) / args.steps, timings, phase_steps, st, eager, tune, chosen
 
 
 def cpu_baseline(g, cams, targets, args):
@@ -158,6 +166,8 @@ def main():
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1, help="1: the batched step is one hipGraph launch")
     ap.add_argument("--xcd-remap", type=int, default=1, help="1: contiguous tile ranges per XCD")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: time the tuner's candidates first and run the batched step with the fastest")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -195,17 +205,18 @@ def main():
             targets.append(tc.image(v))
     log("targets rendered")
 
-    sec_b, tim_b, ps_b, st_b, eager_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args, graph=args.graph)
+    sec_b, tim_b, ps_b, st_b, eager_b, tune_b, chosen_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args,
+                                                                   graph=args.graph)
     log(f"batched: {1e3 * sec_b:.2f} ms/step (graph {args.graph}), I={st_b.n_isect}, "
         f"key bits {st_b.key_bits}, passes {st_b.sort_passes}"
         + (f"; eager launches {1e3 * eager_b:.2f} ms/step" if eager_b else ""))
     plain = sequential = None
     if not args.no_plain:
-        sec_q, tim_q, ps_q, _, _ = run_mode(sv.MODE_SEQUENTIAL, g, cams, targets, args)
+        sec_q, tim_q, ps_q, *_ = run_mode(sv.MODE_SEQUENTIAL, g, cams, targets, args)
         sequential = {"ms_per_step": 1e3 * sec_q, "steps_per_s": 1.0 / sec_q,
                       "phase_ms_per_step": {k: v[0] / ps_q for k, v in tim_q.items() if v[1]}}
         log(f"sequential adjoint: {1e3 * sec_q:.2f} ms/step")
-        sec_p, tim_p, ps_p, _, _ = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
+        sec_p, tim_p, ps_p, *_ = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
         plain = {"ms_per_step": 1e3 * sec_p, "steps_per_s": 1.0 / sec_p,
                  "method": "reference method per view: wavefront fixed-point backward, atomic reduce, "
                            "geometry backward, Adam",
@@ -240,8 +251,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
         "config": {"workload": f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step",
-                   "splats": N, "views": V, "width": W, "height": H, "tile": 16, "lds_batch": args.batch,
-                   "persistent": args.persistent, "graph": args.graph, "xcd_remap": args.xcd_remap},
+                   "splats": N, "views": V, "width": W, "height": H, "tile": 16, "lds_batch": chosen_b["batch"],
+                   "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"]},
+        "autotune": tune_b,
         "eager_launch_ms_per_step": 1e3 * eager_b if eager_b else None,
         "keys_per_s": keys_per_s,
         "intersections_per_step": I,

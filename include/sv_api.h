@@ -112,6 +112,15 @@ typedef struct sv_kernel_time {
   uint64_t launches;
 } sv_kernel_time;
 
+/* One candidate of sv_autotune. */
+typedef struct sv_tune_entry {
+  uint32_t batch;       /* LDS batch of the rasterizers */
+  uint32_t persistent;  /* persistent raster launch */
+  uint32_t xcd_remap;   /* XCD-contiguous tile ranges */
+  uint32_t tile_size;   /* binning tile (16, see sv_autotune) */
+  double ms_per_step;   /* mean device time of one batched step */
+} sv_tune_entry;
+
 /* --- library / context ------------------------------------------------ */
 /* Version string. */
 const char *sv_version(void);
@@ -256,6 +265,21 @@ sv_status sv_densify_decide(sv_ctx *ctx, const int *views, int nviews, float per
 /* Per-tile L1 error of view `view` from the last sv_densify_decide
  * (tiles_x * tiles_y floats, row-major tiles). */
 sv_status sv_read_tile_l1(sv_ctx *ctx, int view, float *out);
+
+/* --- autotuner ------------------------------------------------------ */
+/* No counterpart in the reference (its tile and batch sizes are
+ * constants: 16x16 tiles, 64-entry batches, backward-tiles-local.wgsl:49-53,
+ * 185). Times `steps` batched steps of `views` (after one untimed step) for
+ * every candidate LDS batch 64/128/256 x persistent launch 0/1 x XCD tile
+ * mapping 0/1, restoring parameters, moments, statistics and the step
+ * counter before each candidate and at the end, and leaves the fastest
+ * candidate in the configuration. The binning tile stays 16: it decides
+ * which fringe pixels (power between the 3-sigma radius and ln(255 a)) a
+ * splat is composited into, so another tile size changes the image.
+ * out: up to *count entries (NULL to query); *count receives the number of
+ * candidates. */
+sv_status sv_autotune(sv_ctx *ctx, const int *views, int nviews, int steps,
+                      sv_tune_entry *out, int *count);
 
 /* --- timing --------------------------------------------------------- */
 /* Enable per-kernel hipEvent timing on the context stream. */

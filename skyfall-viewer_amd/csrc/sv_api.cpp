@@ -412,7 +412,8 @@ sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
 // enqueued without any host readback: the pair count, key layout, sort
 // passes and result buffer stay on the device (StepMeta), and every launch
 // is sized by capacities, so the same launches serve any step.
-sv_status enqueue_build(sv_ctx *c) {
+sv_status enqueue_build(sv_ctx *c, bool zero_records) {
+  float *zr = zero_records ? c->isect_rec : nullptr;
   const int nv = (int)c->slot_views.size();
   const DevConfig dc = dev_config(c->cfg);
   const size_t M = (size_t)nv * c->N;
@@ -431,10 +432,10 @@ sv_status enqueue_build(sv_ctx *c) {
     Scoped t(c, PH_EMIT);
     if (c->key64)
       launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
-                            (uint64_t *)c->keys, c->vals);
+                            (uint64_t *)c->keys, c->vals, zr);
     else
       launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
-                            (uint32_t *)c->keys, c->vals);
+                            (uint32_t *)c->keys, c->vals, zr);
   }
   {
     Scoped t(c, PH_SORT);
@@ -482,10 +483,11 @@ sv_status read_meta(sv_ctx *c, uint64_t *grow_to) {
 }
 
 // Build and wait (the per-view entry points and the plain / sequential
-// steps read the pair count on the host).
-sv_status build_batch(sv_ctx *c) {
+// steps read the pair count on the host). zero_records: clear the adjoint
+// records for an adjoint backward of this build.
+sv_status build_batch(sv_ctx *c, bool zero_records = true) {
   for (int attempt = 0; attempt < 3; ++attempt) {
-    if (sv_status s = enqueue_build(c)) return s;
+    if (sv_status s = enqueue_build(c, zero_records)) return s;
     uint64_t grow_to = 0;
     if (sv_status s = read_meta(c, &grow_to)) return s;
     if (!grow_to) {
@@ -782,7 +784,7 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
 // `hp` or, when hp is null, from the mapped host block h_adam (re-read on
 // every replay).
 sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
-  if (sv_status s = enqueue_build(c)) return s;
+  if (sv_status s = enqueue_build(c, true)) return s;
   if (sv_status s = enqueue_forward(c)) return s;
   const bool use_res = batch_needs_residual_buffer(c);
   if (use_res)
@@ -1336,7 +1338,7 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
     for (int i = 0; i < nv; ++i) {
       const int v = vs[i];
       if ((s = prepare_batch(c, &v, 1))) return s;
-      if ((s = build_batch(c))) return s;
+      if ((s = build_batch(c, false))) return s;
       total_isect += c->I;
       if ((s = forward_batch(c))) return s;
       if ((s = loss_views(c))) return s;
@@ -1385,7 +1387,7 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
 sv_status sv_render_batched(sv_ctx *c, const int *vs, int nv) {
   if (sv_status s = check_ctx(c)) return s;
   if (sv_status s = prepare_batch(c, vs, nv)) return s;
-  if (sv_status s = build_batch(c)) return s;
+  if (sv_status s = build_batch(c, false)) return s;
   if (sv_status s = forward_batch(c)) return s;
   return sync(c);
 }
@@ -1646,7 +1648,7 @@ sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, 
     if (!c->views[vs[i]].has_target) return fail(c, SV_E_STATE, "view %d has no target", vs[i]);
   }
   if (sv_status s = prepare_batch(c, vs, nv)) return s;
-  if (sv_status s = build_batch(c)) return s;
+  if (sv_status s = build_batch(c, false)) return s;
   if (sv_status s = forward_batch(c)) return s;
   launch_tile_l1(c->st, c->d_views, c->tile_view, c->tiles_total, c->color, c->target, c->tile_l1);
   if (sv_status s = check_launch(c, "tile_l1")) return s;
@@ -1704,6 +1706,93 @@ sv_status sv_read_tile_l1(sv_ctx *c, int v, float *out) {
       return SV_OK;
     }
   return fail(c, SV_E_STATE, "view %d was not scored by the last sv_densify_decide", v);
+}
+
+sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry *out, int *count) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!count) return fail(c, SV_E_ARG, "count is required");
+  if (!vs || nv <= 0 || steps <= 0) return fail(c, SV_E_ARG, "views and a positive step count are required");
+  for (int i = 0; i < nv; ++i) {
+    if (sv_status s = check_view(c, vs[i])) return s;
+    if (!c->views[vs[i]].has_target) return fail(c, SV_E_STATE, "view %d has no target", vs[i]);
+  }
+  const uint32_t batches[3] = {64, 128, 256};
+  std::vector<sv_tune_entry> cand;
+  for (uint32_t b : batches)
+    for (uint32_t p = 0; p < 2; ++p)
+      for (uint32_t x = 0; x < 2; ++x) cand.push_back(sv_tune_entry{b, p, x, 16, 0.0});
+  if (!out) {
+    *count = (int)cand.size();
+    return SV_OK;
+  }
+  // saved state: parameters, moments, statistics, step counter
+  const size_t plane = (size_t)P_NUM * c->cap;
+  float *sP = nullptr, *sM1 = nullptr, *sM2 = nullptr;
+  float4 *sImp = nullptr, *sAcc = nullptr;
+  sv_status st = dalloc(c, &sP, plane);
+  if (!st) st = dalloc(c, &sM1, plane);
+  if (!st) st = dalloc(c, &sM2, plane);
+  if (!st) st = dalloc(c, &sImp, c->cap);
+  if (!st) st = dalloc(c, &sAcc, c->cap);
+  auto copy_all = [&](bool save) -> sv_status {
+    float *dst[3] = {sP, sM1, sM2}, *src[3] = {c->P, c->M1, c->M2};
+    for (int k = 0; k < 3; ++k)
+      HIP_OK(c, hipMemcpyAsync(save ? dst[k] : src[k], save ? src[k] : dst[k], plane * sizeof(float),
+                               hipMemcpyDeviceToDevice, c->st));
+    HIP_OK(c, hipMemcpyAsync(save ? sImp : c->importance, save ? c->importance : sImp, c->cap * sizeof(float4),
+                             hipMemcpyDeviceToDevice, c->st));
+    HIP_OK(c, hipMemcpyAsync(save ? sAcc : c->grad_accum, save ? c->grad_accum : sAcc, c->cap * sizeof(float4),
+                             hipMemcpyDeviceToDevice, c->st));
+    return sync(c);
+  };
+  const sv_config cfg0 = c->cfg;
+  const uint32_t t0 = c->step_t;
+  const bool timing0 = c->timing;
+  c->timing = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = fail(c, SV_E_HIP, "hipEventCreate failed");
+  if (!st) st = copy_all(true);
+  for (size_t i = 0; i < cand.size() && !st; ++i) {
+    c->cfg = cfg0;
+    c->cfg.batch = cand[i].batch;
+    c->cfg.persistent = cand[i].persistent;
+    c->cfg.xcd_remap = cand[i].xcd_remap;
+    c->step_t = t0;
+    if ((st = step_batched(c, vs, nv))) break;  // untimed: compiles, grows buffers, captures
+    if ((st = copy_all(false))) break;
+    c->step_t = t0;
+    if (hipEventRecord(e0, c->st) != hipSuccess) st = fail(c, SV_E_HIP, "hipEventRecord failed");
+    for (int k = 0; k < steps && !st; ++k) st = step_batched(c, vs, nv);
+    if (st) break;
+    float ms = 0.0f;
+    if (hipEventRecord(e1, c->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      st = fail(c, SV_E_HIP, "tuner timing failed");
+    cand[i].ms_per_step = (double)ms / steps;
+    if (!st) st = copy_all(false);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  dfree(sP);
+  dfree(sM1);
+  dfree(sM2);
+  dfree(sImp);
+  dfree(sAcc);
+  c->step_t = t0;
+  c->timing = timing0;
+  c->cfg = cfg0;
+  if (st) return st;
+  size_t best = 0;
+  for (size_t i = 1; i < cand.size(); ++i)
+    if (cand[i].ms_per_step < cand[best].ms_per_step) best = i;
+  c->cfg.batch = cand[best].batch;
+  c->cfg.persistent = cand[best].persistent;
+  c->cfg.xcd_remap = cand[best].xcd_remap;
+  const int n = std::min(*count, (int)cand.size());
+  for (int i = 0; i < n; ++i) out[i] = cand[i];
+  *count = (int)cand.size();
+  return SV_OK;
 }
 
 sv_status sv_enable_timing(sv_ctx *c, int enable) {

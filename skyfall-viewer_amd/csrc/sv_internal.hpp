@@ -26,11 +26,12 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // Derive dbits, key bits, sort passes, result buffer and overflow from
 // the scan total and the largest depth quantum.
 void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap);
-// emit_intersections (training-tiles.wgsl:201-247) for nv slots.
+// emit_intersections (training-tiles.wgsl:201-247) for nv slots; with
+// zero_rec also zeroes the pairs' 9-float adjoint records.
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
                  const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
-                 K *keys, uint32_t *vals);
+                 K *keys, uint32_t *vals, float *zero_rec);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
 // host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
 // Grid sized for the capacity; n, dbits and the result buffer from m.

@@ -251,7 +251,7 @@ __device__ __forceinline__ void wave_sum_records(const float *__restrict__ recs,
   for (uint32_t base = span_lo; base < span_hi; base += VJP_CHUNK) {
     const uint32_t n = min(VJP_CHUNK, span_hi - base);
     const float *src = recs + (size_t)base * REC9;
-    for (uint32_t z = (uint32_t)lane; z < REC9 * n; z += 64u) s_buf[z] = __builtin_nontemporal_load(src + z);
+    for (uint32_t z = (uint32_t)lane; z < REC9 * n; z += 64u) s_buf[z] = src[z];
     wave_lds_sync();
     const uint32_t j0 = max(lo, base), j1 = min(hi, base + n);
     for (uint32_t j = j0; j < j1; ++j) {

@@ -47,13 +47,22 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
   for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_h[0][0])[i] = 0u;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE + (uint64_t)wave * (64 * SORT_ITEMS);
-#pragma unroll
+  // lanes with equal digits are matched by ballots and counted with one
+  // LDS add by the lowest of them (high digit passes have few distinct
+  // digits, where per-lane atomics would serialise)
+#pragma unroll 4
   for (int j = 0; j < SORT_ITEMS; ++j) {
     const uint64_t i = base + (uint64_t)j * 64 + lane;
-    if (i < n) {
-      const uint32_t d = (uint32_t)(keys[i] >> shift) & (RADIX - 1);
-      atomicAdd(&s_h[wave][d], 1u);
+    const bool valid = i < n;
+    const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & (RADIX - 1) : 0u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < RB; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const uint64_t bb = __ballot(on);
+      m &= on ? bb : ~bb;
     }
+    if (valid && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&s_h[wave][d], (uint32_t)__popcll(m));
   }
   __syncthreads();
   for (int d = threadIdx.x; d < RADIX; d += SORT_T) {

@@ -787,6 +787,36 @@ napi_value Timings(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 3)) return nullptr;
+  sv_ctx *c = ctx_of(env, a.v[0]);
+  if (!c) return nullptr;
+  std::vector<int> vs;
+  int32_t steps;
+  if (!views_from_js(env, a.v[1], vs) || !get_i32(env, a.v[2], &steps)) return nullptr;
+  int n = 0;
+  SV(c, sv_autotune(c, vs.data(), (int)vs.size(), steps, nullptr, &n), "sv_autotune");
+  std::vector<sv_tune_entry> t(n);
+  SV(c, sv_autotune(c, vs.data(), (int)vs.size(), steps, t.data(), &n), "sv_autotune");
+  napi_value arr;
+  CK(napi_create_array(env, &arr));
+  for (int i = 0; i < n; ++i) {
+    napi_value o, v;
+    CK(napi_create_object(env, &o));
+    const uint32_t f[4] = {t[i].batch, t[i].persistent, t[i].xcd_remap, t[i].tile_size};
+    const char *names[4] = {"batch", "persistent", "xcd_remap", "tile"};
+    for (int k = 0; k < 4; ++k) {
+      CK(napi_create_uint32(env, f[k], &v));
+      CK(napi_set_named_property(env, o, names[k], v));
+    }
+    CK(napi_create_double(env, t[i].ms_per_step, &v));
+    CK(napi_set_named_property(env, o, "msPerStep", v));
+    CK(napi_set_element(env, arr, (uint32_t)i, o));
+  }
+  return arr;
+}
+
 napi_value Sync(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
@@ -878,7 +908,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readAlpha", ReadAlpha},     {"readLastIds", ReadLastIds}, {"readGrads", ReadGrads},
              {"readKeys", ReadKeys},       {"pruneOpacity", PruneOpacity}, {"pruneMask", PruneMask},
              {"densify", Densify},         {"resetOpacity", ResetOpacity},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
-             {"timings", Timings},         {"sync", Sync},               {"sceneGaussians", SceneGaussians},
+             {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},               {"sceneGaussians", SceneGaussians},
              {"sceneJitter", SceneJitter}, {"sceneCamera", SceneCamera}};
   for (auto &f : fns) {
     napi_value v;

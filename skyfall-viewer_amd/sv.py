@@ -46,6 +46,11 @@ class StepStats(C.Structure):
                 ("key_bits", C.c_uint32), ("pad", C.c_uint32), ("l1_sum", C.c_double)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("batch", C.c_uint32), ("persistent", C.c_uint32), ("xcd_remap", C.c_uint32),
+                ("tile_size", C.c_uint32), ("ms_per_step", C.c_double)]
+
+
 class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_double), ("launches", C.c_uint64)]
 
@@ -101,6 +106,8 @@ _SIGS = {
     "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, _U8,
                                     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
+    "sv_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(TuneEntry),
+                              C.POINTER(C.c_int)]),
     "sv_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sv_timings": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.POINTER(C.c_int)]),
     "sv_reset_timings": (C.c_int, [C.c_void_p]),
@@ -435,6 +442,16 @@ class Context:
         n = C.c_size_t()
         self._ck(lib().sv_densify(self.h, a.ctypes.data_as(_U8), seed, C.byref(n)), "densify")
         return n.value
+
+    def autotune(self, views, steps: int = 3) -> list:
+        """Time every tuner candidate on `views`; the fastest stays configured."""
+        vs = (C.c_int * len(views))(*views)
+        n = C.c_int(0)
+        self._ck(lib().sv_autotune(self.h, vs, len(views), steps, None, C.byref(n)), "autotune")
+        arr = (TuneEntry * n.value)()
+        self._ck(lib().sv_autotune(self.h, vs, len(views), steps, arr, C.byref(n)), "autotune")
+        return [{"batch": e.batch, "persistent": e.persistent, "xcd_remap": e.xcd_remap, "tile": e.tile_size,
+                 "ms_per_step": e.ms_per_step} for e in arr]
 
     def reset_opacity(self, max_opacity: float = 0.1) -> int:
         n = C.c_size_t()

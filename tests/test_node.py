@@ -26,8 +26,8 @@ EXPORTS = ["version", "create", "destroy", "getConfig", "setConfig", "setGaussia
            "getMoments", "getCount", "setViews", "setTarget", "setResidual", "buildTiles", "forward", "loss",
            "backward", "backwardFixed", "zeroGrad", "adam", "step", "stepAsync", "readImage", "readResidual",
            "readAlpha", "readLastIds", "readGrads", "readKeys", "pruneOpacity", "pruneMask", "densify",
-           "resetOpacity", "densifyDecide", "enableTiming", "timings", "sync", "sceneGaussians", "sceneJitter",
-           "sceneCamera"]
+           "resetOpacity", "densifyDecide", "enableTiming", "timings", "sync", "autotune", "sceneGaussians",
+           "sceneJitter", "sceneCamera"]
 
 
 def _node(script, *args, timeout=120):
