@@ -72,6 +72,22 @@ def step_b_alg(N: int, V: int, I: int, P: int, d: int) -> float:
     return 1488.0 * N + 92.0 * N * V + I * (136.0 + 24.0 * d) + 40.0 * P
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1M_16x1080p.json")
+
+
+def measured_traffic(phase: str, N: int, V: int, W: int, H: int, workload: str):
+    """HBM bytes per launch of `phase` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was measured on this workload; else None."""
+    if workload != "ring16" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as fh:
+        t = json.load(fh)
+    if t["workload"] != {"splats": N, "views": V, "width": W, "height": H} or phase not in t["phases"]:
+        return None
+    k = t["phases"][phase]
+    return k["fetch_bytes"] + k["write_bytes"]
+
+
 def run_mode(mode, g, cams, targets, args, graph=0):
     """Returns (s/step of the timed loop, per-phase timings, phase steps,
     stats of one more step, s/step of eager launches or None)."""
@@ -156,6 +172,9 @@ def cpu_baseline(g, cams, targets, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=("ring16", "uneven64"), default="ring16",
+                    help="ring16: 16 ring views at --width x --height (the headline); uneven64: 64 views "
+                         "with seeded sizes 320-1920 x 240-1080 on a sphere of radius 4, means in [-2,2]^3")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -190,9 +209,19 @@ def main():
         sys.exit(2)
 
     N, V, W, H = args.n, args.views, args.width, args.height
-    log(f"scene: {N} splats, {V} views {W}x{H}")
-    g = sv.scene_gaussians(0, N, 1.0)
-    cams = sv.ring_cameras(V, 3.0, W, H)
+    if args.workload == "uneven64":
+        V = 64
+        g = sv.scene_gaussians(0, N, 2.0)
+        cams, sizes = sv.sphere_cameras(2, V, 4.0)
+        P = sum(w * h for w, h in sizes)
+        workload = (f"{N} splats in [-2,2]^3, {V} views of seeded sizes 320-1920 x 240-1080 "
+                    f"({P} pixels) per step, batched adjoint training step")
+    else:
+        g = sv.scene_gaussians(0, N, 1.0)
+        cams = sv.ring_cameras(V, 3.0, W, H)
+        P = V * W * H
+        workload = f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step"
+    log(f"scene: {workload}")
     tg = dict(g)
     tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
     targets = []
@@ -224,7 +253,6 @@ def main():
         log(f"plain: {1e3 * sec_p:.2f} ms/step")
 
     I = int(st_b.n_isect)
-    P = V * W * H
     key_bytes = 8 if st_b.key_bits > 32 else 4
     passes = int(st_b.sort_passes)
     phase = {k: (v[0] / max(v[1], 1), v[1] / ps_b) for k, v in tim_b.items() if v[1]}
@@ -237,6 +265,7 @@ def main():
     dom_bytes = kernel_bytes(dom, N, V, I, P, key_bytes, passes)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     b_alg = step_b_alg(N, V, I, P, passes)
+    traffic = measured_traffic(dom, N, V, W, H, args.workload)
     out = {
         "metric": METRIC,
         "value": 1.0 / sec_b,
@@ -250,8 +279,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
-        "config": {"workload": f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step",
-                   "splats": N, "views": V, "width": W, "height": H, "tile": 16, "lds_batch": chosen_b["batch"],
+        "config": {"workload": workload, "splats": N, "views": V,
+                   "width": W if args.workload == "ring16" else None,
+                   "height": H if args.workload == "ring16" else None, "pixels": P,
+                   "tile": 16, "lds_batch": chosen_b["batch"],
                    "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"]},
         "autotune": tune_b,
         "eager_launch_ms_per_step": 1e3 * eager_b if eager_b else None,
@@ -268,7 +299,9 @@ def main():
         "step_b_alg_GBps": b_alg / sec_b / 1e9,
         "step_b_alg_frac": b_alg / sec_b / 1e9 / HBM_PEAK_GBPS,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": "profiles/hbm_traffic_1M_16x1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                       "copy-calibrated)" if traffic else None,
                      "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms},
         "cpu_baseline": None,
     }

@@ -98,6 +98,7 @@ struct sv_ctx {
   uint32_t *tile_view = nullptr, *ranges = nullptr;
   float *l1_partial = nullptr;
   float *tile_l1 = nullptr;  // densify scoring, per global tile
+  uint64_t *tile_cut = nullptr;  // per global tile: first unreached entry's (depth_q, id)
   size_t tiles_cap = 0;
   std::vector<int> scored_views;  // batch of the last sv_densify_decide
   std::vector<uint32_t> scored_base;
@@ -336,6 +337,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       dfree(c->ranges);
       dfree(c->l1_partial);
       dfree(c->tile_l1);
+      dfree(c->tile_cut);
       c->tiles_cap = 0;
       c->scored_views.clear();
       size_t want = (size_t)tb + 1;
@@ -343,6 +345,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       if (sv_status s = dalloc(c, &c->ranges, want)) return s;
       if (sv_status s = dalloc(c, &c->l1_partial, want)) return s;
       if (sv_status s = dalloc(c, &c->tile_l1, want)) return s;
+      if (sv_status s = dalloc(c, &c->tile_cut, want)) return s;
       c->tiles_cap = want;
     }
     std::vector<uint32_t> tv(tb);
@@ -412,8 +415,7 @@ sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
 // enqueued without any host readback: the pair count, key layout, sort
 // passes and result buffer stay on the device (StepMeta), and every launch
 // is sized by capacities, so the same launches serve any step.
-sv_status enqueue_build(sv_ctx *c, bool zero_records) {
-  float *zr = zero_records ? c->isect_rec : nullptr;
+sv_status enqueue_build(sv_ctx *c) {
   const int nv = (int)c->slot_views.size();
   const DevConfig dc = dev_config(c->cfg);
   const size_t M = (size_t)nv * c->N;
@@ -432,10 +434,10 @@ sv_status enqueue_build(sv_ctx *c, bool zero_records) {
     Scoped t(c, PH_EMIT);
     if (c->key64)
       launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
-                            (uint64_t *)c->keys, c->vals, zr);
+                            (uint64_t *)c->keys, c->vals);
     else
       launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
-                            (uint32_t *)c->keys, c->vals, zr);
+                            (uint32_t *)c->keys, c->vals);
   }
   {
     Scoped t(c, PH_SORT);
@@ -483,11 +485,10 @@ sv_status read_meta(sv_ctx *c, uint64_t *grow_to) {
 }
 
 // Build and wait (the per-view entry points and the plain / sequential
-// steps read the pair count on the host). zero_records: clear the adjoint
-// records for an adjoint backward of this build.
-sv_status build_batch(sv_ctx *c, bool zero_records = true) {
+// steps read the pair count on the host).
+sv_status build_batch(sv_ctx *c) {
   for (int attempt = 0; attempt < 3; ++attempt) {
-    if (sv_status s = enqueue_build(c, zero_records)) return s;
+    if (sv_status s = enqueue_build(c)) return s;
     uint64_t grow_to = 0;
     if (sv_status s = read_meta(c, &grow_to)) return s;
     if (!grow_to) {
@@ -523,6 +524,10 @@ RasterArgs raster_args(sv_ctx *c) {
   a.residual = nullptr;
   a.offs = c->offs;
   a.isect_rec = c->isect_rec;
+  a.keys0 = c->keys;
+  a.keys1 = c->keys2;
+  a.key64 = c->key64 ? 1u : 0u;
+  a.tile_cut = c->tile_cut;
   a.l1_partial = c->l1_partial;
   a.work_counter = nullptr;
   return a;
@@ -618,6 +623,7 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.rec = c->rec;
   v.aux = c->aux;
   v.isect_rec = c->isect_rec;
+  v.tile_cut = c->tile_cut;
   v.importance = c->importance;
   v.G = c->G;
   v.geom = c->geom;
@@ -784,7 +790,7 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
 // `hp` or, when hp is null, from the mapped host block h_adam (re-read on
 // every replay).
 sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
-  if (sv_status s = enqueue_build(c, true)) return s;
+  if (sv_status s = enqueue_build(c)) return s;
   if (sv_status s = enqueue_forward(c)) return s;
   const bool use_res = batch_needs_residual_buffer(c);
   if (use_res)
@@ -816,7 +822,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
   std::vector<uint64_t> g;
   const void *ptrs[] = {c->P, c->M1, c->M2, c->G, c->geom, c->importance, c->grad_accum, c->color,
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
-                        c->l1_partial, c->rec, c->aux, c->counts, c->offs, c->scan_tmp, c->keys,
+                        c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
                         c->h_adam_dev};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
@@ -1093,6 +1099,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->ranges);
   dfree(c->l1_partial);
   dfree(c->tile_l1);
+  dfree(c->tile_cut);
   dfree(c->rec);
   dfree(c->aux);
   dfree(c->counts);
@@ -1338,7 +1345,7 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
     for (int i = 0; i < nv; ++i) {
       const int v = vs[i];
       if ((s = prepare_batch(c, &v, 1))) return s;
-      if ((s = build_batch(c, false))) return s;
+      if ((s = build_batch(c))) return s;
       total_isect += c->I;
       if ((s = forward_batch(c))) return s;
       if ((s = loss_views(c))) return s;
@@ -1387,7 +1394,7 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
 sv_status sv_render_batched(sv_ctx *c, const int *vs, int nv) {
   if (sv_status s = check_ctx(c)) return s;
   if (sv_status s = prepare_batch(c, vs, nv)) return s;
-  if (sv_status s = build_batch(c, false)) return s;
+  if (sv_status s = build_batch(c)) return s;
   if (sv_status s = forward_batch(c)) return s;
   return sync(c);
 }
@@ -1648,7 +1655,7 @@ sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, 
     if (!c->views[vs[i]].has_target) return fail(c, SV_E_STATE, "view %d has no target", vs[i]);
   }
   if (sv_status s = prepare_batch(c, vs, nv)) return s;
-  if (sv_status s = build_batch(c, false)) return s;
+  if (sv_status s = build_batch(c)) return s;
   if (sv_status s = forward_batch(c)) return s;
   launch_tile_l1(c->st, c->d_views, c->tile_view, c->tiles_total, c->color, c->target, c->tile_l1);
   if (sv_status s = check_launch(c, "tile_l1")) return s;

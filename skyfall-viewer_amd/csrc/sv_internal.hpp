@@ -26,12 +26,11 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // Derive dbits, key bits, sort passes, result buffer and overflow from
 // the scan total and the largest depth quantum.
 void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap);
-// emit_intersections (training-tiles.wgsl:201-247) for nv slots; with
-// zero_rec also zeroes the pairs' 9-float adjoint records.
+// emit_intersections (training-tiles.wgsl:201-247) for nv slots.
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
                  const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
-                 K *keys, uint32_t *vals, float *zero_rec);
+                 K *keys, uint32_t *vals);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
 // host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
 // Grid sized for the capacity; n, dbits and the result buffer from m.
@@ -77,7 +76,12 @@ struct RasterArgs {
   const float *residual;  // 3 planes per view, or null for the fused loss
   // backward outputs
   const uint32_t *offs;   // per (slot, Gaussian) first intersection (emit order)
-  float *isect_rec;       // 64-byte adjoint record per intersection (emit order)
+  float *isect_rec;       // 9-float adjoint record per intersection (emit order)
+  // sorted keys (buffer per meta->sel) and, per global tile, the sort key
+  // (depth_q << 32 | Gaussian) of the first entry no pixel reaches
+  const void *keys0, *keys1;
+  uint32_t key64;
+  uint64_t *tile_cut;
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent queue counter (null: one block per tile)
 };
@@ -100,7 +104,8 @@ struct VjpArgs {
   const uint32_t *offs;
   const float4 *rec;       // projection records
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
-  const float *isect_rec;  // 64-byte adjoint record per intersection
+  const float *isect_rec;  // 9-float adjoint record per intersection
+  const uint64_t *tile_cut; // per global tile: records at or past this (depth_q, id) were not written
   float4 *importance;      // importance statistics (may be null)
   float *G;                // gradient planes (accumulate), or null when fused
   float *geom;             // 5 planes: grad_geom of the last slot (may be null)

@@ -220,54 +220,6 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
   return k < P_SCALE ? 0 : (k < P_QUAT ? 1 : (k < P_LOGIT ? 2 : (k < P_SH ? 3 : 4)));
 }
 
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
-}
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-constexpr uint32_t VJP_CHUNK = 160;  // records staged per wave (5.6 KB of LDS)
-
-// Sum of the lane's adjoint records [off, off + cnt) of one view (fixed
-// order: the first record, then + each following one). The 64 lanes' ranges
-// are adjacent (emit order), so the wave loads the whole span with
-// coalesced dword loads into LDS and each lane sums its part from there.
-// All lanes of the wave must call this together.
-__device__ __forceinline__ void wave_sum_records(const float *__restrict__ recs, uint32_t off, uint32_t cnt,
-                                                 float r[REC9], float *s_buf) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t lo = cnt ? off : 0xffffffffu, hi = cnt ? off + cnt : 0u;
-  const uint32_t span_lo = wave_min_u32(lo), span_hi = wave_max_u32(hi);
-  for (uint32_t base = span_lo; base < span_hi; base += VJP_CHUNK) {
-    const uint32_t n = min(VJP_CHUNK, span_hi - base);
-    const float *src = recs + (size_t)base * REC9;
-    for (uint32_t z = (uint32_t)lane; z < REC9 * n; z += 64u) s_buf[z] = src[z];
-    wave_lds_sync();
-    const uint32_t j0 = max(lo, base), j1 = min(hi, base + n);
-    for (uint32_t j = j0; j < j1; ++j) {
-      const float *x = s_buf + REC9 * (j - base);
-      if (j == off) {
-#pragma unroll
-        for (int q = 0; q < REC9; ++q) r[q] = x[q];
-      } else {
-#pragma unroll
-        for (int q = 0; q < REC9; ++q) r[q] = r[q] + x[q];
-      }
-    }
-    wave_lds_sync();
-  }
-}
-
 // ---------------------------------------------------------------------
 // Per-Gaussian backward over the views of a step: sum the Gaussian's
 // per-tile adjoint records of each view (emit order, fixed), apply the SH
@@ -278,12 +230,10 @@ __device__ __forceinline__ void wave_sum_records(const float *__restrict__ recs,
 // cover all active elements).
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
-  __shared__ float s_recs[4][REC9 * VJP_CHUNK];
   if (a.meta && a.meta->overflow) return;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  // lanes past N take part in the wave's record loads and store nothing
-  const bool live = g < a.N;
-  const uint32_t gl = live ? g : a.N - 1u;
+  if (g >= a.N) return;
+  const uint32_t gl = g;
   float prm[11];
 #pragma unroll
   for (int k = 0; k < 11; ++k) {
@@ -302,9 +252,7 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
   float4 imp = a.importance ? a.importance[gl] : make_float4(0.f, 0.f, 0.f, 0.f);
   for (int s = 0; s < a.nv; ++s) {
     const size_t vg = (size_t)s * a.N + gl;
-    const uint32_t cnt = live ? a.counts[vg] : 0u;
-    float r[REC9];
-    wave_sum_records(a.isect_rec, cnt ? a.offs[vg] : 0u, cnt, r, s_recs[threadIdx.x >> 6]);
+    const uint32_t cnt = a.counts[vg];
     // accumulate_importance (importance-accum.wgsl:19-40), once per view
     imp.x = imp.x * a.cfg.importance_decay;
     imp.y = imp.y * a.cfg.importance_decay;
@@ -314,9 +262,38 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     if (cnt == 0u) continue;
     imp.x = imp.x + (float)cnt;
     imp.y = imp.y + 1.0f;
-    imp.z = fmaxf(imp.z, a.aux[vg].y);
+    const float2 ax = a.aux[vg];
+    imp.z = fmaxf(imp.z, ax.y);
     any = true;
     const DevView &V = a.views[s];
+    // sum of the records of the tiles' entries some pixel reached, in emit
+    // order (ty-major rectangle); a tile's later entries have no record
+    float r[REC9];
+#pragma unroll
+    for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
+    {
+      const float4 q3 = a.rec[vg * 4 + 3];
+      const uint32_t ra = __float_as_uint(q3.y), rb = __float_as_uint(q3.z);
+      const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, w = (rb & 0xffffu) - x0;
+      const uint32_t dq = f2u_sat(clampf(ax.x, 0.0f, 65535.0f)) & 0xffffu;
+      const uint64_t me = ((uint64_t)dq << 32) | g;
+      const uint32_t off = a.offs[vg];
+      bool first = true;
+      for (uint32_t j = 0; j < cnt; ++j) {
+        const uint32_t dy = j / w, dx = j - dy * w;
+        const uint32_t tile = V.tile_base + (y0 + dy) * V.tiles_x + (x0 + dx);
+        if (me >= a.tile_cut[tile]) continue;
+        const float *rp = a.isect_rec + (size_t)(off + j) * REC9;
+        if (first) {
+#pragma unroll
+          for (int q = 0; q < REC9; ++q) r[q] = rp[q];
+          first = false;
+        } else {
+#pragma unroll
+          for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[q];
+        }
+      }
+    }
     float pc[4];
     m4_mul_pos(V.view, pre.wx, pre.wy, pre.wz, pc);
     const float nx = -pc[0], ny = -pc[1], nz = -pc[2];
@@ -344,7 +321,6 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) geo[q] = r[4 + q];
   }
-  if (!live) return;
   if (a.grad_accum) a.grad_accum[g] = dacc;
   if (a.importance) a.importance[g] = imp;
   if (a.geom) {

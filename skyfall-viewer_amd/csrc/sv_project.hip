@@ -362,8 +362,7 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
                                               const uint32_t *__restrict__ counts,
                                               const uint32_t *__restrict__ offs, uint32_t N, int nv,
                                               const DevView *__restrict__ views, const StepMeta *m,
-                                              K *__restrict__ keys, uint32_t *__restrict__ vals,
-                                              float *__restrict__ zero_rec) {
+                                              K *__restrict__ keys, uint32_t *__restrict__ vals) {
   __shared__ uint32_t s_lo[256];   // first output position (0xffffffff past the items)
   __shared__ uint32_t s_rect[256]; // x0 | width << 16
   __shared__ uint32_t s_y0[256];
@@ -418,27 +417,23 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
     keys[k] = (tile << dbits) | (K)s_dq[j];
     vals[k] = s_g[j];
   }
-  // the adjoint records of the same pairs start at zero: the backward only
-  // writes those of pairs that contribute to some pixel
-  if (zero_rec)
-    for (size_t z = (size_t)span_lo * REC9 + t; z < (size_t)span_hi * REC9; z += 256u) zero_rec[z] = 0.0f;
 }
 
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
                  const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
-                 K *keys, uint32_t *vals, float *zero_rec) {
+                 K *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
   hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, aux, counts,
-                     offs, N, nv, views, m, keys, vals, zero_rec);
+                     offs, N, nv, views, m, keys, vals);
 }
 template void launch_emit<uint32_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
                                     const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
-                                    uint32_t *, uint32_t *, float *);
+                                    uint32_t *, uint32_t *);
 template void launch_emit<uint64_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
                                     const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
-                                    uint64_t *, uint32_t *, float *);
+                                    uint64_t *, uint32_t *);
 
 // ---------------------------------------------------------------------
 // Tile ranges: ranges[t] = first sorted index whose tile is >= t, which is

@@ -291,7 +291,6 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
   __shared__ uint32_t s_k[B];
   __shared__ uint32_t s_p[B];
   __shared__ float s_out[B * 4 * REC9];
-  __shared__ uint32_t s_hit[B];
   __shared__ uint32_t s_tg;
   const int lane = threadIdx.x;
   uint32_t tg = first_tile(a, &s_tg);
@@ -347,7 +346,20 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
       if (lane == 63 && a.l1_partial) a.l1_partial[tg] = l1t;
       uint32_t tend = wave_max_u32(m);
       if (tend < start) tend = start;
-      // entries no pixel reaches keep the zero records written by emit
+      // Entries from tend on are reached by no pixel: their records are not
+      // written, and the VJP skips every record at or past this tile's cut
+      // (the sort key (depth_q, id) of entry tend; the list is ordered by it)
+      if (lane == 0) {
+        uint64_t cut = ~0ull;
+        if (tend < end) {
+          const uint32_t dmask = (1u << a.meta->dbits) - 1u;
+          const uint32_t sel = a.meta->sel;
+          const uint32_t dq = a.key64 ? (uint32_t)(static_cast<const uint64_t *>(sel ? a.keys1 : a.keys0)[tend] & dmask)
+                                      : (static_cast<const uint32_t *>(sel ? a.keys1 : a.keys0)[tend] & dmask);
+          cut = ((uint64_t)dq << 32) | a.vals[tend];
+        }
+        a.tile_cut[tg] = cut;
+      }
       const float fx = (float)t.px + 0.5f;
       const float tx0 = (float)(t.tx * 16u) + 0.5f, tx1 = (float)min(t.tx * 16u + 15u, V.W - 1u) + 0.5f;
       const float ty0 = (float)(t.ty * 16u) + 0.5f, ty1 = (float)min(t.ty * 16u + 15u, V.H - 1u) + 0.5f;
@@ -355,7 +367,6 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
         for (uint32_t z = lane; z < cnt * 4 * REC9; z += WT) s_out[z] = 0.0f;
-        for (uint32_t z = lane; z < cnt; z += WT) s_hit[z] = 0u;
         const uint32_t kept =
             load_batch<B>(a, vbase, lo, cnt, tx0, tx1, ty0, ty1, s_q0, s_q1, s_q2, s_k, s_p, t.tx, t.ty);
         __syncthreads();
@@ -405,7 +416,6 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
             }
           }
           if (!__any(hit)) continue;
-          if (lane == 0) s_hit[k] = 1u;
           v0 = row_total(v0);
           v1 = row_total(v1);
           v2 = row_total(v2);
@@ -430,10 +440,8 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
           }
         }
         __syncthreads();
-        // records of entries with a contribution (the rest stay zero)
         for (uint32_t z = lane; z < cnt * REC9; z += WT) {
           const uint32_t k = z / REC9, q9 = z - k * REC9;
-          if (!s_hit[k]) continue;
           const uint32_t p = s_p[k];
           const float *o = &s_out[k * 4 * REC9 + q9];
           const float tot = ((o[3 * REC9] + o[2 * REC9]) + (o[REC9] + o[0]));
