@@ -266,6 +266,56 @@ sv_status sv_densify_decide(sv_ctx *ctx, const int *views, int nviews, float per
  * (tiles_x * tiles_y floats, row-major tiles). */
 sv_status sv_read_tile_l1(sv_ctx *ctx, int view, float *out);
 
+/* --- on-disk inputs (host only; errors via sv_ingest_error) ----------- */
+/* Message of the last failed sv_ply_* / sv_colmap_* / sv_points_* call on
+ * this thread. */
+const char *sv_ingest_error(void);
+/* decodeHeader (plyreader.ts:1-55): vertex count and SH degree (from the
+ * number of f_rest_* properties, load.ts:44-62) of a binary little-endian
+ * PLY. */
+sv_status sv_ply_info(const char *path, size_t *n, uint32_t *sh_deg);
+/* load (load.ts:14-206): 3DGS PLY (x, y, z, f_dc_*, f_rest_* channel-major,
+ * opacity, scale_*, rot_0..3 = (w, x, y, z)) or a plain point cloud (red,
+ * green, blue) into the library's arrays (quaternion (x, y, z, w), SH
+ * coefficient-major with rgb inner, unused bands 0), f16-rounded as the
+ * reference stores them; the reference's defaults fill what the file lacks.
+ * scene_extent (may be NULL) gets the largest bounding-box side. */
+sv_status sv_ply_load(const char *path, size_t n, float *pos, float *logscale, float *quat,
+                      float *logit, float *sh, float *scene_extent);
+/* Writes a 3DGS PLY of sh_deg (the format sv_ply_load reads). No
+ * counterpart in the reference's training path. */
+sv_status sv_ply_save(const char *path, size_t n, const float *pos, const float *logscale,
+                      const float *quat, const float *logit, const float *sh, uint32_t sh_deg);
+/* Image and point counts of a COLMAP sparse model directory (images.bin,
+ * points3D.bin). */
+sv_status sv_colmap_info(const char *sparse_dir, size_t *n_images, size_t *n_points);
+/* loadColmapDatasetScene (colmap-dataset.ts:129-326): training cameras of
+ * every image (view = [R | t] from qvec / tvec, view_inv its inverse, the
+ * pinhole projection of training-camera.ts:29-52, viewport, focal) for
+ * SIMPLE_PINHOLE / PINHOLE cameras, image names (name_len bytes each, may
+ * be NULL) and the 3D points with colours. */
+sv_status sv_colmap_load(const char *sparse_dir, size_t n_images, sv_camera *cams, char *names,
+                         size_t name_len, size_t n_points, float *xyz, uint8_t *rgb);
+/* createPointCloudFromXYZRGB (pointcloud-loader.ts:29-100): one Gaussian
+ * per point, colour as the SH DC term, the reference's default opacity,
+ * scale and rotation, f16-rounded. */
+sv_status sv_points_to_gaussians(size_t n, const float *xyz, const uint8_t *rgb, float *pos,
+                                 float *logscale, float *quat, float *logit, float *sh,
+                                 float *scene_extent);
+
+/* --- checkpoint / resume ------------------------------------------- */
+/* No counterpart in the reference (its nearest is the CPU sync of
+ * gaussian-renderer.ts:3129-3205). Writes the training state to `path`:
+ * header ("SVCKPT01", version, N, step, sv_config), then parameters, the
+ * two Adam moments (59 planes of N floats each), the importance and
+ * densify statistics (4 floats per Gaussian), then an FNV-1a 64 checksum
+ * of everything before it. The random draws of densify come from caller
+ * seeds, so there is no generator state to save. */
+sv_status sv_save_checkpoint(sv_ctx *ctx, const char *path);
+/* Restores a checkpoint written by sv_save_checkpoint (N must fit the
+ * context's capacity); views and targets are not part of it. */
+sv_status sv_load_checkpoint(sv_ctx *ctx, const char *path);
+
 /* --- autotuner ------------------------------------------------------ */
 /* No counterpart in the reference (its tile and batch sizes are
  * constants: 16x16 tiles, 64-entry batches, backward-tiles-local.wgsl:49-53,

@@ -1715,6 +1715,131 @@ sv_status sv_read_tile_l1(sv_ctx *c, int v, float *out) {
   return fail(c, SV_E_STATE, "view %d was not scored by the last sv_densify_decide", v);
 }
 
+namespace {
+
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  }
+};
+
+const char kCkptMagic[8] = {'S', 'V', 'C', 'K', 'P', 'T', '0', '1'};
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version, planes;
+  uint64_t n;
+  uint32_t step, config_bytes;
+};
+
+}  // namespace
+
+sv_status sv_save_checkpoint(sv_ctx *c, const char *path) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!path) return fail(c, SV_E_ARG, "null path");
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return fail(c, SV_E_ARG, "cannot open %s for writing", path);
+  Fnv h;
+  bool ok = true;
+  auto put = [&](const void *p, size_t n) {
+    if (ok && std::fwrite(p, 1, n, f) != n) ok = false;
+    h.add(p, n);
+  };
+  CkptHeader hd{};
+  std::memcpy(hd.magic, kCkptMagic, 8);
+  hd.version = 1;
+  hd.planes = P_NUM;
+  hd.n = c->N;
+  hd.step = c->step_t;
+  hd.config_bytes = (uint32_t)sizeof(sv_config);
+  put(&hd, sizeof(hd));
+  put(&c->cfg, sizeof(sv_config));
+  std::vector<float> plane(std::max<size_t>(c->N, 1) * 4);
+  sv_status st = SV_OK;
+  const float *arrays[3] = {c->P, c->M1, c->M2};
+  for (int a = 0; a < 3 && !st; ++a)
+    for (int k = 0; k < P_NUM && !st; ++k) {
+      if (hipMemcpy(plane.data(), arrays[a] + (size_t)k * c->cap, c->N * sizeof(float), hipMemcpyDeviceToHost) !=
+          hipSuccess)
+        st = fail(c, SV_E_HIP, "checkpoint readback failed");
+      else
+        put(plane.data(), c->N * sizeof(float));
+    }
+  const float4 *stats[2] = {c->importance, c->grad_accum};
+  for (int a = 0; a < 2 && !st; ++a) {
+    if (hipMemcpy(plane.data(), stats[a], c->N * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess)
+      st = fail(c, SV_E_HIP, "checkpoint readback failed");
+    else
+      put(plane.data(), c->N * sizeof(float4));
+  }
+  const uint64_t sum = h.h;
+  if (ok && std::fwrite(&sum, 1, sizeof(sum), f) != sizeof(sum)) ok = false;
+  if (std::fclose(f) != 0) ok = false;
+  if (st) return st;
+  if (!ok) return fail(c, SV_E_ARG, "writing %s failed", path);
+  return SV_OK;
+}
+
+sv_status sv_load_checkpoint(sv_ctx *c, const char *path) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!path) return fail(c, SV_E_ARG, "null path");
+  FILE *f = std::fopen(path, "rb");
+  if (!f) return fail(c, SV_E_ARG, "cannot open %s", path);
+  Fnv h;
+  bool ok = true;
+  auto get = [&](void *p, size_t n) {
+    if (ok && std::fread(p, 1, n, f) != n) ok = false;
+    if (ok) h.add(p, n);
+  };
+  CkptHeader hd{};
+  get(&hd, sizeof(hd));
+  if (!ok || std::memcmp(hd.magic, kCkptMagic, 8) != 0 || hd.version != 1 || hd.planes != (uint32_t)P_NUM ||
+      hd.config_bytes != sizeof(sv_config)) {
+    std::fclose(f);
+    return fail(c, SV_E_ARG, "%s is not a version-1 checkpoint of this library", path);
+  }
+  if (hd.n > c->cap) {
+    std::fclose(f);
+    return fail(c, SV_E_CAPACITY, "checkpoint holds %llu Gaussians, capacity is %zu", (unsigned long long)hd.n,
+                c->cap);
+  }
+  sv_config cfg;
+  get(&cfg, sizeof(cfg));
+  const size_t n = (size_t)hd.n;
+  std::vector<float> data((size_t)3 * P_NUM * n + 8 * n);
+  get(data.data(), data.size() * sizeof(float));
+  uint64_t sum = 0;
+  const uint64_t want = h.h;
+  if (ok && std::fread(&sum, 1, sizeof(sum), f) != sizeof(sum)) ok = false;
+  std::fclose(f);
+  if (!ok) return fail(c, SV_E_ARG, "%s is truncated", path);
+  if (sum != want) return fail(c, SV_E_ARG, "%s: checksum mismatch", path);
+  sv_config old = c->cfg;
+  if (sv_status s = sv_set_config(c, &cfg)) {
+    c->cfg = old;
+    return s;
+  }
+  HIP_OK(c, hipStreamSynchronize(c->st));
+  clear_state(c);
+  float *arrays[3] = {c->P, c->M1, c->M2};
+  const float *src = data.data();
+  for (int a = 0; a < 3; ++a)
+    for (int k = 0; k < P_NUM; ++k, src += n)
+      HIP_OK(c, hipMemcpy(arrays[a] + (size_t)k * c->cap, src, n * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(c, hipMemcpy(c->importance, src, n * sizeof(float4), hipMemcpyHostToDevice));
+  src += 4 * n;
+  HIP_OK(c, hipMemcpy(c->grad_accum, src, n * sizeof(float4), hipMemcpyHostToDevice));
+  c->N = (uint32_t)n;
+  c->step_t = hd.step;
+  c->slot_views.clear();
+  c->built = false;
+  c->forward_done = false;
+  return sync(c);
+}
+
 sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry *out, int *count) {
   if (sv_status s = check_ctx(c)) return s;
   if (!count) return fail(c, SV_E_ARG, "count is required");

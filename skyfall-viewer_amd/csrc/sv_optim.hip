@@ -271,29 +271,31 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     float r[REC9];
 #pragma unroll
     for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
+    bool first = true;
     {
       const float4 q3 = a.rec[vg * 4 + 3];
       const uint32_t ra = __float_as_uint(q3.y), rb = __float_as_uint(q3.z);
-      const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, w = (rb & 0xffffu) - x0;
+      const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
       const uint32_t dq = f2u_sat(clampf(ax.x, 0.0f, 65535.0f)) & 0xffffu;
       const uint64_t me = ((uint64_t)dq << 32) | g;
-      const uint32_t off = a.offs[vg];
-      bool first = true;
-      for (uint32_t j = 0; j < cnt; ++j) {
-        const uint32_t dy = j / w, dx = j - dy * w;
-        const uint32_t tile = V.tile_base + (y0 + dy) * V.tiles_x + (x0 + dx);
-        if (me >= a.tile_cut[tile]) continue;
-        const float *rp = a.isect_rec + (size_t)(off + j) * REC9;
-        if (first) {
+      const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
+      for (uint32_t ty = y0; ty < y1; ++ty) {
+        const uint64_t *cut = a.tile_cut + V.tile_base + ty * V.tiles_x;
+        for (uint32_t tx = x0; tx < x1; ++tx, rp += REC9) {
+          if (me >= cut[tx]) continue;
+          if (first) {
 #pragma unroll
-          for (int q = 0; q < REC9; ++q) r[q] = rp[q];
-          first = false;
-        } else {
+            for (int q = 0; q < REC9; ++q) r[q] = rp[q];
+            first = false;
+          } else {
 #pragma unroll
-          for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[q];
+            for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[q];
+          }
         }
       }
     }
+    // no pixel of this view reached the Gaussian: every term below is zero
+    if (first) continue;
     float pc[4];
     m4_mul_pos(V.view, pre.wx, pre.wy, pre.wz, pc);
     const float nx = -pc[0], ny = -pc[1], nz = -pc[2];

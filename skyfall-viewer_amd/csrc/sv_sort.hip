@@ -47,22 +47,31 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
   for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_h[0][0])[i] = 0u;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * SORT_TILE + (uint64_t)wave * (64 * SORT_ITEMS);
-  // lanes with equal digits are matched by ballots and counted with one
-  // LDS add by the lowest of them (high digit passes have few distinct
-  // digits, where per-lane atomics would serialise)
+  // A pass over at most 4 live key bits has at most 16 distinct digits:
+  // there, lanes with equal digits are matched by ballots and counted with
+  // one LDS add by the lowest of them (per-lane atomics would serialise on
+  // the few addresses); otherwise one LDS atomic per key.
+  if (m->key_bits <= shift + 4u) {
 #pragma unroll 4
-  for (int j = 0; j < SORT_ITEMS; ++j) {
-    const uint64_t i = base + (uint64_t)j * 64 + lane;
-    const bool valid = i < n;
-    const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & (RADIX - 1) : 0u;
-    uint64_t m = __ballot(valid);
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * 64 + lane;
+      const bool valid = i < n;
+      const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & (RADIX - 1) : 0u;
+      uint64_t mk = __ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < RB; ++bit) {
-      const bool on = (d >> bit) & 1u;
-      const uint64_t bb = __ballot(on);
-      m &= on ? bb : ~bb;
+      for (int bit = 0; bit < 4; ++bit) {
+        const bool on = (d >> bit) & 1u;
+        const uint64_t bb = __ballot(on);
+        mk &= on ? bb : ~bb;
+      }
+      if (valid && lane == __ffsll((unsigned long long)mk) - 1) atomicAdd(&s_h[wave][d], (uint32_t)__popcll(mk));
     }
-    if (valid && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&s_h[wave][d], (uint32_t)__popcll(m));
+  } else {
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; ++j) {
+      const uint64_t i = base + (uint64_t)j * 64 + lane;
+      if (i < n) atomicAdd(&s_h[wave][(uint32_t)(keys[i] >> shift) & (RADIX - 1)], 1u);
+    }
   }
   __syncthreads();
   for (int d = threadIdx.x; d < RADIX; d += SORT_T) {

@@ -817,6 +817,28 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+napi_value checkpoint(napi_env env, napi_callback_info info, sv_status (*fn)(sv_ctx *, const char *),
+                      const char *what) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  sv_ctx *c = ctx_of(env, a.v[0]);
+  if (!c) return nullptr;
+  char path[4096];
+  size_t len = 0;
+  if (napi_get_value_string_utf8(env, a.v[1], path, sizeof(path), &len) != napi_ok || len + 1 >= sizeof(path)) {
+    napi_throw_type_error(env, nullptr, "path must be a string");
+    return nullptr;
+  }
+  SV(c, fn(c, path), what);
+  return nullptr;
+}
+napi_value SaveCheckpoint(napi_env env, napi_callback_info info) {
+  return checkpoint(env, info, sv_save_checkpoint, "sv_save_checkpoint");
+}
+napi_value LoadCheckpoint(napi_env env, napi_callback_info info) {
+  return checkpoint(env, info, sv_load_checkpoint, "sv_load_checkpoint");
+}
+
 napi_value Sync(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 1)) return nullptr;
@@ -908,7 +930,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readAlpha", ReadAlpha},     {"readLastIds", ReadLastIds}, {"readGrads", ReadGrads},
              {"readKeys", ReadKeys},       {"pruneOpacity", PruneOpacity}, {"pruneMask", PruneMask},
              {"densify", Densify},         {"resetOpacity", ResetOpacity},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
-             {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},               {"sceneGaussians", SceneGaussians},
+             {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},
+             {"saveCheckpoint", SaveCheckpoint}, {"loadCheckpoint", LoadCheckpoint},               {"sceneGaussians", SceneGaussians},
              {"sceneJitter", SceneJitter}, {"sceneCamera", SceneCamera}};
   for (auto &f : fns) {
     napi_value v;

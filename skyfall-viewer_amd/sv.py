@@ -108,6 +108,8 @@ _SIGS = {
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(TuneEntry),
                               C.POINTER(C.c_int)]),
+    "sv_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "sv_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sv_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sv_timings": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.POINTER(C.c_int)]),
     "sv_reset_timings": (C.c_int, [C.c_void_p]),
@@ -116,6 +118,14 @@ _SIGS = {
     "sv_scene_jitter": (C.c_int, [C.c_uint64, C.c_size_t, C.c_float, _P]),
     "sv_scene_camera": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_float, C.POINTER(Camera)]),
     "sv_scene_uniform": (C.c_double, [C.c_uint64, C.c_uint64]),
+    "sv_ingest_error": (C.c_char_p, []),
+    "sv_ply_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_uint32)]),
+    "sv_ply_load": (C.c_int, [C.c_char_p, C.c_size_t, _P, _P, _P, _P, _P, _P]),
+    "sv_ply_save": (C.c_int, [C.c_char_p, C.c_size_t, _P, _P, _P, _P, _P, C.c_uint32]),
+    "sv_colmap_info": (C.c_int, [C.c_char_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "sv_colmap_load": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(Camera), C.c_char_p, C.c_size_t, C.c_size_t,
+                                 _P, _U8]),
+    "sv_points_to_gaussians": (C.c_int, [C.c_size_t, _P, _U8, _P, _P, _P, _P, _P, _P]),
     "sv_scene_normal": (C.c_double, [C.c_uint64, C.c_uint64]),
 }
 
@@ -173,6 +183,59 @@ def scene_gaussians(seed: int, n: int, half_extent: float = 1.0) -> dict:
     if st:
         raise SvError(st, "sv_scene_gaussians")
     return {"pos": pos, "logscale": ls, "quat": q, "logit": lg, "sh": sh}
+
+
+def _ing(st, what):
+    if st:
+        raise SvError(st, f"{what}: {lib().sv_ingest_error().decode()}")
+
+
+def _param_arrays(n):
+    return {"pos": np.zeros((n, 3), np.float32), "logscale": np.zeros((n, 3), np.float32),
+            "quat": np.zeros((n, 4), np.float32), "logit": np.zeros((n,), np.float32),
+            "sh": np.zeros((n, 48), np.float32)}
+
+
+def ply_load(path: str):
+    """(Gaussians dict, SH degree, scene extent) of a PLY file."""
+    n, deg = C.c_size_t(), C.c_uint32()
+    _ing(lib().sv_ply_info(os.fsencode(path), C.byref(n), C.byref(deg)), "ply_info")
+    g = _param_arrays(n.value)
+    ext = np.zeros(1, np.float32)
+    _ing(lib().sv_ply_load(os.fsencode(path), n.value, *[_fp(g[k]) for k in ("pos", "logscale", "quat", "logit", "sh")],
+                           _fp(ext)), "ply_load")
+    return g, deg.value, float(ext[0])
+
+
+def ply_save(path: str, g: dict, sh_deg: int = 3):
+    arrs = [_f32(g[k]) for k in ("pos", "logscale", "quat", "logit", "sh")]
+    _ing(lib().sv_ply_save(os.fsencode(path), arrs[0].shape[0], *[_fp(a) for a in arrs], sh_deg), "ply_save")
+
+
+def colmap_load(sparse_dir: str):
+    """(cameras, image names, xyz float32[n, 3], rgb uint8[n, 3]) of a COLMAP sparse model."""
+    ni, npt = C.c_size_t(), C.c_size_t()
+    _ing(lib().sv_colmap_info(os.fsencode(sparse_dir), C.byref(ni), C.byref(npt)), "colmap_info")
+    cams = (Camera * max(ni.value, 1))()
+    names = C.create_string_buffer(max(ni.value, 1) * 256)
+    xyz = np.zeros((npt.value, 3), np.float32)
+    rgb = np.zeros((npt.value, 3), np.uint8)
+    _ing(lib().sv_colmap_load(os.fsencode(sparse_dir), ni.value, cams, names, 256, npt.value, _fp(xyz),
+                              rgb.ctypes.data_as(_U8)), "colmap_load")
+    raw = names.raw
+    nm = [raw[i * 256:(i + 1) * 256].split(b"\0", 1)[0].decode() for i in range(ni.value)]
+    return list(cams)[:ni.value], nm, xyz, rgb
+
+
+def points_to_gaussians(xyz: np.ndarray, rgb: np.ndarray):
+    x = _f32(xyz)
+    c = np.ascontiguousarray(rgb, np.uint8)
+    g = _param_arrays(x.shape[0])
+    ext = np.zeros(1, np.float32)
+    _ing(lib().sv_points_to_gaussians(x.shape[0], _fp(x), c.ctypes.data_as(_U8),
+                                      *[_fp(g[k]) for k in ("pos", "logscale", "quat", "logit", "sh")], _fp(ext)),
+         "points_to_gaussians")
+    return g, float(ext[0])
 
 
 def scene_jitter(seed: int, pos: np.ndarray, sigma: float) -> np.ndarray:
@@ -442,6 +505,12 @@ class Context:
         n = C.c_size_t()
         self._ck(lib().sv_densify(self.h, a.ctypes.data_as(_U8), seed, C.byref(n)), "densify")
         return n.value
+
+    def save_checkpoint(self, path: str):
+        self._ck(lib().sv_save_checkpoint(self.h, os.fsencode(path)), "save_checkpoint")
+
+    def load_checkpoint(self, path: str):
+        self._ck(lib().sv_load_checkpoint(self.h, os.fsencode(path)), "load_checkpoint")
 
     def autotune(self, views, steps: int = 3) -> list:
         """Time every tuner candidate on `views`; the fastest stays configured."""
