@@ -105,6 +105,7 @@ struct sv_ctx {
   float4 *rec = nullptr;
   float2 *aux = nullptr;
   uint32_t *counts = nullptr, *offs = nullptr;
+  float *vrec = nullptr;  // 9 planes of slots x N: reached-record sums for the VJP
   size_t vg_cap = 0;
   uint64_t *scan_tmp = nullptr;
   size_t scan_tmp_cap = 0;
@@ -361,7 +362,9 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     dfree(c->aux);
     dfree(c->counts);
     dfree(c->offs);
+    dfree(c->vrec);
     c->vg_cap = 0;
+    if (sv_status s = dalloc(c, &c->vrec, (size_t)REC9 * M)) return s;
     if (sv_status s = dalloc(c, &c->rec, 4 * M)) return s;
     if (sv_status s = dalloc(c, &c->aux, M)) return s;
     if (sv_status s = dalloc(c, &c->counts, M)) return s;
@@ -624,6 +627,7 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.aux = c->aux;
   v.isect_rec = c->isect_rec;
   v.tile_cut = c->tile_cut;
+  v.vrec = c->vrec;
   v.importance = c->importance;
   v.G = c->G;
   v.geom = c->geom;
@@ -802,6 +806,7 @@ sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
     if (sv_status s = zero_grads(c)) return s;
     {
       Scoped t(c, PH_VJP);
+      launch_gather_records(c->st, va);
       launch_vjp(c->st, va);
     }
     if (sv_status s = check_launch(c, "vjp")) return s;
@@ -811,6 +816,7 @@ sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
   va.geom = nullptr;
   {
     Scoped t(c, PH_VJP);
+    launch_gather_records(c->st, va);
     launch_vjp(c->st, va);
   }
   return check_launch(c, "vjp+adam");
@@ -822,7 +828,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
   std::vector<uint64_t> g;
   const void *ptrs[] = {c->P, c->M1, c->M2, c->G, c->geom, c->importance, c->grad_accum, c->color,
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
-                        c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->scan_tmp, c->keys,
+                        c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
                         c->h_adam_dev};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
@@ -1104,6 +1110,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->aux);
   dfree(c->counts);
   dfree(c->offs);
+  dfree(c->vrec);
   dfree(c->scan_tmp);
   drop_graph(c);
   dfree(c->d_small);
@@ -1297,6 +1304,7 @@ sv_status sv_backward(sv_ctx *c, int v) {
   VjpArgs va = vjp_args(c);
   {
     Scoped t(c, PH_VJP);
+    launch_gather_records(c->st, va);
     launch_vjp(c->st, va);
   }
   if (sv_status s = check_launch(c, "vjp")) return s;
@@ -1366,6 +1374,7 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
       VjpArgs va = vjp_args(c);
       {
         Scoped t(c, PH_VJP);
+        launch_gather_records(c->st, va);
         launch_vjp(c->st, va);
       }
       if ((s = check_launch(c, "vjp"))) return s;

@@ -106,6 +106,7 @@ struct VjpArgs {
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
   const float *isect_rec;  // 9-float adjoint record per intersection
   const uint64_t *tile_cut; // per global tile: records at or past this (depth_q, id) were not written
+  float *vrec;             // 9 planes of nv * N: per (slot, Gaussian) sum of its reached records
   float4 *importance;      // importance statistics (may be null)
   float *G;                // gradient planes (accumulate), or null when fused
   float *geom;             // 5 planes: grad_geom of the last slot (may be null)
@@ -116,6 +117,10 @@ struct VjpArgs {
   float *M1, *M2;
   const AdamParams *adam;
 };
+// Per (slot, Gaussian): sum of the adjoint records of its reached pairs
+// into vrec (one thread per item, so the dependent record walks of all
+// items overlap); then the per-Gaussian VJP reads 9 coalesced planes.
+void launch_gather_records(hipStream_t st, const VjpArgs &a);
 void launch_vjp(hipStream_t st, const VjpArgs &a);
 void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G, size_t cap,
                  uint32_t N, const AdamParams *adam);

@@ -229,10 +229,58 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 // path; every Gaussian is updated, as the reference's Adam dispatches
 // cover all active elements).
 // ---------------------------------------------------------------------
+// Sum of the adjoint records of the reached pairs of one (slot, Gaussian),
+// in emit order (ty-major over its tile rectangle): a pair at or past its
+// tile's cut (depth_q, id) was reached by no pixel and has no record.
+__global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
+  if (a.meta && a.meta->overflow) return;
+  const size_t M = (size_t)a.nv * a.N;
+  const size_t vg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (vg >= M) return;
+  float r[REC9];
+#pragma unroll
+  for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
+  const uint32_t cnt = a.counts[vg];
+  if (cnt) {
+    const uint32_t s = (uint32_t)(vg / a.N), g = (uint32_t)(vg - (size_t)s * a.N);
+    const DevView &V = a.views[s];
+    const float4 q3 = a.rec[vg * 4 + 3];
+    const uint32_t ra = __float_as_uint(q3.y), rb = __float_as_uint(q3.z);
+    const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
+    const uint32_t dq = f2u_sat(clampf(a.aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
+    const uint64_t me = ((uint64_t)dq << 32) | g;
+    const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
+    bool first = true;
+    for (uint32_t ty = y0; ty < y1; ++ty) {
+      const uint64_t *cut = a.tile_cut + V.tile_base + ty * V.tiles_x;
+      for (uint32_t tx = x0; tx < x1; ++tx, rp += REC9) {
+        if (me >= cut[tx]) continue;
+        if (first) {
+#pragma unroll
+          for (int q = 0; q < REC9; ++q) r[q] = rp[q];
+          first = false;
+        } else {
+#pragma unroll
+          for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[q];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < REC9; ++q) a.vrec[(size_t)q * M + vg] = r[q];
+}
+
+void launch_gather_records(hipStream_t st, const VjpArgs &a) {
+  const size_t M = (size_t)a.nv * a.N;
+  if (M == 0) return;
+  hipLaunchKernelGGL(k_gather_records, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, a);
+}
+
 __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
   if (a.meta && a.meta->overflow) return;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.N) return;
+  const size_t M = (size_t)a.nv * a.N;
   const uint32_t gl = g;
   float prm[11];
 #pragma unroll
@@ -266,36 +314,16 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     imp.z = fmaxf(imp.z, ax.y);
     any = true;
     const DevView &V = a.views[s];
-    // sum of the records of the tiles' entries some pixel reached, in emit
-    // order (ty-major rectangle); a tile's later entries have no record
+    // sum of the Gaussian's reached records of this view (k_gather_records);
+    // all zero: no pixel reached it, and every term below is zero
     float r[REC9];
+    bool reached = false;
 #pragma unroll
-    for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
-    bool first = true;
-    {
-      const float4 q3 = a.rec[vg * 4 + 3];
-      const uint32_t ra = __float_as_uint(q3.y), rb = __float_as_uint(q3.z);
-      const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
-      const uint32_t dq = f2u_sat(clampf(ax.x, 0.0f, 65535.0f)) & 0xffffu;
-      const uint64_t me = ((uint64_t)dq << 32) | g;
-      const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
-      for (uint32_t ty = y0; ty < y1; ++ty) {
-        const uint64_t *cut = a.tile_cut + V.tile_base + ty * V.tiles_x;
-        for (uint32_t tx = x0; tx < x1; ++tx, rp += REC9) {
-          if (me >= cut[tx]) continue;
-          if (first) {
-#pragma unroll
-            for (int q = 0; q < REC9; ++q) r[q] = rp[q];
-            first = false;
-          } else {
-#pragma unroll
-            for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[q];
-          }
-        }
-      }
+    for (int q = 0; q < REC9; ++q) {
+      r[q] = a.vrec[(size_t)q * M + vg];
+      reached = reached || r[q] != 0.0f;
     }
-    // no pixel of this view reached the Gaussian: every term below is zero
-    if (first) continue;
+    if (!reached) continue;
     float pc[4];
     m4_mul_pos(V.view, pre.wx, pre.wy, pre.wz, pc);
     const float nx = -pc[0], ny = -pc[1], nz = -pc[2];

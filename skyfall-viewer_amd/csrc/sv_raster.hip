@@ -135,13 +135,20 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       q0 = q[0];
       q1 = q[1];
       q2 = q[2];
-      if (s_p) {
-        q3 = q[3];
-        s_p[k] = emit_pos(q3, tx, ty);
-      }
       const float ddx = fabsf(q0.x - fminf(fmaxf(q0.x, tx0), tx1));
       const float ddy = fabsf(q0.y - fminf(fmaxf(q0.y, ty0), ty1));
       touch = ddx <= q2.z && ddy <= q2.w;
+      if (s_p) {
+        q3 = q[3];
+        const uint32_t p = emit_pos(q3, tx, ty);
+        s_p[k] = p;
+        // an entry that touches no pixel of the tile contributes nothing
+        if (!touch && (uint64_t)p < a.n_isect) {
+          float *o = a.isect_rec + (size_t)p * REC9;
+#pragma unroll
+          for (int q9 = 0; q9 < REC9; ++q9) o[q9] = 0.0f;
+        }
+      }
     }
     const uint64_t m = __ballot(touch);
     if (touch) {
@@ -290,7 +297,6 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
   __shared__ float4 s_q2[B];
   __shared__ uint32_t s_k[B];
   __shared__ uint32_t s_p[B];
-  __shared__ float s_out[B * 4 * REC9];
   __shared__ uint32_t s_tg;
   const int lane = threadIdx.x;
   uint32_t tg = first_tile(a, &s_tg);
@@ -366,7 +372,6 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        for (uint32_t z = lane; z < cnt * 4 * REC9; z += WT) s_out[z] = 0.0f;
         const uint32_t kept =
             load_batch<B>(a, vbase, lo, cnt, tx0, tx1, ty0, ty1, s_q0, s_q1, s_q2, s_k, s_p, t.tx, t.ty);
         __syncthreads();
@@ -415,19 +420,23 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
               T[j] = Tb;
             }
           }
-          if (!__any(hit)) continue;
-          v0 = row_total(v0);
-          v1 = row_total(v1);
-          v2 = row_total(v2);
-          v3 = row_total(v3);
-          v4 = row_total(v4);
-          v5 = row_total(v5);
-          v6 = row_total(v6);
-          v7 = row_total(v7);
-          v8 = row_total(v8);
-          if ((lane & 15) == 15) {
-            // row partials of entry k: s_out[(k * 4 + row) * 9 + q]
-            float *o = &s_out[(k * 4 + (lane >> 4)) * REC9];
+          // the entry's record at its emit position, written by lane 63: the
+          // sums over the tile's 256 pixels (fixed DPP tree), or zeros
+          const uint32_t p = s_p[k];
+          const bool any_hit = __any(hit);
+          if (any_hit) {
+            v0 = wave_total63(v0);
+            v1 = wave_total63(v1);
+            v2 = wave_total63(v2);
+            v3 = wave_total63(v3);
+            v4 = wave_total63(v4);
+            v5 = wave_total63(v5);
+            v6 = wave_total63(v6);
+            v7 = wave_total63(v7);
+            v8 = wave_total63(v8);
+          }
+          if (lane == 63 && (uint64_t)p < a.n_isect) {
+            float *o = a.isect_rec + (size_t)p * REC9;
             o[0] = v0;
             o[1] = v1;
             o[2] = v2;
@@ -438,14 +447,6 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
             o[7] = v7;
             o[8] = v8;
           }
-        }
-        __syncthreads();
-        for (uint32_t z = lane; z < cnt * REC9; z += WT) {
-          const uint32_t k = z / REC9, q9 = z - k * REC9;
-          const uint32_t p = s_p[k];
-          const float *o = &s_out[k * 4 * REC9 + q9];
-          const float tot = ((o[3 * REC9] + o[2 * REC9]) + (o[REC9] + o[0]));
-          if ((uint64_t)p < a.n_isect) a.isect_rec[(size_t)p * REC9 + q9] = tot;
         }
         __syncthreads();
         hi = lo;
