@@ -1,8 +1,10 @@
 // sv_raster.hip -- tiled forward rasterizer and adjoint backward for gfx950.
 //
-// One wave64 owns one 16x16 pixel tile of one view: lane l shades the
-// pixels (x = l & 15, y = (l >> 4) + 4 j), j = 0..3, so one wave-wide
-// reduction in the backward covers all 256 pixels of the tile. The tile's
+// One wave64 owns one 16x16 pixel tile of one view. The tile is four 8x8
+// quadrants; lane l shades pixel (l & 7, l >> 3) of every quadrant j =
+// 0..3 (slot j of its 4 pixels), so one wave-wide reduction in the
+// backward covers all 256 pixels, and a splat that misses a quadrant costs
+// nothing there: the quadrant test is uniform over the wave. The tile's
 // sorted splat list is staged through LDS in batches of B 64-byte records.
 // Work items are (view, global tile); with a work counter the grid is
 // persistent and pulls items from a device queue.
@@ -89,7 +91,7 @@ __device__ __forceinline__ float sign_ref(float x) {
 }
 
 struct TileGeom {
-  uint32_t s, tx, ty, px, py0;
+  uint32_t s, tx, ty;
   const DevView *V;
 };
 
@@ -100,9 +102,15 @@ __device__ __forceinline__ TileGeom tile_geom(const RasterArgs &a, uint32_t tg) 
   const uint32_t lt = tg - t.V->tile_base;
   t.tx = lt % t.V->tiles_x;
   t.ty = lt / t.V->tiles_x;
-  t.px = t.tx * 16u + (threadIdx.x & 15u);
-  t.py0 = t.ty * 16u + (threadIdx.x >> 4);
   return t;
+}
+
+// Pixel of slot j (quadrant j) of this lane.
+__device__ __forceinline__ uint32_t pix_x(const TileGeom &t, int j) {
+  return t.tx * 16u + 8u * (uint32_t)(j & 1) + (threadIdx.x & 7u);
+}
+__device__ __forceinline__ uint32_t pix_y(const TileGeom &t, int j) {
+  return t.ty * 16u + 8u * (uint32_t)(j >> 1) + ((threadIdx.x & 63u) >> 3);
 }
 
 // Position of an intersection in emit order: the Gaussian's first
@@ -116,11 +124,32 @@ __device__ __forceinline__ uint32_t emit_pos(const float4 &q3, uint32_t tx, uint
 
 // Load up to B entries of a tile list from `first` (count cnt) into LDS,
 // keeping only those whose power <= power_max region (axis-aligned bounds
-// ext_x, ext_y around the mean) overlaps the tile's pixel centres; kept
-// entries stay in list order. Returns how many were kept.
+// ext_x, ext_y around the mean) overlaps the pixel centres of some quadrant
+// of the tile; kept entries stay in list order. s_k holds the entry's index
+// in the batch (bits 0..15) and the mask of quadrants it overlaps (bits
+// 16..19). Returns how many were kept.
+// Pixel-centre bounds of the quadrants of a tile: x of quadrant columns
+// 0/1, y of quadrant rows 0/1 (an empty range has lo > hi).
+struct QuadBounds {
+  float x0[2], x1[2], y0[2], y1[2];
+};
+
+__device__ __forceinline__ QuadBounds quad_bounds(const TileGeom &t, const DevView &V) {
+  QuadBounds b;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t xs = t.tx * 16u + 8u * h, ys = t.ty * 16u + 8u * h;
+    b.x0[h] = (float)xs + 0.5f;
+    b.x1[h] = xs < V.W ? (float)min(xs + 7u, V.W - 1u) + 0.5f : -1.0f;
+    b.y0[h] = (float)ys + 0.5f;
+    b.y1[h] = ys < V.H ? (float)min(ys + 7u, V.H - 1u) + 0.5f : -1.0f;
+  }
+  return b;
+}
+
 template <int B>
 __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
-                                               uint32_t cnt, float tx0, float tx1, float ty0, float ty1,
+                                               uint32_t cnt, const QuadBounds &qb,
                                                float4 *s_q0, float4 *s_q1, float4 *s_q2, uint32_t *s_k,
                                                uint32_t *s_p, uint32_t tx, uint32_t ty) {
   const int lane = threadIdx.x;
@@ -129,15 +158,22 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
   for (uint32_t r = 0; r < cnt; r += WT) {
     const uint32_t k = r + lane;
     bool touch = false;
+    uint32_t qm = 0;
     float4 q0, q1, q2, q3;
     if (k < cnt) {
       const float4 *q = a.rec + (vbase + min(a.vals[first + k], a.N - 1u)) * 4;
       q0 = q[0];
       q1 = q[1];
       q2 = q[2];
-      const float ddx = fabsf(q0.x - fminf(fmaxf(q0.x, tx0), tx1));
-      const float ddy = fabsf(q0.y - fminf(fmaxf(q0.y, ty0), ty1));
-      touch = ddx <= q2.z && ddy <= q2.w;
+      bool cx[2], cy[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        cx[h] = qb.x1[h] >= qb.x0[h] && fabsf(q0.x - fminf(fmaxf(q0.x, qb.x0[h]), qb.x1[h])) <= q2.z;
+        cy[h] = qb.y1[h] >= qb.y0[h] && fabsf(q0.y - fminf(fmaxf(q0.y, qb.y0[h]), qb.y1[h])) <= q2.w;
+      }
+      qm = (uint32_t)(cx[0] && cy[0]) | ((uint32_t)(cx[1] && cy[0]) << 1) | ((uint32_t)(cx[0] && cy[1]) << 2) |
+           ((uint32_t)(cx[1] && cy[1]) << 3);
+      touch = qm != 0;
       if (s_p) {
         q3 = q[3];
         const uint32_t p = emit_pos(q3, tx, ty);
@@ -156,7 +192,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       s_q0[pos] = q0;
       s_q1[pos] = q1;
       s_q2[pos] = q2;
-      s_k[pos] = k;
+      s_k[pos] = k | (qm << 16);
     }
     kept += (uint32_t)__popcll(m);
   }
@@ -188,13 +224,12 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
     const size_t P = (size_t)V.W * V.H;
     float *col = a.color + 3 * V.pix_off;
     const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
-    const bool xin = t.px < V.W;
     if (start >= end) {
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
-        const uint32_t py = t.py0 + 4u * j;
-        if (xin && py < V.H) {
-          const size_t lp = (size_t)py * V.W + t.px;
+        const uint32_t px = pix_x(t, j), py = pix_y(t, j);
+        if (px < V.W && py < V.H) {
+          const size_t lp = (size_t)py * V.W + px;
           col[lp] = a.cfg.bg[0];
           col[P + lp] = a.cfg.bg[1];
           col[2 * P + lp] = a.cfg.bg[2];
@@ -203,51 +238,52 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
         }
       }
     } else {
-      const float fx = (float)t.px + 0.5f;
       const size_t vbase = (size_t)t.s * a.N;
-      const float tx0 = (float)(t.tx * 16u) + 0.5f, tx1 = (float)min(t.tx * 16u + 15u, V.W - 1u) + 0.5f;
-      const float ty0 = (float)(t.ty * 16u) + 0.5f, ty1 = (float)min(t.ty * 16u + 15u, V.H - 1u) + 0.5f;
-      float c0[PPL], c1[PPL], c2[PPL], A[PPL], fy[PPL];
+      const QuadBounds qb = quad_bounds(t, V);
+      float c0[PPL], c1[PPL], c2[PPL], A[PPL], fxh[2], fyh[2];
       uint32_t last[PPL];
       bool done[PPL];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        fxh[h] = (float)pix_x(t, h) + 0.5f;
+        fyh[h] = (float)pix_y(t, 2 * h) + 0.5f;
+      }
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
         c0[j] = c1[j] = c2[j] = A[j] = 0.0f;
         last[j] = end - 1u;
-        const uint32_t py = t.py0 + 4u * j;
-        fy[j] = (float)py + 0.5f;
-        done[j] = !(xin && py < V.H);
+        done[j] = !(pix_x(t, j) < V.W && pix_y(t, j) < V.H);
       }
       for (uint32_t b0 = start; b0 < end; b0 += B) {
         const uint32_t cnt = min((uint32_t)B, end - b0);
-        const uint32_t kept =
-            load_batch<B>(a, vbase, b0, cnt, tx0, tx1, ty0, ty1, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0);
+        const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0);
         __syncthreads();
         for (uint32_t e = 0; e < kept; ++e) {
+          // same word on every lane: the quadrant tests below are uniform
+          const uint32_t ke = __builtin_amdgcn_readfirstlane(s_k[e]);
           const float4 x0 = s_q0[e];
           const float4 x1 = s_q1[e];
           const float4 x2 = s_q2[e];
-          const float dx = fx - x0.x;
-          if (fabsf(dx) <= x2.z) {
 #pragma unroll
-            for (int j = 0; j < PPL; ++j) {
-              if (done[j]) continue;
-              const float dy = fy[j] - x0.y;
-              if (fabsf(dy) > x2.w) continue;
-              const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-              if (sg < 0.0f || sg > x2.y) continue;
-              const float gauss = sv_expf_neg(-sg);
-              const float as = fminf(x1.y * gauss, SV_MAX_ALPHA);
-              if (as < SV_MIN_ALPHA) continue;
-              const float contrib = as * (1.0f - A[j]);
-              c0[j] = c0[j] + x1.z * contrib;
-              c1[j] = c1[j] + x1.w * contrib;
-              c2[j] = c2[j] + x2.x * contrib;
-              A[j] = A[j] + contrib;
-              if (1.0f - A[j] < SV_T_EPS) {
-                done[j] = true;
-                last[j] = b0 + s_k[e];
-              }
+          for (int j = 0; j < PPL; ++j) {
+            if (!((ke >> (16 + j)) & 1u)) continue;
+            if (done[j]) continue;
+            const float dx = fxh[j & 1] - x0.x;
+            const float dy = fyh[j >> 1] - x0.y;
+            if (fabsf(dx) > x2.z || fabsf(dy) > x2.w) continue;
+            const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+            if (sg < 0.0f || sg > x2.y) continue;
+            const float gauss = sv_expf_neg(-sg);
+            const float as = fminf(x1.y * gauss, SV_MAX_ALPHA);
+            if (as < SV_MIN_ALPHA) continue;
+            const float contrib = as * (1.0f - A[j]);
+            c0[j] = c0[j] + x1.z * contrib;
+            c1[j] = c1[j] + x1.w * contrib;
+            c2[j] = c2[j] + x2.x * contrib;
+            A[j] = A[j] + contrib;
+            if (1.0f - A[j] < SV_T_EPS) {
+              done[j] = true;
+              last[j] = b0 + (ke & 0xFFFFu);
             }
           }
           if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
@@ -256,9 +292,9 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
       }
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
-        const uint32_t py = t.py0 + 4u * j;
-        if (xin && py < V.H) {
-          const size_t lp = (size_t)py * V.W + t.px;
+        const uint32_t px = pix_x(t, j), py = pix_y(t, j);
+        if (px < V.W && py < V.H) {
+          const size_t lp = (size_t)py * V.W + px;
           const float T = 1.0f - A[j];
           col[lp] = c0[j] + T * a.cfg.bg[0];
           col[P + lp] = c1[j] + T * a.cfg.bg[1];
@@ -306,9 +342,8 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
     const size_t P = (size_t)V.W * V.H;
     const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
     const size_t vbase = (size_t)t.s * a.N;
-    const bool xin = t.px < V.W;
     if (start < end) {
-      float r0[PPL], r1[PPL], r2[PPL], T[PPL], acc0[PPL], acc1[PPL], acc2[PPL], fy[PPL];
+      float r0[PPL], r1[PPL], r2[PPL], T[PPL], acc0[PPL], acc1[PPL], acc2[PPL], fxh[2], fyh[2];
       uint32_t lastp[PPL];
       bool valid[PPL];
       float l1 = 0.0f;
@@ -316,9 +351,13 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
       const float *colv = a.color + 3 * V.pix_off;
       const float *tgt = a.target + 3 * V.pix_off;
 #pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        fxh[h] = (float)pix_x(t, h) + 0.5f;
+        fyh[h] = (float)pix_y(t, 2 * h) + 0.5f;
+      }
+#pragma unroll
       for (int j = 0; j < PPL; ++j) {
-        const uint32_t py = t.py0 + 4u * j;
-        fy[j] = (float)py + 0.5f;
+        const uint32_t px = pix_x(t, j), py = pix_y(t, j);
         r0[j] = r1[j] = r2[j] = 0.0f;
         T[j] = 1.0f;
         lastp[j] = 0;
@@ -326,8 +365,8 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
         acc0[j] = a.cfg.bg[0];
         acc1[j] = a.cfg.bg[1];
         acc2[j] = a.cfg.bg[2];
-        if (xin && py < V.H) {
-          const size_t lp = (size_t)py * V.W + t.px;
+        if (px < V.W && py < V.H) {
+          const size_t lp = (size_t)py * V.W + px;
           lastp[j] = a.last[V.pix_off + lp];
           T[j] = 1.0f - a.alpha[V.pix_off + lp];
           const float d0 = colv[lp] - tgt[lp];
@@ -366,30 +405,30 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
         }
         a.tile_cut[tg] = cut;
       }
-      const float fx = (float)t.px + 0.5f;
-      const float tx0 = (float)(t.tx * 16u) + 0.5f, tx1 = (float)min(t.tx * 16u + 15u, V.W - 1u) + 0.5f;
-      const float ty0 = (float)(t.ty * 16u) + 0.5f, ty1 = (float)min(t.ty * 16u + 15u, V.H - 1u) + 0.5f;
+      const QuadBounds qb = quad_bounds(t, V);
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        const uint32_t kept =
-            load_batch<B>(a, vbase, lo, cnt, tx0, tx1, ty0, ty1, s_q0, s_q1, s_q2, s_k, s_p, t.tx, t.ty);
+        const uint32_t kept = load_batch<B>(a, vbase, lo, cnt, qb, s_q0, s_q1, s_q2, s_k, s_p, t.tx, t.ty);
         __syncthreads();
         for (int e = (int)kept - 1; e >= 0; --e) {
-          const uint32_t k = s_k[e];
+          // same word on every lane: the quadrant tests below are uniform
+          const uint32_t ke = __builtin_amdgcn_readfirstlane(s_k[e]);
+          const uint32_t k = ke & 0xFFFFu;
           const uint32_t i = lo + k;
           const float4 x0 = s_q0[e];
           const float4 x1 = s_q1[e];
           const float4 x2 = s_q2[e];
-          const float dx = fx - x0.x;
           float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
           bool hit = false;
-          if (fabsf(dx) <= x2.z) {
+          {
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
+              if (!((ke >> (16 + j)) & 1u)) continue;
               if (!(valid[j] && i <= lastp[j])) continue;
-              const float dy = fy[j] - x0.y;
-              if (fabsf(dy) > x2.w) continue;
+              const float dx = fxh[j & 1] - x0.x;
+              const float dy = fyh[j >> 1] - x0.y;
+              if (fabsf(dx) > x2.z || fabsf(dy) > x2.w) continue;
               const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
               if (sg < 0.0f || sg > x2.y) continue;
               const float gauss = sv_expf_neg(-sg);
