@@ -80,6 +80,45 @@ __device__ __forceinline__ float wave_total63(float v) {
   return v;
 }
 
+// Butterfly levels of a reduce-scatter over the wave. Each takes two
+// per-lane values x, y and returns, in the half of the lanes whose level
+// bit is 0, x summed with its partner lane's x, and in the other half y
+// summed with its partner's y.
+// Level 32: v_permlane32_swap exchanges the upper half of x with the lower
+// half of y.
+__device__ __forceinline__ float pair_sum32(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// Level 16: v_permlane16_swap exchanges the odd rows of x with the even
+// rows of y.
+__device__ __forceinline__ float pair_sum16(float x, float y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+// Level 8: row_ror:8 is lane ^ 8 within a row.
+__device__ __forceinline__ float pair_sum8(float x, float y) {
+  const bool hi = (threadIdx.x & 8u) != 0;
+  const float mine = hi ? y : x, other = hi ? x : y;
+  return mine + dpp<0x128>(other);
+}
+
+// Sums of eight per-lane values over the 64 lanes: lane l receives the sum
+// of value l >> 3 (the eight lanes of a group hold the same bits). Three
+// butterfly levels, then a sum over the group (quad_perm, quad_perm,
+// row_half_mirror); fixed association, deterministic.
+__device__ __forceinline__ float wave_scatter8(float v0, float v1, float v2, float v3, float v4, float v5,
+                                               float v6, float v7) {
+  const float a0 = pair_sum32(v0, v4), a1 = pair_sum32(v1, v5);
+  const float a2 = pair_sum32(v2, v6), a3 = pair_sum32(v3, v7);
+  const float b0 = pair_sum16(a0, a2), b1 = pair_sum16(a1, a3);
+  float c = pair_sum8(b0, b1);
+  c = c + dpp<0xB1>(c);   // quad_perm [1,0,3,2]
+  c = c + dpp<0x4E>(c);   // quad_perm [2,3,0,1]
+  c = c + dpp<0x141>(c);  // row_half_mirror
+  return c;
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t m) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
@@ -317,7 +356,9 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
 // atomics into 54 per-Gaussian slots, the wave sums the 9 screen-space
 // quantities of one splat (sum of res * alpha * T_before per channel,
 // dL/dlogit, dL/dconic abc, dL/dmean xy) over the tile's 256 pixels (in
-// registers over each lane's 4 pixels, then a fixed DPP tree) and writes
+// registers over each lane's 4 pixels, then a fixed butterfly
+// reduce-scatter over the wave for eight of them and a DPP tree for the
+// ninth) and writes
 // one 36-byte record at the intersection's emit position. The SH chain is
 // applied per Gaussian afterwards (exact: the SH basis depends only on the
 // Gaussian and the view). Deterministic.
@@ -459,33 +500,18 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
               T[j] = Tb;
             }
           }
-          // the entry's record at its emit position, written by lane 63: the
-          // sums over the tile's 256 pixels (fixed DPP tree), or zeros
+          // the entry's record at its emit position: the sums over the
+          // tile's 256 pixels, or zeros. Components 0..7 end in lanes 8c,
+          // component 8 in lane 63; one store of nine lanes.
           const uint32_t p = s_p[k];
-          const bool any_hit = __any(hit);
-          if (any_hit) {
-            v0 = wave_total63(v0);
-            v1 = wave_total63(v1);
-            v2 = wave_total63(v2);
-            v3 = wave_total63(v3);
-            v4 = wave_total63(v4);
-            v5 = wave_total63(v5);
-            v6 = wave_total63(v6);
-            v7 = wave_total63(v7);
-            v8 = wave_total63(v8);
+          float out = 0.0f;
+          if (__any(hit)) {
+            const float s07 = wave_scatter8(v0, v1, v2, v3, v4, v5, v6, v7);
+            const float s8 = wave_total63(v8);
+            out = lane == 63 ? s8 : s07;
           }
-          if (lane == 63 && (uint64_t)p < a.n_isect) {
-            float *o = a.isect_rec + (size_t)p * REC9;
-            o[0] = v0;
-            o[1] = v1;
-            o[2] = v2;
-            o[3] = v3;
-            o[4] = v4;
-            o[5] = v5;
-            o[6] = v6;
-            o[7] = v7;
-            o[8] = v8;
-          }
+          if (((lane & 7) == 0 || lane == 63) && (uint64_t)p < a.n_isect)
+            a.isect_rec[(size_t)p * REC9 + (lane == 63 ? 8u : (uint32_t)lane >> 3)] = out;
         }
         __syncthreads();
         hi = lo;
