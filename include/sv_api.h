@@ -245,6 +245,16 @@ sv_status sv_prune_opacity(sv_ctx *ctx, float min_opacity, size_t *n_removed);
 /* resetOpacity (gaussian-renderer.ts:3867-3920): logits above
  * logit(max_opacity) are set to it, on the device; moments are kept. */
 sv_status sv_reset_opacity(sv_ctx *ctx, float max_opacity, size_t *n_clamped);
+/* initializeKnn (gaussian-renderer.ts:2131-2146, the knn_init shader
+ * knn-init.wgsl:40-90, constants :77-79): for every Gaussian the mean of the
+ * distances to its three nearest other Gaussians (positions rounded to f16;
+ * 1e10 stands in for a missing neighbour), sigma = min(clamp(max(mean / 3,
+ * 0.0015 e), 0.0007 e, 0.007 e), 0.1) with e = scene_extent, and all three
+ * log-scales set to f16(log(max(sigma, 1e-6))); on the device, by an exact
+ * grid search instead of the shader's all-pairs scan (same results). Moments
+ * are kept (initializeKnnAndSync, :2159-2172, runs before training).
+ * density: count floats (the shader's density_buffer) or NULL. */
+sv_status sv_knn_init(sv_ctx *ctx, float scene_extent, float *density);
 /* performDensification (gaussian-renderer.ts:3211-3466) with a seeded RNG:
  * actions[i] = 0 none, 1 clone, 2 split. Output order: non-split originals
  * in order, then children in parent order (clone: 1, split: 2); children

@@ -284,6 +284,16 @@ def prune_keep(logit: np.ndarray, min_opacity: float) -> np.ndarray:
     return keep
 
 
+def knn_init(pos: np.ndarray, extent: float):
+    """(density, logscale) per point; pos (n, 3)."""
+    p = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    n = p.shape[0]
+    dens = np.zeros(n, np.float32)
+    ls = np.zeros(n, np.float32)
+    lib().orc_knn_init(C.c_size_t(n), _fp(p), C.c_float(extent), _fp(dens), _fp(ls))
+    return dens, ls
+
+
 def densify(g: dict, m1: dict, m2: dict, actions: np.ndarray, seed: int):
     """Returns (params, m1, m2) dicts after clone / split."""
     r, a1, a2 = to_rows(g), to_rows(m1), to_rows(m2)

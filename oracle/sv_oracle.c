@@ -36,6 +36,9 @@
  *                       -lr*g/(|g|+eps)
  *   orc_accumulate_*    closed forms
  *   orc_compact_rows    numpy boolean indexing
+ *   orc_knn_init        scipy cKDTree 3-NN distances in float64 and the
+ *                       sigma rule in numpy; special cases n <= 3,
+ *                       duplicate points
  *   orc_prune_keep, orc_densify, orc_tile_l1, orc_densify_decide
  *                       parity unpinned against the reference (its rule is
  *                       weighted random, its draws Math.random); pinned by
@@ -1435,4 +1438,48 @@ void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
     }
     actions[i] = !cand ? 0u : (rmax > split_px ? 2u : 1u);
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* knn_init: knn-init.wgsl:40-90                                        */
+/* ------------------------------------------------------------------ */
+static void knn_insert_sorted(float d[3], float v) { /* insert_sorted, :27-39 */
+  if (v < d[0]) {
+    d[2] = d[1];
+    d[1] = d[0];
+    d[0] = v;
+  } else if (v < d[1]) {
+    d[2] = d[1];
+    d[1] = v;
+  } else if (v < d[2]) {
+    d[2] = v;
+  }
+}
+
+void orc_knn_init(size_t n, const float *pos, float extent, float *density, float *logscale) {
+  const float min_ratio = 0.0007f, max_ratio = 0.007f, bias_ratio = 0.0015f;
+  float *p = (float *)malloc((n ? n : 1) * 3 * sizeof(float));
+  for (size_t i = 0; i < 3 * n; ++i) p[i] = orc_f16_round(pos[i]); /* read_position, :20-25 */
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long ii = 0; ii < (long long)n; ++ii) {
+    const size_t idx = (size_t)ii;
+    float d[3] = {1e10f, 1e10f, 1e10f};
+    const float *me = p + 3 * idx;
+    for (size_t i = 0; i < n; ++i) {
+      if (i == idx) continue;
+      const float *o = p + 3 * i;
+      const float dx = me[0] - o[0], dy = me[1] - o[1], dz = me[2] - o[2];
+      knn_insert_sorted(d, sqrtf((dx * dx + dy * dy) + dz * dz)); /* length(my_pos - other_pos) */
+    }
+    const float mean_dist = (d[0] + d[1] + d[2]) / 3.0f;
+    density[idx] = mean_dist;
+    const float base_sigma = mean_dist / 3.0f;
+    const float min_sigma = extent * min_ratio, max_sigma = extent * max_ratio;
+    const float bias_sigma = extent * bias_ratio;
+    float sigma = fmaxf(base_sigma, bias_sigma);
+    sigma = fminf(fmaxf(sigma, min_sigma), max_sigma); /* clamp */
+    sigma = fminf(sigma, 0.1f);                        /* MAX_INITIAL_SIGMA */
+    logscale[idx] = orc_f16_round((float)log((double)fmaxf(sigma, 1e-6f)));
+  }
+  free(p);
 }

@@ -187,6 +187,15 @@ void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
                         const orc_view *views, const float *const *tile_l1,
                         const float *thr, float split_px, uint8_t *actions);
 
+/* knn_init (knn-init.wgsl:40-90; constants gaussian-renderer.ts:77-79):
+ * for each of the n points (positions xyz interleaved, rounded to f16 as
+ * read_position does) the three smallest distances to the other points
+ * (self excluded by index, duplicates count; 1e10 when fewer than three),
+ * density = (d0 + d1 + d2) / 3, sigma = min(clamp(max(density / 3, 0.0015
+ * e), 0.0007 e, 0.007 e), 0.1) with e = extent, logscale = f16(log(max(sigma,
+ * 1e-6))). O(n^2), as the shader. density / logscale: n floats each. */
+void orc_knn_init(size_t n, const float *pos, float extent, float *density, float *logscale);
+
 /* Vector helpers for the tests. */
 void orc_expf_v(size_t n, const float *x, float *y);
 void orc_f16_round_v(size_t n, const float *x, float *y);

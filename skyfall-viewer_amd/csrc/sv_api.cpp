@@ -527,9 +527,7 @@ RasterArgs raster_args(sv_ctx *c) {
   a.residual = nullptr;
   a.offs = c->offs;
   a.isect_rec = c->isect_rec;
-  a.keys0 = c->keys;
-  a.keys1 = c->keys2;
-  a.key64 = c->key64 ? 1u : 0u;
+  a.aux = c->aux;
   a.tile_cut = c->tile_cut;
   a.l1_partial = c->l1_partial;
   a.work_counter = nullptr;
@@ -1649,6 +1647,51 @@ sv_status sv_reset_opacity(sv_ctx *c, float max_opacity, size_t *n_clamped) {
   HIP_OK(c, hipMemcpyAsync(c->h_small, c->d_small, sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
   if (sv_status s = sync(c)) return s;
   if (n_clamped) *n_clamped = (size_t)(c->h_small[0] & 0xffffffffu);
+  c->built = false;
+  c->forward_done = false;
+  return SV_OK;
+}
+
+sv_status sv_knn_init(sv_ctx *c, float scene_extent, float *density) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!(scene_extent >= 0.0f) || !std::isfinite(scene_extent))
+    return fail(c, SV_E_ARG, "scene_extent must be finite and >= 0");
+  const uint32_t N = c->N;
+  if (N == 0) return SV_OK;
+  const size_t cells = knn_grid_cells(N);
+  KnnScratch w{};
+  uint8_t *grid = nullptr;
+  auto release = [&]() {
+    dfree(grid);
+    dfree(w.cnt);
+    dfree(w.start);
+    dfree(w.cell);
+    dfree(w.rank);
+    dfree(w.spos);
+    dfree(w.scan_tmp);
+    dfree(w.total);
+    dfree(w.density);
+  };
+  sv_status s = dalloc(c, &grid, 64);
+  if (!s) s = dalloc(c, &w.cnt, cells);
+  if (!s) s = dalloc(c, &w.start, cells);
+  if (!s) s = dalloc(c, &w.cell, N);
+  if (!s) s = dalloc(c, &w.rank, N);
+  if (!s) s = dalloc(c, &w.spos, N);
+  if (!s) s = dalloc(c, &w.scan_tmp, scan_tmp_words(cells));
+  if (!s) s = dalloc(c, &w.total, 1);
+  if (!s) s = dalloc(c, &w.density, N);
+  if (s) {
+    release();
+    return s;
+  }
+  w.grid = grid;
+  hipError_t e = launch_knn_init(c->st, c->P, c->cap, N, scene_extent, w);
+  if (e == hipSuccess && density)
+    e = hipMemcpyAsync(density, w.density, (size_t)N * sizeof(float), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  release();
+  if (e != hipSuccess) return fail(c, SV_E_HIP, "knn_init: %s", hipGetErrorString(e));
   c->built = false;
   c->forward_done = false;
   return SV_OK;

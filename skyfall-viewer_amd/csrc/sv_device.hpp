@@ -40,8 +40,15 @@ struct DevView {
   uint32_t W, H, tiles_x, tiles_y;
   uint32_t tile_base;  // first global tile id of this view
   uint32_t n_tiles;
+  uint32_t super_base; // first global supertile id of this view (binned build)
+  uint32_t super_x;    // supertiles per row
   uint64_t pix_off;    // first pixel of this view in the pixel arrays
 };
+
+// Binned tile build: supertiles of BIN_S x BIN_S tiles; a supertile pair
+// carries its tile rectangle (3 bits per bound) above the key bits.
+constexpr uint32_t BIN_S = 8;
+constexpr uint32_t BIN_RECT_BITS = 12;
 
 // Values of the current tile build that only the device knows; kernels
 // read them so that no launch depends on a host readback (the batched step

@@ -17,7 +17,7 @@ namespace sv {
 // radius); counts: u32 per (slot, Gaussian).
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq);
+                    uint32_t *max_dq, uint32_t *scounts = nullptr);
 // Exclusive scan of u32 counts (M entries) into u32 offsets; total (u64)
 // written to *total. tmp must hold scan_tmp_words(M) u64 words.
 size_t scan_tmp_words(size_t M);
@@ -37,6 +37,20 @@ void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t 
 template <typename K>
 void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap, uint32_t n_tiles,
                    const StepMeta *m, uint32_t *ranges);
+// Binned build (sv_bin.hip): pass info of the supertile pairs from m[0]
+// and m[1].n_isect; supertile pairs of each item plus the tile-count
+// difference array; per-tile counts by row / column prefix sums; tile
+// lists from the sorted supertile pairs.
+void launch_bin_info(hipStream_t st, StepMeta *m, uint32_t super_bits, uint64_t cap_super);
+void launch_emit_super(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+                       const uint32_t *offs, const uint32_t *scounts, const uint32_t *soffs, uint32_t N, int nv,
+                       const DevView *views, const StepMeta *m, uint32_t *diff, uint32_t *keys, uint32_t *vals);
+void launch_tile_counts(hipStream_t st, const DevView *views, int nv, uint32_t max_tiles_x,
+                        uint32_t max_tiles_y, uint32_t *diff);
+void launch_fill_tiles(hipStream_t st, const uint32_t *keys0, const uint32_t *vals0, const uint32_t *keys1,
+                       const uint32_t *vals1, const uint32_t *sranges, const uint32_t *super_view,
+                       uint32_t n_super_tiles, const DevView *views, const StepMeta *m, const uint32_t *ranges,
+                       uint32_t *out);
 // Residual of training_loss (training-loss.wgsl:86-126) for one slot.
 void launch_loss(hipStream_t st, const DevView &hv, const DevConfig &cfg, const float *color,
                  const float *target, float *residual);
@@ -77,10 +91,9 @@ struct RasterArgs {
   // backward outputs
   const uint32_t *offs;   // per (slot, Gaussian) first intersection (emit order)
   float *isect_rec;       // 9-float adjoint record per intersection (emit order)
-  // sorted keys (buffer per meta->sel) and, per global tile, the sort key
+  // per (slot, Gaussian) depth and radius; per global tile, the sort key
   // (depth_q << 32 | Gaussian) of the first entry no pixel reaches
-  const void *keys0, *keys1;
-  uint32_t key64;
+  const float2 *aux;
   uint64_t *tile_cut;
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent queue counter (null: one block per tile)
@@ -159,5 +172,24 @@ void launch_densify(hipStream_t st, const float *P, const float *M1, const float
 void launch_u8_to_u32(hipStream_t st, const uint8_t *a, uint32_t *o, uint32_t N, int mode);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_f32(hipStream_t st, float *p, float v, size_t n);
+
+// sv_knn.hip ------------------------------------------------------------
+// knn_init (knn-init.wgsl:40-90) by an exact grid search: cells per axis
+// for N points, the cell count, and the device scratch of one call.
+uint32_t knn_grid_dim(uint32_t N);
+size_t knn_grid_cells(uint32_t N);
+struct KnnScratch {
+  void *grid;           // 64 bytes: grid box, cell size, cells per axis
+  uint32_t *cnt;        // per cell
+  uint32_t *start;      // per cell
+  uint32_t *cell;       // per point
+  uint32_t *rank;       // per point
+  float4 *spos;         // per point, in cell order: xyz (f16-rounded), id
+  uint64_t *scan_tmp;   // scan_tmp_words(cells)
+  uint64_t *total;
+  float *density;       // per point
+};
+// Writes the three log-scale planes of P and w.density.
+hipError_t launch_knn_init(hipStream_t st, float *P, size_t cap, uint32_t N, float extent, const KnnScratch &w);
 
 }  // namespace sv

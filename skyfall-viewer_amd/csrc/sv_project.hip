@@ -34,7 +34,8 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
                                                  DevConfig cfg, float4 *__restrict__ rec,
                                                  float2 *__restrict__ aux,
                                                  uint32_t *__restrict__ counts,
-                                                 uint32_t *__restrict__ max_dq) {
+                                                 uint32_t *__restrict__ max_dq,
+                                                 uint32_t *__restrict__ scounts) {
   // Records of one view for the block's 256 Gaussians are staged in LDS
   // and stored as whole contiguous lines (each store instruction of a wave
   // writes 1 KB), instead of 64-byte pieces 64 bytes apart per lane.
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
       if (depth < cfg.near_plane || depth > cfg.far_plane) alive = false;
       float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
       float2 ax = make_float2(0.f, 0.f);
-      uint32_t count = 0;
+      uint32_t count = 0, scount = 0;
       if (alive) {
         const float tx = pc[0], ty = pc[1], tz = pc[2];
         const float fx = V.focal[0], fy = V.focal[1];
@@ -140,6 +141,8 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           y1 = min(y1, V.tiles_y);
           const int32_t nx = (int32_t)x1 - (int32_t)x0, ny = (int32_t)y1 - (int32_t)y0;
           count = (nx <= 0 || ny <= 0) ? 0u : (uint32_t)(nx * ny);
+          // supertiles of BIN_S x BIN_S tiles the rectangle touches (binned build)
+          if (count) scount = ((x1 - 1u) / BIN_S - x0 / BIN_S + 1u) * ((y1 - 1u) / BIN_S - y0 / BIN_S + 1u);
           // viewDir = normalize(-(view * pos).xyz), training-forward-tiles.wgsl:185
           const float dxv = -pc[0], dyv = -pc[1], dzv = -pc[2];
           const float len = sqrtf(dot3(dxv, dyv, dzv, dxv, dyv, dzv));
@@ -177,6 +180,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
       if (live) {
         aux[vg] = ax;
         counts[vg] = count;
+        if (scounts) scounts[vg] = scount;
       }
       __syncthreads();
       // piece j of the block's 1024 float4 belongs to Gaussian
@@ -207,11 +211,11 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
 
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq) {
+                    uint32_t *max_dq, uint32_t *scounts) {
   if (N == 0) return;
   const uint32_t blocks = (N + 255u) / 256u;
   hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, aux,
-                     counts, max_dq);
+                     counts, max_dq, scounts);
 }
 
 // ---------------------------------------------------------------------
@@ -457,8 +461,10 @@ __global__ __launch_bounds__(256) void k_ranges(const K *__restrict__ keys0, con
   if (i >= I) return;
   const K *__restrict__ keys = m->sel ? keys1 : keys0;
   const uint32_t dbits = m->dbits;
-  const uint32_t t = (uint32_t)(keys[i] >> dbits);
-  const uint32_t lo = (i == 0) ? 0u : (uint32_t)(keys[i - 1] >> dbits) + 1u;
+  // bits above the key bits carry a payload (binned build)
+  const K kmask = m->key_bits >= 8 * sizeof(K) ? ~(K)0 : (((K)1 << m->key_bits) - 1);
+  const uint32_t t = (uint32_t)((keys[i] & kmask) >> dbits);
+  const uint32_t lo = (i == 0) ? 0u : (uint32_t)((keys[i - 1] & kmask) >> dbits) + 1u;
   for (uint32_t u = lo; u <= t && u <= n_tiles; ++u) ranges[u] = (uint32_t)i;
   if (i == I - 1)
     for (uint32_t u = t + 1u; u <= n_tiles; ++u) ranges[u] = (uint32_t)I;

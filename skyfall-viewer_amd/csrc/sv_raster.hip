@@ -438,11 +438,10 @@ __global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
       if (lane == 0) {
         uint64_t cut = ~0ull;
         if (tend < end) {
-          const uint32_t dmask = (1u << a.meta->dbits) - 1u;
-          const uint32_t sel = a.meta->sel;
-          const uint32_t dq = a.key64 ? (uint32_t)(static_cast<const uint64_t *>(sel ? a.keys1 : a.keys0)[tend] & dmask)
-                                      : (static_cast<const uint32_t *>(sel ? a.keys1 : a.keys0)[tend] & dmask);
-          cut = ((uint64_t)dq << 32) | a.vals[tend];
+          // the depth quantum of the sort key, as emit_intersections forms it
+          const uint32_t g = a.vals[tend];
+          const uint32_t dq = f2u_sat(clampf(a.aux[vbase + min(g, a.N - 1u)].x, 0.0f, 65535.0f)) & 0xffffu;
+          cut = ((uint64_t)dq << 32) | g;
         }
         a.tile_cut[tg] = cut;
       }

@@ -65,6 +65,7 @@ export interface MI355XGaussianRenderer {
   getTrainingStatus(): TrainingStatus;
   prune(customParams?: PruneParams): Promise<number>;
   resetOpacity(maxOpacity?: number): Promise<void>;
+  initializeKnnAndSync(sceneExtent?: number): Promise<Float32Array>;
   densify(opts?: { views?: number[]; percentile?: number; splitPx?: number; seed?: number }): Promise<{ cloned: number; split: number }>;
   enableProfiling(enabled: boolean): void;
   printProfilingReport(): void;
@@ -78,7 +79,7 @@ export interface MI355XGaussianRenderer {
 
 export declare function createGaussianRenderer(
   pointCloud: PointCloudArrays,
-  options?: { capacity?: number; device?: number; viewOrder?: 'cycle' | 'batched'; config?: Record<string, number> },
+  options?: { capacity?: number; device?: number; viewOrder?: 'cycle' | 'batched'; config?: Record<string, number>; sceneExtent?: number },
 ): MI355XGaussianRenderer;
 
 export declare const native: Record<string, unknown>;

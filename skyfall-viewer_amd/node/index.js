@@ -163,6 +163,10 @@ function createGaussianRenderer(pointCloud, options = {}) {
       return native.pruneOpacity(ctx.handle, minOpacity);
     },
     async resetOpacity(maxOpacity = 0.1) { native.resetOpacity(ctx.handle, maxOpacity); },
+    // initializeKnnAndSync (gaussian-renderer.ts:2159-2172): density-aware
+    // initial log-scales from the 3 nearest neighbours; the trainer's
+    // parameters are the device masters, so there is nothing to sync.
+    async initializeKnnAndSync(sceneExtent = options.sceneExtent || 1.0) { return native.knnInit(ctx.handle, sceneExtent); },
     async densify({ views, percentile = 0.9, splitPx = 8, seed = iteration } = {}) {
       const d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx);
       native.densify(ctx.handle, d.actions, seed);

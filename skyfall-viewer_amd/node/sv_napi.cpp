@@ -711,6 +711,22 @@ napi_value ResetOpacity(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// knnInit(ctx, sceneExtent) -> Float32Array density (initializeKnn)
+napi_value KnnInit(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  sv_ctx *c = ctx_of(env, a.v[0]);
+  if (!c) return nullptr;
+  double e;
+  if (!get_f64(env, a.v[1], &e)) return nullptr;
+  const size_t n = sv_get_count(c);
+  float *d = nullptr;
+  napi_value arr = new_typed<float>(env, napi_float32_array, n, &d);
+  if (!arr) return nullptr;
+  SV(c, sv_knn_init(c, (float)e, n ? d : nullptr), "sv_knn_init");
+  return arr;
+}
+
 napi_value Densify(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
@@ -929,7 +945,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"stepAsync", StepAsync},     {"readImage", ReadImage},     {"readResidual", ReadResidual},
              {"readAlpha", ReadAlpha},     {"readLastIds", ReadLastIds}, {"readGrads", ReadGrads},
              {"readKeys", ReadKeys},       {"pruneOpacity", PruneOpacity}, {"pruneMask", PruneMask},
-             {"densify", Densify},         {"resetOpacity", ResetOpacity},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
+             {"densify", Densify},         {"resetOpacity", ResetOpacity}, {"knnInit", KnnInit},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
              {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},
              {"saveCheckpoint", SaveCheckpoint}, {"loadCheckpoint", LoadCheckpoint},               {"sceneGaussians", SceneGaussians},
              {"sceneJitter", SceneJitter}, {"sceneCamera", SceneCamera}};

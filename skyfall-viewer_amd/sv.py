@@ -103,6 +103,7 @@ _SIGS = {
     "sv_prune_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
     "sv_densify": (C.c_int, [C.c_void_p, _U8, C.c_uint64, C.POINTER(C.c_size_t)]),
     "sv_reset_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
+    "sv_knn_init": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
     "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, _U8,
                                     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
@@ -526,6 +527,13 @@ class Context:
         n = C.c_size_t()
         self._ck(lib().sv_reset_opacity(self.h, max_opacity, C.byref(n)), "reset_opacity")
         return n.value
+
+    def knn_init(self, scene_extent: float) -> np.ndarray:
+        """Density-aware initial log-scales (initializeKnn); returns the
+        per-Gaussian mean distance to the three nearest neighbours."""
+        dens = np.zeros(self.count, np.float32)
+        self._ck(lib().sv_knn_init(self.h, scene_extent, dens.ctypes.data if dens.size else None), "knn_init")
+        return dens
 
     def densify_decide(self, views, percentile: float = 0.9, split_px: float = 8.0):
         """Returns (actions uint8[N], n_clone, n_split)."""
