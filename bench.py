@@ -51,6 +51,7 @@ def configure(ctx: sv.Context, args, keep_grads=0):
     c.persistent = args.persistent
     c.keep_grads = keep_grads
     c.xcd_remap = args.xcd_remap
+    c.binned_build = args.binned
     ctx.set_config(c)
 
 
@@ -185,6 +186,8 @@ def main():
     ap.add_argument("--persistent", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1, help="1: the batched step is one hipGraph launch")
     ap.add_argument("--xcd-remap", type=int, default=1, help="1: contiguous tile ranges per XCD")
+    ap.add_argument("--binned", type=int, default=1,
+                    help="1: tile lists by supertile binning; 0: sort every (tile, splat) pair")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: time the tuner's candidates first and run the batched step with the fastest")
     ap.add_argument("--no-plain", action="store_true")
@@ -237,7 +240,7 @@ def main():
     sec_b, tim_b, ps_b, st_b, eager_b, tune_b, chosen_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args,
                                                                    graph=args.graph)
     log(f"batched: {1e3 * sec_b:.2f} ms/step (graph {args.graph}), I={st_b.n_isect}, "
-        f"key bits {st_b.key_bits}, passes {st_b.sort_passes}"
+        f"key bits {st_b.key_bits}, passes {st_b.sort_passes}, binned {st_b.binned}"
         + (f"; eager launches {1e3 * eager_b:.2f} ms/step" if eager_b else ""))
     plain = sequential = None
     if not args.no_plain:
@@ -255,16 +258,19 @@ def main():
     I = int(st_b.n_isect)
     key_bytes = 8 if st_b.key_bits > 32 else 4
     passes = int(st_b.sort_passes)
+    pair_passes = (int(st_b.key_bits) + 7) // 8  # passes of a sort of every (tile, depth) key
     phase = {k: (v[0] / max(v[1], 1), v[1] / ps_b) for k, v in tim_b.items() if v[1]}
-    sort_ms_step = tim_b.get("sort", (0.0, 0))[0] / ps_b
-    keys_per_s = I / (sort_ms_step * 1e-3) if sort_ms_step > 0 else None
+    # sorted keys/s: intersections per second of building the depth-ordered
+    # tile lists from the projections (emit + sort + ranges [+ fill])
+    list_ms_step = sum(tim_b.get(k, (0.0, 0))[0] for k in ("emit", "sort", "ranges", "fill")) / ps_b
+    keys_per_s = I / (list_ms_step * 1e-3) if list_ms_step > 0 else None
     # dominant single-kernel phase
     cands = {k: phase[k][0] for k in ("forward", "backward", "vjp_adam") if k in phase}
     dom = max(cands, key=cands.get)
     dom_ms = cands[dom]
-    dom_bytes = kernel_bytes(dom, N, V, I, P, key_bytes, passes)
+    dom_bytes = kernel_bytes(dom, N, V, I, P, key_bytes, pair_passes)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    b_alg = step_b_alg(N, V, I, P, passes)
+    b_alg = step_b_alg(N, V, I, P, pair_passes)
     traffic = measured_traffic(dom, N, V, W, H, args.workload)
     out = {
         "metric": METRIC,
@@ -283,11 +289,14 @@ def main():
                    "width": W if args.workload == "ring16" else None,
                    "height": H if args.workload == "ring16" else None, "pixels": P,
                    "tile": 16, "lds_batch": chosen_b["batch"],
-                   "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"]},
+                   "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"],
+                   "binned_build": args.binned},
         "autotune": tune_b,
         "eager_launch_ms_per_step": 1e3 * eager_b if eager_b else None,
         "keys_per_s": keys_per_s,
+        "tile_list_ms_per_step": list_ms_step,
         "intersections_per_step": I,
+        "tile_lists": "binned (supertile pairs sorted, lists filled)" if st_b.binned else "pair sort",
         "sort_key_bits": int(st_b.key_bits),
         "sort_passes": passes,
         "plain": plain,

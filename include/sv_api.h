@@ -80,6 +80,10 @@ typedef struct sv_config {
   uint32_t xcd_remap;       /* 1: raster workgroups on one XCD take a
                                contiguous range of tiles (L2 locality of
                                neighbouring tiles' records) */
+  uint32_t binned_build;    /* 1: build the tile lists by binning splats into
+                               8x8-tile supertiles (sort of one pair per
+                               splat and supertile) instead of sorting every
+                               (tile, splat) pair; same lists and ranges */
 } sv_config;
 
 /* Step modes for sv_step. */
@@ -99,9 +103,10 @@ typedef struct sv_config {
 typedef struct sv_step_stats {
   uint64_t n_isect;     /* intersections sorted this step (all views) */
   uint32_t n_views;
-  uint32_t sort_passes; /* 8-bit digit passes of the last sort */
-  uint32_t key_bits;    /* significant key bits of the last sort */
-  uint32_t pad;
+  uint32_t sort_passes; /* 8-bit digit passes of the last sort (of the
+                           supertile pairs when binned) */
+  uint32_t key_bits;    /* significant bits of the (tile, depth) key */
+  uint32_t binned;      /* 1: the tile lists were built by binning */
   double l1_sum;        /* sum |pred - target| over all pixels and channels */
 } sv_step_stats;
 
@@ -255,6 +260,19 @@ sv_status sv_reset_opacity(sv_ctx *ctx, float max_opacity, size_t *n_clamped);
  * are kept (initializeKnnAndSync, :2159-2172, runs before training).
  * density: count floats (the shader's density_buffer) or NULL. */
 sv_status sv_knn_init(sv_ctx *ctx, float scene_extent, float *density);
+/* generateMultiViewScores (gaussian-renderer.ts:2754-2781) over the given
+ * views, in order, on the device (metric-map.wgsl:32-147; the per-view
+ * min / max / mean of computePixelLossStats, :2709-2745, reduced on the
+ * device instead of read back): each view is rendered (one batched build),
+ * pixels with (l1 - min) / (max - min) > 0.1 are flagged, and each splat
+ * of a flagged pixel's tile list up to its last index with power in
+ * [0, 6] counts once. importance[i] = sum of counts, pruning[i] = sum of
+ * counts x the view's mean l1 (count floats each). The caller picks the
+ * views (the reference samples up to 10 with Math.random). Views without
+ * targets are skipped, as views without a finite metric. n_scored: views
+ * scored (may be NULL). */
+sv_status sv_multiview_scores(sv_ctx *ctx, const int *views, int nv, uint32_t *importance, float *pruning,
+                              int *n_scored);
 /* performDensification (gaussian-renderer.ts:3211-3466) with a seeded RNG:
  * actions[i] = 0 none, 1 clone, 2 split. Output order: non-split originals
  * in order, then children in parent order (clone: 1, split: 2); children

@@ -294,6 +294,32 @@ def knn_init(pos: np.ndarray, extent: float):
     return dens, ls
 
 
+def multiview_scores(g: dict, cams, views, targets, n_views=None):
+    """generateMultiViewScores over the given views in order
+    (gaussian-renderer.ts:2754-2781): per view render, score
+    (orc_metric_view), then importance += counts and pruning += counts *
+    mean (a double sum stored to float32). Returns (importance u32,
+    pruning f32, views scored)."""
+    n = g["pos"].shape[0]
+    imp = np.zeros(n, np.uint32)
+    pru = np.zeros(n, np.float32)
+    scored = 0
+    for cam, view, tgt in zip(cams, views, targets):
+        r = render_and_grad(g, cam, view)
+        counts = np.zeros(n, np.uint32)
+        mean = C.c_double(0.0)
+        t = np.ascontiguousarray(tgt, np.float32)
+        ok = lib().orc_metric_view(C.byref(view), C.c_size_t(n), r["proj"].ctypes.data_as(C.c_void_p),
+                                   _u32p(r["offsets"]), _u32p(r["vals"]), _u32p(r["last_ids"]),
+                                   _fp(r["color"]), _fp(t), _u32p(counts), C.byref(mean))
+        if not ok:
+            continue
+        scored += 1
+        imp += counts
+        pru = (pru.astype(np.float64) + counts.astype(np.float64) * mean.value).astype(np.float32)
+    return imp, pru, scored
+
+
 def densify(g: dict, m1: dict, m2: dict, actions: np.ndarray, seed: int):
     """Returns (params, m1, m2) dicts after clone / split."""
     r, a1, a2 = to_rows(g), to_rows(m1), to_rows(m2)

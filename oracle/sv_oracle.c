@@ -36,6 +36,8 @@
  *                       -lr*g/(|g|+eps)
  *   orc_accumulate_*    closed forms
  *   orc_compact_rows    numpy boolean indexing
+ *   orc_metric_view     numpy restatement of the per-pixel metric and
+ *                       brute-force splat walks on small scenes
  *   orc_knn_init        scipy cKDTree 3-NN distances in float64 and the
  *                       sigma rule in numpy; special cases n <= 3,
  *                       duplicate points
@@ -1482,4 +1484,64 @@ void orc_knn_init(size_t n, const float *pos, float extent, float *density, floa
     logscale[idx] = orc_f16_round((float)log((double)fmaxf(sigma, 1e-6f)));
   }
   free(p);
+}
+
+/* ------------------------------------------------------------------ */
+/* Multi-view scoring of one view: metric-map.wgsl:32-147 and          */
+/* gaussian-renderer.ts:2709-2745 (stats, threshold uniform)           */
+/* ------------------------------------------------------------------ */
+int orc_metric_view(const orc_view *v, size_t n, const orc_proj *proj, const uint32_t *offsets,
+                    const uint32_t *vals, const uint32_t *last_ids, const float *color,
+                    const float *target, uint32_t *counts, double *mean) {
+  const uint32_t W = v->width, H = v->height, ts = v->tile_size;
+  const size_t P = (size_t)W * H;
+  float *loss = (float *)malloc((P ? P : 1) * sizeof(float));
+  for (size_t i = 0; i < P; ++i) { /* compute_metric_map, :32-50 */
+    const float d0 = fabsf(color[3 * i] - target[3 * i]);
+    const float d1 = fabsf(color[3 * i + 1] - target[3 * i + 1]);
+    const float d2 = fabsf(color[3 * i + 2] - target[3 * i + 2]);
+    loss[i] = (d0 + d1 + d2) / 3.0f;
+  }
+  /* computePixelLossStats (gaussian-renderer.ts:2709-2745) */
+  double mn = INFINITY, mx = -INFINITY, sum = 0.0;
+  for (size_t i = 0; i < P; ++i) {
+    const double x = loss[i];
+    if (!isfinite(x)) continue;
+    if (x < mn) mn = x;
+    if (x > mx) mx = x;
+    sum += x;
+  }
+  if (!isfinite(mn) || !isfinite(mx)) {
+    free(loss);
+    return 0;
+  }
+  *mean = P > 0 ? sum / (double)P : 0.0;
+  const float min_f = (float)mn;
+  const float inv_range = (float)(mx > mn ? 1.0 / (mx - mn) : 0.0);
+  const float thresh = 0.1f; /* MULTI_VIEW_LOSS_THRESH, :70 */
+  for (size_t idx = 0; idx < P; ++idx) {
+    const float value = (loss[idx] - min_f) * inv_range; /* threshold_metric_map, :60-62 */
+    if (!(value > thresh)) continue;
+    const uint32_t x = (uint32_t)(idx % W), y = (uint32_t)(idx / W); /* accumulate_metric_counts */
+    const uint32_t tx = x / ts, ty = y / ts;
+    if (tx >= v->tiles_x || ty >= v->tiles_y) continue;
+    const uint32_t tile = ty * v->tiles_x + tx;
+    const uint32_t start = offsets[tile], end = offsets[tile + 1];
+    if (start >= end) continue;
+    const uint32_t last = last_ids[idx];
+    if (last < start) continue;
+    const uint32_t fin = last < end - 1u ? last : end - 1u;
+    const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+    for (uint32_t i = start; i <= fin; ++i) {
+      const uint32_t g = vals[i];
+      if (g >= n) continue;
+      const orc_proj *gp = &proj[g];
+      const float dx = px - gp->mean2d[0], dy = py - gp->mean2d[1];
+      const float s2 = 0.5f * (gp->conic[0] * dx * dx + gp->conic[2] * dy * dy) + gp->conic[1] * dx * dy;
+      if (s2 < 0.0f || s2 > 6.0f) continue;
+      counts[g] += 1u;
+    }
+  }
+  free(loss);
+  return 1;
 }

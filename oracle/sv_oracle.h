@@ -196,6 +196,19 @@ void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
  * 1e-6))). O(n^2), as the shader. density / logscale: n floats each. */
 void orc_knn_init(size_t n, const float *pos, float extent, float *density, float *logscale);
 
+/* Scoring of one rendered view for generateMultiViewScores
+ * (gaussian-renderer.ts:2640-2781, metric-map.wgsl:32-147):
+ * l1 = ((|dr| + |dg|) + |db|) / 3 per pixel; min / max over finite values
+ * and mean = (sequential double sum) / pixels (computePixelLossStats); a
+ * pixel is flagged when (l1 - min) * f32(1 / (max - min)) > 0.1; for every
+ * flagged pixel, counts[g] += 1 for each splat g of its tile list from the
+ * start to min(last id, end - 1) with 0 <= power <= 6. Returns 0 when the
+ * view has no finite value (the reference skips it), else 1; *mean set.
+ * color / target: H x W x 3; counts: n, accumulated (not cleared). */
+int orc_metric_view(const orc_view *v, size_t n, const orc_proj *proj, const uint32_t *offsets,
+                    const uint32_t *vals, const uint32_t *last_ids, const float *color,
+                    const float *target, uint32_t *counts, double *mean);
+
 /* Vector helpers for the tests. */
 void orc_expf_v(size_t n, const float *x, float *y);
 void orc_f16_round_v(size_t n, const float *x, float *y);

@@ -42,7 +42,8 @@ struct TimedPhase {
 };
 
 const char *kPhaseNames[] = {"project", "scan", "emit", "sort", "ranges", "forward", "backward",
-                             "vjp_adam", "adam", "loss", "backward_fixed", "compact", "readback_sync"};
+                             "vjp_adam", "adam", "loss", "backward_fixed", "compact", "readback_sync",
+                             "fill"};
 constexpr int kNumPhases = sizeof(kPhaseNames) / sizeof(kPhaseNames[0]);
 enum Phase {
   PH_PROJECT,
@@ -57,7 +58,8 @@ enum Phase {
   PH_LOSS,
   PH_FIXED,
   PH_COMPACT,
-  PH_SYNC
+  PH_SYNC,
+  PH_FILL
 };
 
 uint32_t bitlen(uint64_t v) {
@@ -105,6 +107,7 @@ struct sv_ctx {
   float4 *rec = nullptr;
   float2 *aux = nullptr;
   uint32_t *counts = nullptr, *offs = nullptr;
+  uint32_t *scounts = nullptr, *soffs = nullptr;  // supertile pairs per (slot, Gaussian)
   float *vrec = nullptr;  // 9 planes of slots x N: reached-record sums for the VJP
   size_t vg_cap = 0;
   uint64_t *scan_tmp = nullptr;
@@ -124,6 +127,22 @@ struct sv_ctx {
   int32_t *fixed = nullptr;
   uint64_t fixed_cap = 0;
   uint64_t *h_small = nullptr;  // pinned
+
+  // binned build: supertiles of the batch, pair buffers, difference array
+  uint32_t super_total = 0, super_bits = 0, max_tiles_x = 0, max_tiles_y = 0;
+  uint32_t *super_view = nullptr, *sranges = nullptr;
+  size_t super_tab_cap = 0;
+  uint32_t *diff = nullptr;  // per global tile + 1: tile counts
+  uint32_t *nch = nullptr, *cbase = nullptr;  // per supertile + 1
+  uint32_t *skeys = nullptr, *skeys2 = nullptr, *svals = nullptr, *svals2 = nullptr, *ssort_tmp = nullptr;
+  uint64_t super_cap = 0;
+  uint32_t *chunk_super = nullptr, *ccount = nullptr;
+  size_t chunk_cap = 0;
+  int max_super_passes = 0;
+  bool binned = false;     // the next build bins
+  bool bin_unfit = false;  // a build found supertile key + rectangle wider than 32 bits
+  bool last_binned = false;  // the last completed build binned
+  uint32_t spasses = 0;
 
   // key plan of the batch (host bounds) and the last tile build (read back)
   uint32_t tile_bits = 0, max_key_bits = 0;
@@ -298,6 +317,7 @@ sv_status check_view(sv_ctx *c, int v) {
 // ---------------------------------------------------------------------
 void key_plan(sv_ctx *c);
 sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n);
+sv_status ensure_super_capacity(sv_ctx *c, uint64_t n);
 
 sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
   if (nv <= 0) return fail(c, SV_E_ARG, "no views");
@@ -311,7 +331,8 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
   if (!same) {
     c->slot_views.assign(vs, vs + nv);
     c->h_slots.resize(nv);
-    uint32_t tb = 0;
+    uint32_t tb = 0, sb = 0;
+    c->max_tiles_x = c->max_tiles_y = 0;
     for (int s = 0; s < nv; ++s) {
       const HostView &hv = c->views[vs[s]];
       DevView &d = c->h_slots[s];
@@ -328,9 +349,15 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       d.n_tiles = hv.n_tiles;
       d.tile_base = tb;
       d.pix_off = hv.pix_off;
+      d.super_x = (hv.tiles_x + BIN_S - 1) / BIN_S;
+      d.super_base = sb;
       tb += hv.n_tiles;
+      sb += d.super_x * ((hv.tiles_y + BIN_S - 1) / BIN_S);
+      c->max_tiles_x = std::max(c->max_tiles_x, hv.tiles_x);
+      c->max_tiles_y = std::max(c->max_tiles_y, hv.tiles_y);
     }
     c->tiles_total = tb;
+    c->super_total = sb;
     if (sv_status s = grow(c, &c->d_views, c->d_views_cap, (size_t)nv)) return s;
     HIP_OK(c, hipMemcpyAsync(c->d_views, c->h_slots.data(), nv * sizeof(DevView), hipMemcpyHostToDevice, c->st));
     if (c->tiles_cap < (size_t)tb + 1) {
@@ -339,6 +366,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       dfree(c->l1_partial);
       dfree(c->tile_l1);
       dfree(c->tile_cut);
+      dfree(c->diff);
       c->tiles_cap = 0;
       c->scored_views.clear();
       size_t want = (size_t)tb + 1;
@@ -347,13 +375,33 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       if (sv_status s = dalloc(c, &c->l1_partial, want)) return s;
       if (sv_status s = dalloc(c, &c->tile_l1, want)) return s;
       if (sv_status s = dalloc(c, &c->tile_cut, want)) return s;
+      if (sv_status s = dalloc(c, &c->diff, want)) return s;
       c->tiles_cap = want;
     }
+    if (c->super_tab_cap < (size_t)sb + 1) {
+      dfree(c->super_view);
+      dfree(c->sranges);
+      dfree(c->nch);
+      dfree(c->cbase);
+      c->super_tab_cap = 0;
+      if (sv_status s = dalloc(c, &c->super_view, (size_t)sb + 1)) return s;
+      if (sv_status s = dalloc(c, &c->sranges, (size_t)sb + 1)) return s;
+      if (sv_status s = dalloc(c, &c->nch, (size_t)sb + 1)) return s;
+      if (sv_status s = dalloc(c, &c->cbase, (size_t)sb + 1)) return s;
+      c->super_tab_cap = (size_t)sb + 1;
+    }
+    std::vector<uint32_t> stv(sb);
+    for (int s = 0; s < nv; ++s) {
+      const DevView &d = c->h_slots[s];
+      const uint32_t ns = d.super_x * ((d.tiles_y + BIN_S - 1) / BIN_S);
+      for (uint32_t t = 0; t < ns; ++t) stv[d.super_base + t] = (uint32_t)s;
+    }
+    if (sb) HIP_OK(c, hipMemcpyAsync(c->super_view, stv.data(), sb * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
     std::vector<uint32_t> tv(tb);
     for (int s = 0; s < nv; ++s)
       for (uint32_t t = 0; t < c->h_slots[s].n_tiles; ++t) tv[c->h_slots[s].tile_base + t] = (uint32_t)s;
     if (tb) HIP_OK(c, hipMemcpyAsync(c->tile_view, tv.data(), tb * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
-    // the host vector must outlive the async copy
+    // the host vectors must outlive the async copies
     HIP_OK(c, hipStreamSynchronize(c->st));
   }
   const size_t M = (size_t)nv * c->N;
@@ -362,6 +410,8 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     dfree(c->aux);
     dfree(c->counts);
     dfree(c->offs);
+    dfree(c->scounts);
+    dfree(c->soffs);
     dfree(c->vrec);
     c->vg_cap = 0;
     if (sv_status s = dalloc(c, &c->vrec, (size_t)REC9 * M)) return s;
@@ -369,11 +419,15 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     if (sv_status s = dalloc(c, &c->aux, M)) return s;
     if (sv_status s = dalloc(c, &c->counts, M)) return s;
     if (sv_status s = dalloc(c, &c->offs, M)) return s;
+    if (sv_status s = dalloc(c, &c->scounts, M)) return s;
+    if (sv_status s = dalloc(c, &c->soffs, M)) return s;
     c->vg_cap = M;
   }
-  if (sv_status s = grow(c, &c->scan_tmp, c->scan_tmp_cap, scan_tmp_words(M))) return s;
+  const size_t scan_n = std::max(M, (size_t)c->tiles_total + 1);
+  if (sv_status s = grow(c, &c->scan_tmp, c->scan_tmp_cap, scan_tmp_words(scan_n))) return s;
   key_plan(c);
-  return ensure_isect_capacity(c, 1);
+  if (sv_status s = ensure_isect_capacity(c, 1)) return s;
+  return c->binned ? ensure_super_capacity(c, 1) : SV_OK;
 }
 
 size_t rec_stride(const sv_ctx *c) { return (size_t)c->slot_views.size() * c->N; }
@@ -388,6 +442,41 @@ void key_plan(sv_ctx *c) {
   c->max_key_bits = c->tile_bits + bitlen(dq_max);
   c->key64 = c->max_key_bits > 32;
   c->max_passes = (int)((c->max_key_bits + 7) / 8);
+  // binned build: supertile key + depth bits (the device checks the live
+  // width against the 32-bit key with the rectangle bits)
+  c->super_bits = bitlen(c->super_total > 0 ? c->super_total - 1 : 0);
+  c->max_super_passes = (int)std::min<uint32_t>(4u, (c->super_bits + bitlen(dq_max) + 7) / 8);
+  c->binned = c->cfg.binned_build && !c->bin_unfit && c->super_bits + BIN_RECT_BITS < 32;
+}
+
+// Supertile pair buffers for at least n pairs (binned build).
+sv_status ensure_super_capacity(sv_ctx *c, uint64_t n) {
+  if (n <= c->super_cap && c->skeys) {
+    // chunks of the supertile lists: bounded by pairs / BIN_CH + supertiles
+    const size_t cc = bin_chunk_cap(c->super_cap, c->super_total);
+    if (cc <= c->chunk_cap && c->chunk_super) return SV_OK;
+    dfree(c->chunk_super);
+    dfree(c->ccount);
+    c->chunk_cap = 0;
+    if (sv_status s = dalloc(c, &c->chunk_super, cc)) return s;
+    if (sv_status s = dalloc(c, &c->ccount, cc * 64)) return s;
+    c->chunk_cap = cc;
+    return SV_OK;
+  }
+  const uint64_t want = std::max<uint64_t>(n + n / 4, 1 << 16);
+  dfree(c->skeys);
+  dfree(c->skeys2);
+  dfree(c->svals);
+  dfree(c->svals2);
+  dfree(c->ssort_tmp);
+  c->super_cap = 0;
+  if (sv_status s = dalloc(c, &c->skeys, want)) return s;
+  if (sv_status s = dalloc(c, &c->skeys2, want)) return s;
+  if (sv_status s = dalloc(c, &c->svals, want)) return s;
+  if (sv_status s = dalloc(c, &c->svals2, want)) return s;
+  if (sv_status s = dalloc(c, &c->ssort_tmp, sort_tmp_words<uint32_t>(want))) return s;
+  c->super_cap = want;
+  return ensure_super_capacity(c, n);
 }
 
 // Pair buffers for at least n intersections (contents are not kept).
@@ -422,68 +511,131 @@ sv_status enqueue_build(sv_ctx *c) {
   const int nv = (int)c->slot_views.size();
   const DevConfig dc = dev_config(c->cfg);
   const size_t M = (size_t)nv * c->N;
-  launch_fill_u32(c->st, (uint32_t *)c->d_meta, 0u, sizeof(StepMeta) / 4);
+  const bool bin = c->binned;
+  StepMeta *m = c->d_meta;
+  launch_fill_u32(c->st, (uint32_t *)m, 0u, 4 * sizeof(StepMeta) / 4);
   {
     Scoped t(c, PH_PROJECT);
-    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts,
-                   &c->d_meta->max_dq);
+    launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts, &m->max_dq,
+                   bin ? c->scounts : nullptr);
   }
   {
     Scoped t(c, PH_SCAN);
-    launch_scan(c->st, c->counts, c->offs, M, c->scan_tmp, &c->d_meta->n_isect);
-    launch_keyinfo(c->st, c->d_meta, c->tile_bits, c->max_key_bits, c->isect_cap);
+    launch_scan(c->st, c->counts, c->offs, M, c->scan_tmp, &m->n_isect);
+    if (bin) {
+      launch_scan(c->st, c->scounts, c->soffs, M, c->scan_tmp, &m[1].n_isect);
+      launch_keyinfo(c->st, m, c->tile_bits, 64u, c->isect_cap);
+      launch_bin_info(c->st, m, c->super_bits, c->super_cap);
+    } else {
+      launch_keyinfo(c->st, m, c->tile_bits, c->max_key_bits, c->isect_cap);
+    }
+  }
+  if (bin) {
+    {
+      Scoped t(c, PH_EMIT);
+      launch_emit_super(c->st, c->rec, c->aux, c->counts, c->offs, c->scounts, c->soffs, c->N, nv, c->d_views, m,
+                        c->skeys, c->svals);
+    }
+    {
+      Scoped t(c, PH_SORT);
+      launch_sort<uint32_t>(c->st, c->skeys, c->svals, c->skeys2, c->svals2, c->super_cap, c->max_super_passes,
+                            c->ssort_tmp, &m[1]);
+    }
+    {
+      Scoped t(c, PH_RANGES);
+      launch_ranges<uint32_t>(c->st, c->skeys, c->skeys2, c->super_cap, c->super_total, &m[1], c->sranges);
+    }
+    {
+      Scoped t(c, PH_FILL);
+      BinArgs b{};
+      b.keys0 = c->skeys;
+      b.vals0 = c->svals;
+      b.keys1 = c->skeys2;
+      b.vals1 = c->svals2;
+      b.sranges = c->sranges;
+      b.super_view = c->super_view;
+      b.views = c->d_views;
+      b.m = m;
+      b.n_super_tiles = c->super_total;
+      b.n_tiles = c->tiles_total;
+      b.chunk_cap = c->chunk_cap;
+      b.nch = c->nch;
+      b.cbase = c->cbase;
+      b.chunk_super = c->chunk_super;
+      b.ccount = c->ccount;
+      b.tcount = c->diff;
+      b.scan_tmp = c->scan_tmp;
+      b.ranges = c->ranges;
+      b.out = c->vals;
+      launch_fill_tiles(c->st, b);
+    }
+    return check_launch(c, "binned tile build");
   }
   {
     Scoped t(c, PH_EMIT);
     if (c->key64)
-      launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
+      launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
                             (uint64_t *)c->keys, c->vals);
     else
-      launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, c->d_meta,
+      launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
                             (uint32_t *)c->keys, c->vals);
   }
   {
     Scoped t(c, PH_SORT);
     if (c->key64)
       launch_sort<uint64_t>(c->st, (uint64_t *)c->keys, c->vals, (uint64_t *)c->keys2, c->vals2,
-                            c->isect_cap, c->max_passes, c->sort_tmp, c->d_meta);
+                            c->isect_cap, c->max_passes, c->sort_tmp, m);
     else
       launch_sort<uint32_t>(c->st, (uint32_t *)c->keys, c->vals, (uint32_t *)c->keys2, c->vals2,
-                            c->isect_cap, c->max_passes, c->sort_tmp, c->d_meta);
+                            c->isect_cap, c->max_passes, c->sort_tmp, m);
   }
   {
     Scoped t(c, PH_RANGES);
     if (c->key64)
       launch_ranges<uint64_t>(c->st, (const uint64_t *)c->keys, (const uint64_t *)c->keys2, c->isect_cap,
-                              c->tiles_total, c->d_meta, c->ranges);
+                              c->tiles_total, m, c->ranges);
     else
       launch_ranges<uint32_t>(c->st, (const uint32_t *)c->keys, (const uint32_t *)c->keys2, c->isect_cap,
-                              c->tiles_total, c->d_meta, c->ranges);
+                              c->tiles_total, m, c->ranges);
   }
   return check_launch(c, "tile build");
 }
 
-// Wait for the stream and read the build's metadata. *grow_to > 0: the
-// buffers were too small and nothing after the scan ran; grow and redo.
-sv_status read_meta(sv_ctx *c, uint64_t *grow_to) {
-  *grow_to = 0;
-  HIP_OK(c, hipMemcpyAsync(c->h_meta, c->d_meta, sizeof(StepMeta), hipMemcpyDeviceToHost, c->st));
+// Wait for the stream and read the build's metadata. *retry: nothing past
+// the scans wrote any state and the build must run again, after the pair
+// buffers grew (overflow 1) or with the tile pairs sorted instead of
+// binned (overflow 3: supertile key and rectangle wider than 32 bits).
+sv_status read_meta(sv_ctx *c, bool *retry) {
+  *retry = false;
+  HIP_OK(c, hipMemcpyAsync(c->h_meta, c->d_meta, 2 * sizeof(StepMeta), hipMemcpyDeviceToHost, c->st));
   if (sv_status s = sync(c)) return s;
-  const StepMeta &m = *c->h_meta;
+  const StepMeta &m = c->h_meta[0];
+  const StepMeta &m1 = c->h_meta[1];
   if (m.overflow == 2)
     return fail(c, SV_E_CAPACITY, "sort key of %u bits exceeds the planned %u bits", m.key_bits, c->max_key_bits);
+  if (m.overflow == 3) {
+    c->bin_unfit = true;
+    key_plan(c);
+    *retry = true;
+    return SV_OK;
+  }
   if (m.overflow == 1) {
-    if (m.n_isect >= 0xffffffffull)
+    if (m.n_isect >= 0xffffffffull || (c->binned && m1.n_isect >= 0xffffffffull))
       return fail(c, SV_E_CAPACITY, "%llu intersections exceed the 32-bit index range",
-                  (unsigned long long)m.n_isect);
-    *grow_to = m.n_isect;
+                  (unsigned long long)std::max(m.n_isect, m1.n_isect));
+    if (sv_status s = ensure_isect_capacity(c, m.n_isect)) return s;
+    if (c->binned)
+      if (sv_status s = ensure_super_capacity(c, m1.n_isect)) return s;
+    *retry = true;
     return SV_OK;
   }
   c->I = m.n_isect;
   c->dbits = m.dbits;
   c->key_bits = m.key_bits;
-  c->passes = (int)m.passes;
+  c->passes = (int)(c->binned ? m1.passes : m.passes);
+  c->spasses = m1.passes;
   c->sel = (int)m.sel;
+  c->last_binned = c->binned;
   return SV_OK;
 }
 
@@ -492,13 +644,12 @@ sv_status read_meta(sv_ctx *c, uint64_t *grow_to) {
 sv_status build_batch(sv_ctx *c) {
   for (int attempt = 0; attempt < 3; ++attempt) {
     if (sv_status s = enqueue_build(c)) return s;
-    uint64_t grow_to = 0;
-    if (sv_status s = read_meta(c, &grow_to)) return s;
-    if (!grow_to) {
+    bool retry = false;
+    if (sv_status s = read_meta(c, &retry)) return s;
+    if (!retry) {
       c->built = true;
       return SV_OK;
     }
-    if (sv_status s = ensure_isect_capacity(c, grow_to)) return s;
   }
   return fail(c, SV_E_STATE, "tile build did not fit after growing the buffers");
 }
@@ -828,11 +979,16 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
                         c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
-                        c->h_adam_dev};
+                        c->h_adam_dev, c->scounts, c->soffs, c->super_view, c->sranges, c->diff,
+                        c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
+                        c->chunk_super, c->ccount};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
   g.push_back(c->N);
   g.push_back(c->cap);
   g.push_back(c->isect_cap);
+  g.push_back(c->super_cap);
+  g.push_back(c->chunk_cap);
+  g.push_back(c->binned ? 1u : 0u);
   g.push_back(c->tiles_total);
   for (int v : c->slot_views) g.push_back((uint64_t)(uint32_t)v);
   uint64_t w[(sizeof(sv_config) + 7) / 8] = {0};
@@ -893,15 +1049,14 @@ sv_status step_batched(sv_ctx *c, const int *vs, int nv) {
     } else {
       if (sv_status s = enqueue_batched_step(c, &hp)) return s;
     }
-    uint64_t grow_to = 0;
-    if (sv_status s = read_meta(c, &grow_to)) return s;
-    if (!grow_to) {
+    bool retry = false;
+    if (sv_status s = read_meta(c, &retry)) return s;
+    if (!retry) {
       c->step_t = t;
       c->built = true;
       c->forward_done = true;
       return SV_OK;
     }
-    if (sv_status s = ensure_isect_capacity(c, grow_to)) return s;
   }
   return fail(c, SV_E_STATE, "batched step did not fit after growing the buffers");
 }
@@ -1027,6 +1182,7 @@ sv_status sv_default_config(sv_config *o) {
   o->keep_grads = 0;
   o->plain_wf_flags = 0;
   o->xcd_remap = 1;
+  o->binned_build = 1;
   return SV_OK;
 }
 
@@ -1052,11 +1208,11 @@ sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
   c->cap = capacity;
   sv_status s = alloc_gaussian_arrays(c, capacity);
   if (!s) s = dalloc(c, &c->d_small, 8);
-  if (!s) s = dalloc(c, &c->d_meta, 1);
+  if (!s) s = dalloc(c, &c->d_meta, 4);
   if (!s) s = dalloc(c, &c->d_adam, 1);
   if (!s && hipHostMalloc((void **)&c->h_small, 8 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
     s = fail(c, SV_E_NOMEM, "hipHostMalloc failed");
-  if (!s && hipHostMalloc((void **)&c->h_meta, sizeof(StepMeta), hipHostMallocDefault) != hipSuccess)
+  if (!s && hipHostMalloc((void **)&c->h_meta, 3 * sizeof(StepMeta), hipHostMallocDefault) != hipSuccess)
     s = fail(c, SV_E_NOMEM, "hipHostMalloc failed");
   if (!s && (hipHostMalloc((void **)&c->h_adam, sizeof(AdamParams), hipHostMallocMapped) != hipSuccess ||
              hipHostGetDevicePointer((void **)&c->h_adam_dev, c->h_adam, 0) != hipSuccess))
@@ -1122,6 +1278,20 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->vals2);
   dfree(c->sort_tmp);
   dfree(c->isect_rec);
+  dfree(c->scounts);
+  dfree(c->soffs);
+  dfree(c->super_view);
+  dfree(c->sranges);
+  dfree(c->diff);
+  dfree(c->skeys);
+  dfree(c->skeys2);
+  dfree(c->svals);
+  dfree(c->svals2);
+  dfree(c->ssort_tmp);
+  dfree(c->nch);
+  dfree(c->cbase);
+  dfree(c->chunk_super);
+  dfree(c->ccount);
   dfree(c->fixed);
   if (c->h_small) (void)hipHostFree(c->h_small);
   if (c->st) (void)hipStreamDestroy(c->st);
@@ -1138,6 +1308,7 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->sh_deg > 3) return fail(c, SV_E_ARG, "sh_deg must be <= 3");
   if (k->plain_wf_flags > 3) return fail(c, SV_E_ARG, "plain_wf_flags must be a combination of SV_WF_*");
   if (k->xcd_remap > 1) return fail(c, SV_E_ARG, "xcd_remap must be 0 or 1");
+  if (k->binned_build > 1) return fail(c, SV_E_ARG, "binned_build must be 0 or 1");
   c->cfg = *k;
   return SV_OK;
 }
@@ -1151,6 +1322,7 @@ sv_status sv_get_config(const sv_ctx *c, sv_config *k) {
 sv_status sv_set_gaussians(sv_ctx *c, size_t n, const float *pos, const float *ls, const float *q,
                            const float *lg, const float *sh) {
   if (sv_status s = check_ctx(c)) return s;
+  c->bin_unfit = false;
   if (!pos || !ls || !q || !lg || !sh) return fail(c, SV_E_ARG, "null parameter pointer");
   if (n > c->cap) return fail(c, SV_E_CAPACITY, "%zu Gaussians exceed capacity %zu", n, c->cap);
   if (n > 0xffffffffull) return fail(c, SV_E_CAPACITY, "too many Gaussians");
@@ -1199,6 +1371,7 @@ sv_status sv_set_step(sv_ctx *c, uint32_t t) {
 
 sv_status sv_set_views(sv_ctx *c, int nv, const sv_camera *cams) {
   if (sv_status s = check_ctx(c)) return s;
+  c->bin_unfit = false;
   if (nv <= 0 || !cams) return fail(c, SV_E_ARG, "need at least one camera");
   HIP_OK(c, hipStreamSynchronize(c->st));
   std::vector<HostView> vs(nv);
@@ -1392,7 +1565,7 @@ sv_status sv_step(sv_ctx *c, const int *vs, int nv, int mode, sv_step_stats *sta
     stats->n_views = (uint32_t)nv;
     stats->sort_passes = (uint32_t)c->passes;
     stats->key_bits = c->key_bits;
-    stats->pad = 0;
+    stats->binned = c->last_binned ? 1u : 0u;
     stats->l1_sum = l1;
   }
   return SV_OK;
@@ -1444,7 +1617,20 @@ sv_status sv_read_keys(sv_ctx *c, int v, uint32_t *keys, uint32_t *vals) {
   HIP_OK(c, hipStreamSynchronize(c->st));
   const uint64_t I = c->I;
   if (vals && I) HIP_OK(c, hipMemcpy(vals, cur_vals(c), I * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  if (keys && I) {
+  if (keys && I && c->last_binned) {
+    // binned build: no key array; the key of entry i of tile t is (t,
+    // depth_q of its splat), depth_q formed as emit_intersections does
+    std::vector<uint32_t> rg((size_t)c->tiles_total + 1), vv(I);
+    std::vector<float2> ax(c->N);
+    HIP_OK(c, hipMemcpy(rg.data(), c->ranges, rg.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_OK(c, hipMemcpy(vv.data(), cur_vals(c), I * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_OK(c, hipMemcpy(ax.data(), c->aux, (size_t)c->N * sizeof(float2), hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < c->tiles_total; ++t)
+      for (uint32_t i = rg[t]; i < rg[t + 1] && i < I; ++i) {
+        const float d = std::fmin(std::fmax(ax[std::min(vv[i], c->N - 1)].x, 0.0f), 65535.0f);
+        keys[i] = ((t & 0xfffu) << 16) | ((uint32_t)d & 0xffffu);
+      }
+  } else if (keys && I) {
     const uint64_t dmask = c->dbits ? ((1ull << c->dbits) - 1ull) : 0ull;
     if (c->key64) {
       std::vector<uint64_t> k(I);
@@ -1694,6 +1880,65 @@ sv_status sv_knn_init(sv_ctx *c, float scene_extent, float *density) {
   if (e != hipSuccess) return fail(c, SV_E_HIP, "knn_init: %s", hipGetErrorString(e));
   c->built = false;
   c->forward_done = false;
+  return SV_OK;
+}
+
+sv_status sv_multiview_scores(sv_ctx *c, const int *vs, int nv, uint32_t *importance, float *pruning,
+                              int *n_scored) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!vs || nv <= 0 || !importance || !pruning) return fail(c, SV_E_ARG, "views and outputs are required");
+  if (n_scored) *n_scored = 0;
+  std::vector<int> use;
+  for (int i = 0; i < nv; ++i) {
+    if (sv_status s = check_view(c, vs[i])) return s;
+    if (c->views[vs[i]].has_target) use.push_back(vs[i]);
+  }
+  const uint32_t N = c->N;
+  if (use.empty() || N == 0) {
+    std::fill(importance, importance + N, 0u);
+    std::fill(pruning, pruning + N, 0.0f);
+    return SV_OK;
+  }
+  const int k = (int)use.size();
+  if (sv_status s = prepare_batch(c, use.data(), k)) return s;
+  if (sv_status s = build_batch(c)) return s;
+  if (sv_status s = forward_batch(c)) return s;
+  uint32_t max_pixels = 0;
+  for (int i = 0; i < k; ++i) max_pixels = std::max(max_pixels, c->h_slots[i].W * c->h_slots[i].H);
+  float *loss = nullptr, *pr = nullptr;
+  MetricStats *stats = nullptr;
+  uint32_t *counts = nullptr, *imp = nullptr;
+  auto release = [&]() {
+    dfree(loss);
+    dfree(pr);
+    dfree(stats);
+    dfree(counts);
+    dfree(imp);
+  };
+  sv_status s = dalloc(c, &loss, c->pix_total);
+  if (!s) s = dalloc(c, &stats, (size_t)k);
+  if (!s) s = dalloc(c, &counts, (size_t)k * N);
+  if (!s) s = dalloc(c, &imp, N);
+  if (!s) s = dalloc(c, &pr, N);
+  if (s) {
+    release();
+    return s;
+  }
+  launch_multiview_scores(c->st, c->d_views, k, max_pixels, c->color, c->target, c->ranges, cur_vals(c), c->last,
+                          c->rec, N, loss, stats, counts, imp, pr);
+  hipError_t e = hipGetLastError();
+  std::vector<MetricStats> hs(k);
+  if (e == hipSuccess) e = hipMemcpyAsync(importance, imp, (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(pruning, pr, (size_t)N * sizeof(float), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), stats, k * sizeof(MetricStats), hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  release();
+  if (e != hipSuccess) return fail(c, SV_E_HIP, "multiview scores: %s", hipGetErrorString(e));
+  if (n_scored) {
+    int ns = 0;
+    for (const MetricStats &m : hs) ns += m.valid ? 1 : 0;
+    *n_scored = ns;
+  }
   return SV_OK;
 }
 

@@ -38,19 +38,29 @@ template <typename K>
 void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap, uint32_t n_tiles,
                    const StepMeta *m, uint32_t *ranges);
 // Binned build (sv_bin.hip): pass info of the supertile pairs from m[0]
-// and m[1].n_isect; supertile pairs of each item plus the tile-count
-// difference array; per-tile counts by row / column prefix sums; tile
-// lists from the sorted supertile pairs.
+// and m[1].n_isect; supertile pairs of each item; tile counts, ranges and
+// tile lists from the sorted supertile pairs (m[2], m[3]: scan totals).
 void launch_bin_info(hipStream_t st, StepMeta *m, uint32_t super_bits, uint64_t cap_super);
 void launch_emit_super(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
                        const uint32_t *offs, const uint32_t *scounts, const uint32_t *soffs, uint32_t N, int nv,
-                       const DevView *views, const StepMeta *m, uint32_t *diff, uint32_t *keys, uint32_t *vals);
-void launch_tile_counts(hipStream_t st, const DevView *views, int nv, uint32_t max_tiles_x,
-                        uint32_t max_tiles_y, uint32_t *diff);
-void launch_fill_tiles(hipStream_t st, const uint32_t *keys0, const uint32_t *vals0, const uint32_t *keys1,
-                       const uint32_t *vals1, const uint32_t *sranges, const uint32_t *super_view,
-                       uint32_t n_super_tiles, const DevView *views, const StepMeta *m, const uint32_t *ranges,
-                       uint32_t *out);
+                       const DevView *views, const StepMeta *m, uint32_t *keys, uint32_t *vals);
+struct BinArgs {
+  const uint32_t *keys0, *vals0, *keys1, *vals1;  // supertile pairs, sorted into buffer m[1].sel
+  const uint32_t *sranges;      // per global supertile + 1: lower bound in the sorted pairs
+  const uint32_t *super_view;   // global supertile -> slot
+  const DevView *views;
+  StepMeta *m;                  // m[0..3]
+  uint32_t n_super_tiles, n_tiles;
+  size_t chunk_cap;             // bin_chunk_cap(super pair capacity, n_super_tiles)
+  uint32_t *nch, *cbase;        // per supertile + 1: chunks, first chunk
+  uint32_t *chunk_super;        // per chunk: supertile
+  uint32_t *ccount;             // per chunk x 64 tiles
+  uint32_t *tcount;             // per global tile + 1
+  uint64_t *scan_tmp;           // scan_tmp_words(max(n_tiles, n_super_tiles) + 1)
+  uint32_t *ranges, *out;       // tile ranges (n_tiles + 1) and lists (pair capacity)
+};
+size_t bin_chunk_cap(uint64_t super_cap, uint32_t n_super_tiles);
+void launch_fill_tiles(hipStream_t st, const BinArgs &b);
 // Residual of training_loss (training-loss.wgsl:86-126) for one slot.
 void launch_loss(hipStream_t st, const DevView &hv, const DevConfig &cfg, const float *color,
                  const float *target, float *residual);
@@ -172,6 +182,18 @@ void launch_densify(hipStream_t st, const float *P, const float *M1, const float
 void launch_u8_to_u32(hipStream_t st, const uint8_t *a, uint32_t *o, uint32_t N, int mode);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_f32(hipStream_t st, float *p, float v, size_t n);
+
+// sv_score.hip ----------------------------------------------------------
+// Multi-view scoring (generateMultiViewScores) of the rendered batch.
+struct MetricStats {
+  float min, max, inv_range;
+  uint32_t valid;  // the view has a finite metric value
+  double mean;
+};
+void launch_multiview_scores(hipStream_t st, const DevView *views, int nv, uint32_t max_pixels,
+                             const float *color, const float *target, const uint32_t *ranges, const uint32_t *vals,
+                             const uint32_t *last, const float4 *rec, uint32_t N, float *loss, MetricStats *stats,
+                             uint32_t *counts, uint32_t *importance, float *pruning);
 
 // sv_knn.hip ------------------------------------------------------------
 // knn_init (knn-init.wgsl:40-90) by an exact grid search: cells per axis

@@ -20,9 +20,11 @@ constexpr int SORT_TILE = SORT_T * SORT_ITEMS;  // 4096 keys per workgroup
 constexpr int NWAVE = SORT_T / 64;
 
 // Pairs, tiles and whether pass `pass` runs, from the step metadata.
+// dmask keeps the digit's bits below key_bits: bits above the key carry a
+// payload (the binned build's tile rectangle) and must not order the pairs.
 struct PassInfo {
   uint64_t n;
-  uint32_t nb, shift;
+  uint32_t nb, shift, dmask;
   bool run;
 };
 __device__ __forceinline__ PassInfo pass_info(const StepMeta *m, uint32_t pass) {
@@ -31,6 +33,8 @@ __device__ __forceinline__ PassInfo pass_info(const StepMeta *m, uint32_t pass) 
   p.run = pass < m->passes;
   p.nb = (uint32_t)((p.n + SORT_TILE - 1) / SORT_TILE);
   p.shift = 8u * pass;
+  const uint32_t live = m->key_bits > p.shift ? m->key_bits - p.shift : 0u;
+  p.dmask = live >= 8u ? 0xffu : ((1u << live) - 1u);
   return p;
 }
 
@@ -41,7 +45,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
   const PassInfo pi = pass_info(m, pass);
   if (!pi.run || blockIdx.x >= pi.nb) return;
   const uint64_t n = pi.n;
-  const uint32_t shift = pi.shift, nb = pi.nb;
+  const uint32_t shift = pi.shift, nb = pi.nb, dmask = pi.dmask & (RADIX - 1);
   __shared__ uint32_t s_h[NWAVE][RADIX];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < NWAVE * RADIX; i += SORT_T) (&s_h[0][0])[i] = 0u;
@@ -56,7 +60,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
     for (int j = 0; j < SORT_ITEMS; ++j) {
       const uint64_t i = base + (uint64_t)j * 64 + lane;
       const bool valid = i < n;
-      const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & (RADIX - 1) : 0u;
+      const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & dmask : 0u;
       uint64_t mk = __ballot(valid);
 #pragma unroll
       for (int bit = 0; bit < 4; ++bit) {
@@ -70,7 +74,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
 #pragma unroll
     for (int j = 0; j < SORT_ITEMS; ++j) {
       const uint64_t i = base + (uint64_t)j * 64 + lane;
-      if (i < n) atomicAdd(&s_h[wave][(uint32_t)(keys[i] >> shift) & (RADIX - 1)], 1u);
+      if (i < n) atomicAdd(&s_h[wave][(uint32_t)(keys[i] >> shift) & dmask], 1u);
     }
   }
   __syncthreads();
@@ -156,7 +160,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
   const PassInfo pi = pass_info(m, pass);
   if (!pi.run || blockIdx.x >= pi.nb) return;
   const uint64_t n = pi.n;
-  const uint32_t shift = pi.shift, nb = pi.nb;
+  const uint32_t shift = pi.shift, nb = pi.nb, dmask = pi.dmask & (RADIX - 1);
   __shared__ uint32_t s_wcnt[NWAVE][RADIX];
   __shared__ uint32_t s_lbase[RADIX];
   __shared__ uint32_t s_gbase[RADIX];
@@ -181,7 +185,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
     const bool valid = i < n;
     key[j] = valid ? kin[i] : (K)0;
     val[j] = valid ? vin[i] : 0u;
-    const uint32_t d = (uint32_t)(key[j] >> shift) & (RADIX - 1);
+    const uint32_t d = (uint32_t)(key[j] >> shift) & dmask;
     // lanes holding the same digit: AND of ballots over the 8 digit bits
     uint64_t m = __ballot(valid);
 #pragma unroll
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
   for (int j = 0; j < SORT_ITEMS; ++j) {
     const uint64_t i = base + (uint64_t)j * 64 + lane;
     if (i < n) {
-      const uint32_t d = (uint32_t)(key[j] >> shift) & (RADIX - 1);
+      const uint32_t d = (uint32_t)(key[j] >> shift) & dmask;
       const uint32_t lpos = s_lbase[d] + s_wcnt[wave][d] + rank[j];
       if (lpos < (uint32_t)SORT_TILE) {
         s_keys[lpos] = key[j];
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
   const uint32_t cnt = (uint32_t)((n - tile0) < (uint64_t)SORT_TILE ? (n - tile0) : SORT_TILE);
   for (uint32_t k = threadIdx.x; k < cnt; k += SORT_T) {
     const K kk = s_keys[k];
-    const uint32_t d = (uint32_t)(kk >> shift) & (RADIX - 1);
+    const uint32_t d = (uint32_t)(kk >> shift) & dmask;
     const uint32_t g = s_gbase[d] + (k - s_lbase[d]);
     if ((uint64_t)g < n) {
       kout[g] = kk;

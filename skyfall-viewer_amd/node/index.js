@@ -167,6 +167,11 @@ function createGaussianRenderer(pointCloud, options = {}) {
     // initial log-scales from the 3 nearest neighbours; the trainer's
     // parameters are the device masters, so there is nothing to sync.
     async initializeKnnAndSync(sceneExtent = options.sceneExtent || 1.0) { return native.knnInit(ctx.handle, sceneExtent); },
+    // generateMultiViewScores (gaussian-renderer.ts:2754-2781) over the given
+    // views (the reference samples MULTI_VIEW_SAMPLE_COUNT = 10 at random)
+    async getMultiViewScores(views = cameras.map((_, i) => i).slice(0, 10)) {
+      return native.multiViewScores(ctx.handle, views);
+    },
     async densify({ views, percentile = 0.9, splitPx = 8, seed = iteration } = {}) {
       const d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx);
       native.densify(ctx.handle, d.actions, seed);

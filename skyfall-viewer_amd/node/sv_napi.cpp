@@ -133,7 +133,8 @@ const Field kFields[] = {FU(tile_size), FU(batch), FU(raster_px), FU(sh_deg), FF
                          FF(radius_clip), FF(eps2d), FF(w_l2), FF(w_l1), FF(w_ssim), FF(lr_pos), FF(lr_sh),
                          FF(lr_opacity), FF(lr_scale), FF(lr_rot), FF(beta1), FF(beta2), FF(eps),
                          FU(sh_update_interval), FU(pos_lr_schedule), FU(render_f16), FF(importance_decay),
-                         FU(persistent), FU(use_graph), FU(keep_grads), FU(plain_wf_flags), FU(xcd_remap)};
+                         FU(persistent), FU(use_graph), FU(keep_grads), FU(plain_wf_flags), FU(xcd_remap),
+                         FU(binned_build)};
 
 napi_value config_to_js(napi_env env, const sv_config &k) {
   napi_value o;
@@ -727,6 +728,32 @@ napi_value KnnInit(napi_env env, napi_callback_info info) {
   return arr;
 }
 
+// multiViewScores(ctx, views) -> {importance: Uint32Array, pruning: Float32Array,
+// sampleCount} (generateMultiViewScores)
+napi_value MultiViewScores(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  sv_ctx *c = ctx_of(env, a.v[0]);
+  if (!c) return nullptr;
+  std::vector<int> vs;
+  if (!views_from_js(env, a.v[1], vs)) return nullptr;
+  const size_t n = sv_get_count(c);
+  uint32_t *imp = nullptr;
+  float *pru = nullptr;
+  napi_value ai = new_typed<uint32_t>(env, napi_uint32_array, n, &imp);
+  napi_value ap = new_typed<float>(env, napi_float32_array, n, &pru);
+  if (!ai || !ap) return nullptr;
+  int ns = 0;
+  SV(c, sv_multiview_scores(c, vs.data(), (int)vs.size(), imp, pru, &ns), "sv_multiview_scores");
+  napi_value o, k;
+  CK(napi_create_object(env, &o));
+  CK(napi_set_named_property(env, o, "importance", ai));
+  CK(napi_set_named_property(env, o, "pruning", ap));
+  CK(napi_create_int32(env, ns, &k));
+  CK(napi_set_named_property(env, o, "sampleCount", k));
+  return o;
+}
+
 napi_value Densify(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
@@ -945,7 +972,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"stepAsync", StepAsync},     {"readImage", ReadImage},     {"readResidual", ReadResidual},
              {"readAlpha", ReadAlpha},     {"readLastIds", ReadLastIds}, {"readGrads", ReadGrads},
              {"readKeys", ReadKeys},       {"pruneOpacity", PruneOpacity}, {"pruneMask", PruneMask},
-             {"densify", Densify},         {"resetOpacity", ResetOpacity}, {"knnInit", KnnInit},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
+             {"densify", Densify},         {"resetOpacity", ResetOpacity}, {"knnInit", KnnInit},
+             {"multiViewScores", MultiViewScores},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
              {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},
              {"saveCheckpoint", SaveCheckpoint}, {"loadCheckpoint", LoadCheckpoint},               {"sceneGaussians", SceneGaussians},
              {"sceneJitter", SceneJitter}, {"sceneCamera", SceneCamera}};

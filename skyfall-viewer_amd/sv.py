@@ -37,13 +37,13 @@ class Config(C.Structure):
         ("sh_update_interval", C.c_uint32), ("pos_lr_schedule", C.c_uint32),
         ("render_f16", C.c_uint32), ("importance_decay", C.c_float),
         ("persistent", C.c_uint32), ("use_graph", C.c_uint32), ("keep_grads", C.c_uint32),
-        ("plain_wf_flags", C.c_uint32), ("xcd_remap", C.c_uint32),
+        ("plain_wf_flags", C.c_uint32), ("xcd_remap", C.c_uint32), ("binned_build", C.c_uint32),
     ]
 
 
 class StepStats(C.Structure):
     _fields_ = [("n_isect", C.c_uint64), ("n_views", C.c_uint32), ("sort_passes", C.c_uint32),
-                ("key_bits", C.c_uint32), ("pad", C.c_uint32), ("l1_sum", C.c_double)]
+                ("key_bits", C.c_uint32), ("binned", C.c_uint32), ("l1_sum", C.c_double)]
 
 
 class TuneEntry(C.Structure):
@@ -104,6 +104,7 @@ _SIGS = {
     "sv_densify": (C.c_int, [C.c_void_p, _U8, C.c_uint64, C.POINTER(C.c_size_t)]),
     "sv_reset_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
     "sv_knn_init": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
+    "sv_multiview_scores": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, _U32, _P, C.POINTER(C.c_int)]),
     "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, _U8,
                                     C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
@@ -534,6 +535,17 @@ class Context:
         dens = np.zeros(self.count, np.float32)
         self._ck(lib().sv_knn_init(self.h, scene_extent, dens.ctypes.data if dens.size else None), "knn_init")
         return dens
+
+    def multiview_scores(self, views):
+        """generateMultiViewScores over `views` in order: (importance u32[N],
+        pruning f32[N], views scored)."""
+        vs = (C.c_int * len(views))(*views)
+        imp = np.zeros(self.count, np.uint32)
+        pru = np.zeros(self.count, np.float32)
+        ns = C.c_int()
+        self._ck(lib().sv_multiview_scores(self.h, vs, len(views), imp.ctypes.data_as(_U32), _fp(pru),
+                                           C.byref(ns)), "multiview_scores")
+        return imp, pru, ns.value
 
     def densify_decide(self, views, percentile: float = 0.9, split_px: float = 8.0):
         """Returns (actions uint8[N], n_clone, n_split)."""

@@ -241,10 +241,12 @@ def test_batched_sort_1080p(sv):
             for v in range(2):
                 c.set_target(v, targets[v])
             st = c.step_views([0, 1], mode, stats=True)
-            res[mode] = ([c.image(v) for v in range(2)], c.grads(), st.key_bits, st.sort_passes)
-    (iq, gq, bq, pq), (ib, gb, bb, pb) = res[sv.MODE_SEQUENTIAL], res[sv.MODE_BATCHED]
-    print(f"[parity] 1080p x2: sequential key bits {bq} passes {pq}; batched key bits {bb} passes {pb}", flush=True)
-    assert bb >= 17 and pb == (bb + 7) // 8 and pq == (bq + 7) // 8
+            res[mode] = ([c.image(v) for v in range(2)], c.grads(), st.key_bits, st.sort_passes, st.binned)
+    (iq, gq, bq, pq, nq), (ib, gb, bb, pb, nb) = res[sv.MODE_SEQUENTIAL], res[sv.MODE_BATCHED]
+    print(f"[parity] 1080p x2: sequential key bits {bq} passes {pq}; batched key bits {bb} passes {pb} "
+          f"(binned {nq}, {nb})", flush=True)
+    # binned builds sort supertile pairs (fewer passes); pair sorts need every key bit
+    assert bb >= 17 and (nb or pb == (bb + 7) // 8) and (nq or pq == (bq + 7) // 8)
     for v in range(2):
         assert np.array_equal(iq[v], ib[v])
     for k in ("pos", "logscale", "quat", "logit", "sh"):
