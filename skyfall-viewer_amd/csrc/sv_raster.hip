@@ -366,7 +366,7 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
 // a.residual.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(WT) void k_backward(RasterArgs ain) {
+__global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
   RasterArgs a = ain;
   if (!resolve_meta(a)) return;
   __shared__ float4 s_q0[B];
