@@ -10,7 +10,8 @@ gradients summed, one Adam step over all 59 parameters of every Gaussian.
 
 value      = steps/s of the batched adjoint path (all views in one pipeline)
 plain      = the same step with the views run one after another
-keys/s     = intersections sorted per step / time of the sort kernels
+keys/s     = intersections per step / time to build the depth-ordered tile
+             lists from the projections (emit + sort + ranges + fill)
 roofline   = the dominant kernel's algorithmic bytes per launch / its mean
              launch time (hipEvents on the library stream), vs 8 TB/s
 cpu_baseline = the CPU oracle on one of the 16 views, scaled to a step
