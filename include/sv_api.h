@@ -212,6 +212,15 @@ sv_status sv_step(sv_ctx *ctx, const int *views, int nv, int mode,
                   sv_step_stats *stats);
 /* Forward only for nv views in one batched pipeline. */
 sv_status sv_render_batched(sv_ctx *ctx, const int *views, int nv);
+/* The viewer path on the training pipeline (in place of the instanced-quad
+ * render of preprocess.wgsl:116-222 / gaussian.wgsl:15-67,
+ * gaussian-renderer.ts:2217-2259: the tile lists and the tiled forward of
+ * training-forward-tiles.wgsl:107-215): renders the current Gaussians from
+ * any camera, not one of the training views, with the training forward's
+ * rules (alpha <= 0.999, 1/255 cut, early stop, background). rgb: H x W x 3
+ * floats; alpha: H x W accumulated alpha, or NULL. The training views, their
+ * images and targets are untouched. */
+sv_status sv_render_camera(sv_ctx *ctx, const sv_camera *cam, float *rgb, float *alpha);
 
 /* --- readbacks --------------------------------------------------------- */
 sv_status sv_read_image(sv_ctx *ctx, int view, float *rgb);

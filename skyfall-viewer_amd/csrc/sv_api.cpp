@@ -155,6 +155,11 @@ struct sv_ctx {
   int passes = 0;
   int sel = 0;  // sorted pairs in (keys, vals) = 0 or (keys2, vals2) = 1
 
+  // viewer render target (sv_render_camera)
+  float *vw_color = nullptr, *vw_alpha = nullptr;
+  uint32_t *vw_last = nullptr;
+  size_t vw_cap = 0;
+
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1288,6 +1293,9 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->svals);
   dfree(c->svals2);
   dfree(c->ssort_tmp);
+  dfree(c->vw_color);
+  dfree(c->vw_alpha);
+  dfree(c->vw_last);
   dfree(c->nch);
   dfree(c->cbase);
   dfree(c->chunk_super);
@@ -1577,6 +1585,62 @@ sv_status sv_render_batched(sv_ctx *c, const int *vs, int nv) {
   if (sv_status s = build_batch(c)) return s;
   if (sv_status s = forward_batch(c)) return s;
   return sync(c);
+}
+
+sv_status sv_render_camera(sv_ctx *c, const sv_camera *cam, float *rgb, float *alpha) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!cam || !rgb) return fail(c, SV_E_ARG, "camera and output are required");
+  if (c->N == 0) return fail(c, SV_E_STATE, "no Gaussians");
+  const uint32_t ts = c->cfg.tile_size;
+  const double w = std::round((double)cam->viewport[0]), hh = std::round((double)cam->viewport[1]);
+  if (!(w >= 1.0) || !(hh >= 1.0) || w > 65535.0 * ts || hh > 65535.0 * ts)
+    return fail(c, SV_E_ARG, "camera has an invalid viewport");
+  HostView h{};
+  h.cam = *cam;
+  h.W = (uint32_t)w;
+  h.H = (uint32_t)hh;
+  h.tiles_x = (h.W + ts - 1) / ts;
+  h.tiles_y = (h.H + ts - 1) / ts;
+  h.n_tiles = h.tiles_x * h.tiles_y;
+  h.pix_off = 0;
+  h.has_target = false;
+  const size_t P = (size_t)h.W * h.H;
+  if (P > c->vw_cap || !c->vw_color) {
+    dfree(c->vw_color);
+    dfree(c->vw_alpha);
+    dfree(c->vw_last);
+    c->vw_cap = 0;
+    if (sv_status s = dalloc(c, &c->vw_color, 3 * P)) return s;
+    if (sv_status s = dalloc(c, &c->vw_alpha, P)) return s;
+    if (sv_status s = dalloc(c, &c->vw_last, P)) return s;
+    c->vw_cap = P;
+  }
+  // The camera renders as one more view whose pixels live in the viewer
+  // buffers; the training views, their pixels and the last build are left
+  // as they were (the next step rebuilds its batch).
+  const int vid = (int)c->views.size();
+  c->views.push_back(h);
+  c->residual_ready.push_back(0);
+  std::swap(c->color, c->vw_color);
+  std::swap(c->alpha, c->vw_alpha);
+  std::swap(c->last, c->vw_last);
+  sv_status s = prepare_batch(c, &vid, 1);
+  if (!s) s = build_batch(c);
+  if (!s) s = forward_batch(c);
+  if (!s) s = read_pixels3(c, c->color, vid, rgb);
+  if (!s && alpha) {
+    hipError_t e = hipMemcpy(alpha, c->alpha, P * sizeof(float), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) s = fail(c, SV_E_HIP, "viewer alpha readback: %s", hipGetErrorString(e));
+  }
+  std::swap(c->color, c->vw_color);
+  std::swap(c->alpha, c->vw_alpha);
+  std::swap(c->last, c->vw_last);
+  c->views.pop_back();
+  c->residual_ready.pop_back();
+  c->slot_views.clear();
+  c->built = false;
+  c->forward_done = false;
+  return s;
 }
 
 sv_status sv_read_image(sv_ctx *c, int v, float *rgb) {

@@ -238,6 +238,22 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
   return kept;
 }
 
+// Alpha of one splat record (q0, q1, q2) at pixel centre (fx, fy), and
+// whether the pixel takes it (training-forward-tiles.wgsl:183-197): power
+// = 0.5 (a dx^2 + c dy^2) + b dx dy, skipped if < 0; alpha = min(opacity
+// e^-power, 0.999), skipped if < 1/255. The extent and power_max tests
+// before the exp only drop pairs whose alpha is below 1/255.
+__device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0, const float4 &x1,
+                                           const float4 &x2, float *as) {
+  const float dx = fx - x0.x;
+  const float dy = fy - x0.y;
+  const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+  const bool in = !(fabsf(dx) > x2.z || fabsf(dy) > x2.w) && !(sg < 0.0f || sg > x2.y);
+  const float gauss = sv_expf_neg(-sg);
+  *as = fminf(x1.y * gauss, SV_MAX_ALPHA);
+  return in && !(*as < SV_MIN_ALPHA);
+}
+
 // ---------------------------------------------------------------------
 // training_tiled_forward (training-forward-tiles.wgsl:107-215). Per pixel
 // (centre +0.5) over the tile list front to back: power = 0.5(a dx^2 +
@@ -297,35 +313,68 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
         const uint32_t cnt = min((uint32_t)B, end - b0);
         const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0);
         __syncthreads();
-        for (uint32_t e = 0; e < kept; ++e) {
-          // same word on every lane: the quadrant tests below are uniform
-          const uint32_t ke = __builtin_amdgcn_readfirstlane(s_k[e]);
-          const float4 x0 = s_q0[e];
-          const float4 x1 = s_q1[e];
-          const float4 x2 = s_q2[e];
+        // Two entries per iteration: their alphas are independent (only the
+        // compositing that follows is ordered), so the two exp chains
+        // interleave. Per pixel the same operations run in the same order.
+        uint32_t e = 0;
+        for (; e + 1u < kept; e += 2u) {
+          // same words on every lane: the quadrant tests below are uniform
+          const uint32_t ka = __builtin_amdgcn_readfirstlane(s_k[e]);
+          const uint32_t kb = __builtin_amdgcn_readfirstlane(s_k[e + 1u]);
+          const float4 a0 = s_q0[e], a1 = s_q1[e], a2 = s_q2[e];
+          const float4 b0r = s_q0[e + 1u], b1 = s_q1[e + 1u], b2 = s_q2[e + 1u];
 #pragma unroll
           for (int j = 0; j < PPL; ++j) {
-            if (!((ke >> (16 + j)) & 1u)) continue;
+            const bool ua = (ka >> (16 + j)) & 1u, ub = (kb >> (16 + j)) & 1u;
+            if (!(ua || ub)) continue;
             if (done[j]) continue;
-            const float dx = fxh[j & 1] - x0.x;
-            const float dy = fyh[j >> 1] - x0.y;
-            if (fabsf(dx) > x2.z || fabsf(dy) > x2.w) continue;
-            const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-            if (sg < 0.0f || sg > x2.y) continue;
-            const float gauss = sv_expf_neg(-sg);
-            const float as = fminf(x1.y * gauss, SV_MAX_ALPHA);
-            if (as < SV_MIN_ALPHA) continue;
-            const float contrib = as * (1.0f - A[j]);
-            c0[j] = c0[j] + x1.z * contrib;
-            c1[j] = c1[j] + x1.w * contrib;
-            c2[j] = c2[j] + x2.x * contrib;
-            A[j] = A[j] + contrib;
-            if (1.0f - A[j] < SV_T_EPS) {
-              done[j] = true;
-              last[j] = b0 + (ke & 0xFFFFu);
+            float asa, asb;
+            const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, a2, &asa);
+            const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, b2, &asb);
+            if (va) {
+              const float contrib = asa * (1.0f - A[j]);
+              c0[j] = c0[j] + a1.z * contrib;
+              c1[j] = c1[j] + a1.w * contrib;
+              c2[j] = c2[j] + a2.x * contrib;
+              A[j] = A[j] + contrib;
+              if (1.0f - A[j] < SV_T_EPS) {
+                done[j] = true;
+                last[j] = b0 + (ka & 0xFFFFu);
+              }
+            }
+            if (vb && !done[j]) {
+              const float contrib = asb * (1.0f - A[j]);
+              c0[j] = c0[j] + b1.z * contrib;
+              c1[j] = c1[j] + b1.w * contrib;
+              c2[j] = c2[j] + b2.x * contrib;
+              A[j] = A[j] + contrib;
+              if (1.0f - A[j] < SV_T_EPS) {
+                done[j] = true;
+                last[j] = b0 + (kb & 0xFFFFu);
+              }
             }
           }
           if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
+        }
+        if (e + 1u == kept) {
+          const uint32_t ka = __builtin_amdgcn_readfirstlane(s_k[e]);
+          const float4 a0 = s_q0[e], a1 = s_q1[e], a2 = s_q2[e];
+#pragma unroll
+          for (int j = 0; j < PPL; ++j) {
+            if (!((ka >> (16 + j)) & 1u)) continue;
+            if (done[j]) continue;
+            float asa;
+            if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, a2, &asa)) continue;
+            const float contrib = asa * (1.0f - A[j]);
+            c0[j] = c0[j] + a1.z * contrib;
+            c1[j] = c1[j] + a1.w * contrib;
+            c2[j] = c2[j] + a2.x * contrib;
+            A[j] = A[j] + contrib;
+            if (1.0f - A[j] < SV_T_EPS) {
+              done[j] = true;
+              last[j] = b0 + (ka & 0xFFFFu);
+            }
+          }
         }
         if (__syncthreads_and((done[0] && done[1]) && (done[2] && done[3]))) break;
       }

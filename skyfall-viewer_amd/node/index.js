@@ -163,6 +163,9 @@ function createGaussianRenderer(pointCloud, options = {}) {
       return native.pruneOpacity(ctx.handle, minOpacity);
     },
     async resetOpacity(maxOpacity = 0.1) { native.resetOpacity(ctx.handle, maxOpacity); },
+    // A live view of training from any camera (the viewer's render, drawn
+    // with the training tile pipeline): {image, alpha, width, height}.
+    renderCamera(camera) { return native.renderCamera(ctx.handle, camera); },
     // initializeKnnAndSync (gaussian-renderer.ts:2159-2172): density-aware
     // initial log-scales from the 3 nearest neighbours; the trainer's
     // parameters are the device masters, so there is nothing to sync.

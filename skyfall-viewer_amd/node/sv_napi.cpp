@@ -9,6 +9,8 @@
 #define NAPI_VERSION 4
 #include <node_api.h>
 
+#include <cmath>
+
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -754,6 +756,31 @@ napi_value MultiViewScores(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// renderCamera(ctx, camera) -> {image: Float32Array(h*w*3), alpha: Float32Array(h*w), width, height}
+napi_value RenderCamera(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 2)) return nullptr;
+  sv_ctx *c = ctx_of(env, a.v[0]);
+  if (!c) return nullptr;
+  sv_camera cam;
+  if (!camera_from_js(env, a.v[1], &cam)) return nullptr;
+  const uint32_t w = (uint32_t)std::lround(cam.viewport[0]), h = (uint32_t)std::lround(cam.viewport[1]);
+  float *rgb = nullptr, *al = nullptr;
+  napi_value img = new_typed<float>(env, napi_float32_array, (size_t)w * h * 3, &rgb);
+  napi_value alp = new_typed<float>(env, napi_float32_array, (size_t)w * h, &al);
+  if (!img || !alp) return nullptr;
+  SV(c, sv_render_camera(c, &cam, rgb, al), "sv_render_camera");
+  napi_value o, vw, vh;
+  CK(napi_create_object(env, &o));
+  CK(napi_set_named_property(env, o, "image", img));
+  CK(napi_set_named_property(env, o, "alpha", alp));
+  CK(napi_create_uint32(env, w, &vw));
+  CK(napi_create_uint32(env, h, &vh));
+  CK(napi_set_named_property(env, o, "width", vw));
+  CK(napi_set_named_property(env, o, "height", vh));
+  return o;
+}
+
 napi_value Densify(napi_env env, napi_callback_info info) {
   Args a;
   if (!get_args(env, info, a, 3)) return nullptr;
@@ -973,7 +1000,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readAlpha", ReadAlpha},     {"readLastIds", ReadLastIds}, {"readGrads", ReadGrads},
              {"readKeys", ReadKeys},       {"pruneOpacity", PruneOpacity}, {"pruneMask", PruneMask},
              {"densify", Densify},         {"resetOpacity", ResetOpacity}, {"knnInit", KnnInit},
-             {"multiViewScores", MultiViewScores},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
+             {"multiViewScores", MultiViewScores}, {"renderCamera", RenderCamera},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
              {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},
              {"saveCheckpoint", SaveCheckpoint}, {"loadCheckpoint", LoadCheckpoint},               {"sceneGaussians", SceneGaussians},
              {"sceneJitter", SceneJitter}, {"sceneCamera", SceneCamera}};

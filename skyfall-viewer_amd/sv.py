@@ -89,6 +89,7 @@ _SIGS = {
     "sv_adam": (C.c_int, [C.c_void_p]),
     "sv_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(StepStats)]),
     "sv_render_batched": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
+    "sv_render_camera": (C.c_int, [C.c_void_p, C.POINTER(Camera), _P, _P]),
     "sv_read_image": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_read_alpha": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_read_last_ids": (C.c_int, [C.c_void_p, C.c_int, _U32]),
@@ -424,6 +425,14 @@ class Context:
         s = StepStats() if stats else None
         self._ck(lib().sv_step(self.h, vs, len(views), mode, C.byref(s) if s is not None else None), "step")
         return s
+
+    def render_camera(self, cam: Camera):
+        """Render any camera through the tile pipeline: (rgb H x W x 3, alpha H x W)."""
+        w, h = int(round(cam.viewport[0])), int(round(cam.viewport[1]))
+        rgb = np.empty((h, w, 3), np.float32)
+        alpha = np.empty((h, w), np.float32)
+        self._ck(lib().sv_render_camera(self.h, C.byref(cam), _fp(rgb), _fp(alpha)), "render_camera")
+        return rgb, alpha
 
     def render_batched(self, views):
         vs = (C.c_int * len(views))(*views)
