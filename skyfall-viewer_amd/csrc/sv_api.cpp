@@ -108,6 +108,7 @@ struct sv_ctx {
   float2 *aux = nullptr;
   uint32_t *counts = nullptr, *offs = nullptr;
   uint32_t *scounts = nullptr, *soffs = nullptr;  // supertile pairs per (slot, Gaussian)
+  uint2 *rects = nullptr;  // tile rectangle per (slot, Gaussian)
   float *vrec = nullptr;  // 9 planes of slots x N: reached-record sums for the VJP
   size_t vg_cap = 0;
   uint64_t *scan_tmp = nullptr;
@@ -417,6 +418,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     dfree(c->offs);
     dfree(c->scounts);
     dfree(c->soffs);
+    dfree(c->rects);
     dfree(c->vrec);
     c->vg_cap = 0;
     if (sv_status s = dalloc(c, &c->vrec, (size_t)REC9 * M)) return s;
@@ -426,6 +428,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     if (sv_status s = dalloc(c, &c->offs, M)) return s;
     if (sv_status s = dalloc(c, &c->scounts, M)) return s;
     if (sv_status s = dalloc(c, &c->soffs, M)) return s;
+    if (sv_status s = dalloc(c, &c->rects, M)) return s;
     c->vg_cap = M;
   }
   const size_t scan_n = std::max(M, (size_t)c->tiles_total + 1);
@@ -522,7 +525,7 @@ sv_status enqueue_build(sv_ctx *c) {
   {
     Scoped t(c, PH_PROJECT);
     launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts, &m->max_dq,
-                   bin ? c->scounts : nullptr);
+                   bin ? c->scounts : nullptr, c->rects);
   }
   {
     Scoped t(c, PH_SCAN);
@@ -538,7 +541,7 @@ sv_status enqueue_build(sv_ctx *c) {
   if (bin) {
     {
       Scoped t(c, PH_EMIT);
-      launch_emit_super(c->st, c->rec, c->aux, c->counts, c->offs, c->scounts, c->soffs, c->N, nv, c->d_views, m,
+      launch_emit_super(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->scounts, c->soffs, c->N, nv, c->d_views, m,
                         c->skeys, c->svals);
     }
     {
@@ -579,10 +582,10 @@ sv_status enqueue_build(sv_ctx *c) {
   {
     Scoped t(c, PH_EMIT);
     if (c->key64)
-      launch_emit<uint64_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
+      launch_emit<uint64_t>(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
                             (uint64_t *)c->keys, c->vals);
     else
-      launch_emit<uint32_t>(c->st, c->rec, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
+      launch_emit<uint32_t>(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
                             (uint32_t *)c->keys, c->vals);
   }
   {
@@ -778,6 +781,7 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.counts = c->counts;
   v.offs = c->offs;
   v.rec = c->rec;
+  v.rects = c->rects;
   v.aux = c->aux;
   v.isect_rec = c->isect_rec;
   v.tile_cut = c->tile_cut;
@@ -984,7 +988,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
                         c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
-                        c->h_adam_dev, c->scounts, c->soffs, c->super_view, c->sranges, c->diff,
+                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
                         c->chunk_super, c->ccount};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
@@ -1285,6 +1289,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->isect_rec);
   dfree(c->scounts);
   dfree(c->soffs);
+  dfree(c->rects);
   dfree(c->super_view);
   dfree(c->sranges);
   dfree(c->diff);

@@ -61,7 +61,8 @@ __global__ void k_bin_info(StepMeta *m, uint32_t super_bits, uint64_t cap_super)
 // its supertile pairs at soffs (supertiles row-major). One workgroup writes the pairs of
 // 256 consecutive items as one contiguous span (binary search of the
 // owning item per position), so the stores are whole lines.
-__global__ __launch_bounds__(BIN_T) void k_emit_super(float4 *__restrict__ rec, const float2 *__restrict__ aux,
+__global__ __launch_bounds__(BIN_T) void k_emit_super(float4 *__restrict__ rec, const uint2 *__restrict__ rects,
+                                                      const float2 *__restrict__ aux,
                                                       const uint32_t *__restrict__ counts,
                                                       const uint32_t *__restrict__ offs,
                                                       const uint32_t *__restrict__ scounts,
@@ -89,10 +90,9 @@ __global__ __launch_bounds__(BIN_T) void k_emit_super(float4 *__restrict__ rec, 
     if (cnt) {
       const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
       const DevView &V = views[s];
-      float4 r3 = rec[vg * 4 + 3];
-      const uint32_t a = __float_as_uint(r3.y), b = __float_as_uint(r3.z);
-      r3.x = __uint_as_float(offs[vg]);
-      rec[vg * 4 + 3] = r3;
+      const uint2 rc = rects[vg];
+      const uint32_t a = rc.x, b = rc.y;
+      reinterpret_cast<float *>(rec + vg * 4 + 3)[0] = __uint_as_float(offs[vg]);  // q3.x
       const uint32_t x0 = rect_lo(a), y0 = rect_hi(a), x1 = rect_lo(b), y1 = rect_hi(b);
       const uint32_t sx0 = x0 / BIN_S, sy0 = y0 / BIN_S;
       const uint32_t sx1 = (x1 - 1u) / BIN_S + 1u;
@@ -380,12 +380,12 @@ void launch_bin_info(hipStream_t st, StepMeta *m, uint32_t super_bits, uint64_t 
   hipLaunchKernelGGL(k_bin_info, dim3(1), dim3(64), 0, st, m, super_bits, cap_super);
 }
 
-void launch_emit_super(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+void launch_emit_super(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
                        const uint32_t *offs, const uint32_t *scounts, const uint32_t *soffs, uint32_t N, int nv,
                        const DevView *views, const StepMeta *m, uint32_t *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_emit_super, dim3((uint32_t)((M + BIN_T - 1) / BIN_T)), dim3(BIN_T), 0, st, rec, aux,
+  hipLaunchKernelGGL(k_emit_super, dim3((uint32_t)((M + BIN_T - 1) / BIN_T)), dim3(BIN_T), 0, st, rec, rects, aux,
                      counts, offs, scounts, soffs, N, nv, views, m, keys, vals);
 }
 

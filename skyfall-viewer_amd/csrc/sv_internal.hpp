@@ -17,7 +17,7 @@ namespace sv {
 // radius); counts: u32 per (slot, Gaussian).
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq, uint32_t *scounts = nullptr);
+                    uint32_t *max_dq, uint32_t *scounts = nullptr, uint2 *rects = nullptr);
 // Exclusive scan of u32 counts (M entries) into u32 offsets; total (u64)
 // written to *total. tmp must hold scan_tmp_words(M) u64 words.
 size_t scan_tmp_words(size_t M);
@@ -27,8 +27,9 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // the scan total and the largest depth quantum.
 void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap);
 // emit_intersections (training-tiles.wgsl:201-247) for nv slots.
+// rects: per (slot, Gaussian) tile rectangle (x0 | y0 << 16, x1 | y1 << 16).
 template <typename K>
-void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+void launch_emit(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
                  const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
                  K *keys, uint32_t *vals);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
@@ -41,7 +42,7 @@ void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap,
 // and m[1].n_isect; supertile pairs of each item; tile counts, ranges and
 // tile lists from the sorted supertile pairs (m[2], m[3]: scan totals).
 void launch_bin_info(hipStream_t st, StepMeta *m, uint32_t super_bits, uint64_t cap_super);
-void launch_emit_super(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+void launch_emit_super(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
                        const uint32_t *offs, const uint32_t *scounts, const uint32_t *soffs, uint32_t N, int nv,
                        const DevView *views, const StepMeta *m, uint32_t *keys, uint32_t *vals);
 struct BinArgs {
@@ -126,6 +127,7 @@ struct VjpArgs {
   const uint32_t *counts;  // per (slot, Gaussian)
   const uint32_t *offs;
   const float4 *rec;       // projection records
+  const uint2 *rects;      // tile rectangle per record
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
   const float *isect_rec;  // 9-float adjoint record per intersection
   const uint64_t *tile_cut; // per global tile: records at or past this (depth_q, id) were not written

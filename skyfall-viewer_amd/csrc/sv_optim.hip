@@ -244,8 +244,8 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   if (cnt) {
     const uint32_t s = (uint32_t)(vg / a.N), g = (uint32_t)(vg - (size_t)s * a.N);
     const DevView &V = a.views[s];
-    const float4 q3 = a.rec[vg * 4 + 3];
-    const uint32_t ra = __float_as_uint(q3.y), rb = __float_as_uint(q3.z);
+    const uint2 rc = a.rects[vg];
+    const uint32_t ra = rc.x, rb = rc.y;
     const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
     const uint32_t dq = f2u_sat(clampf(a.aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
     const uint64_t me = ((uint64_t)dq << 32) | g;
@@ -276,7 +276,13 @@ void launch_gather_records(hipStream_t st, const VjpArgs &a) {
   hipLaunchKernelGGL(k_gather_records, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, a);
 }
 
-__global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
+// The per-Gaussian backward runs as two kernels with the same arithmetic
+// as one: k_vjp_sh (the 48 SH gradients and their Adam) and then
+// k_vjp_geom (logit, geometry VJP, statistics and the 11 non-SH
+// parameters, whose fused Adam moves the positions k_vjp_sh reads).
+// Split, each holds about half the registers, so twice as many waves hide
+// the memory latency of the plane reads.
+__global__ __launch_bounds__(256) void k_vjp_geom(VjpArgs a) {
   if (a.meta && a.meta->overflow) return;
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.N) return;
@@ -289,9 +295,6 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     prm[k] = a.cfg.render_f16 ? f16_round(v) : v;
   }
   const GeomPre pre = geom_pre(prm);
-  float gsh[48];
-#pragma unroll
-  for (int k = 0; k < 48; ++k) gsh[k] = 0.0f;
   float glogit = 0.0f;
   float gp[3] = {0.f, 0.f, 0.f}, gs[3] = {0.f, 0.f, 0.f}, gq[4] = {0.f, 0.f, 0.f, 0.f};
   bool any = false;
@@ -313,7 +316,6 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
     const float2 ax = a.aux[vg];
     imp.z = fmaxf(imp.z, ax.y);
     any = true;
-    const DevView &V = a.views[s];
     // sum of the Gaussian's reached records of this view (k_gather_records);
     // all zero: no pixel reached it, and every term below is zero
     float r[REC9];
@@ -324,18 +326,7 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
       reached = reached || r[q] != 0.0f;
     }
     if (!reached) continue;
-    float pc[4];
-    m4_mul_pos(V.view, pre.wx, pre.wy, pre.wz, pc);
-    const float nx = -pc[0], ny = -pc[1], nz = -pc[2];
-    const float len = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
-    float b[16];
-    sh_basis(nx / len, ny / len, nz / len, a.cfg.sh_deg, b);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      gsh[3 * k + 0] = gsh[3 * k + 0] + b[k] * r[0];
-      gsh[3 * k + 1] = gsh[3 * k + 1] + b[k] * r[1];
-      gsh[3 * k + 2] = gsh[3 * k + 2] + b[k] * r[2];
-    }
+    const DevView &V = a.views[s];
     glogit = glogit + r[3];
     float dp[3], dl[3], dqq[4];
     if (geom_vjp(V, pre, r[4], r[5], r[6], r[7], r[8], dp, dl, dqq)) {
@@ -357,7 +348,7 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
 #pragma unroll
     for (int q = 0; q < 5; ++q) a.geom[(size_t)q * a.cap + g] = geo[q];
   }
-  float grad[P_NUM];
+  float grad[P_SH];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     grad[P_POS + k] = gp[k];
@@ -366,12 +357,10 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
 #pragma unroll
   for (int k = 0; k < 4; ++k) grad[P_QUAT + k] = gq[k];
   grad[P_LOGIT] = glogit;
-#pragma unroll
-  for (int k = 0; k < 48; ++k) grad[P_SH + k] = gsh[k];
   if (a.G) {
     if (!any) return;
 #pragma unroll
-    for (int k = 0; k < P_NUM; ++k) {
+    for (int k = 0; k < P_SH; ++k) {
       float *dst = a.G + (size_t)k * a.cap + g;
       *dst = *dst + grad[k];
     }
@@ -379,9 +368,8 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
   }
   const AdamParams &H = *a.adam;
 #pragma unroll
-  for (int k = 0; k < P_NUM; ++k) {
+  for (int k = 0; k < P_SH; ++k) {
     const int t = tensor_of_plane(k);
-    if (t == 4 && !H.update_sh) continue;
     const float lr = H.lr[t];
     if (t == 0 && !(lr > 0.0f)) continue;
     const size_t o = (size_t)k * a.cap + g;
@@ -393,9 +381,73 @@ __global__ __launch_bounds__(256) void k_vjp(VjpArgs a) {
   }
 }
 
+// SH chain: grad_sh[k][c] = sum over views of Y_k(dir_view) * sum res * alpha
+// * T (channel c), in view order. A view whose colour sums are all zero adds
+// nothing (its products are zeros), as in the skipped views of k_vjp_geom.
+__global__ __launch_bounds__(256) void k_vjp_sh(VjpArgs a) {
+  if (a.meta && a.meta->overflow) return;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.N) return;
+  const size_t M = (size_t)a.nv * a.N;
+  float w[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float v = a.P[(size_t)(P_POS + k) * a.cap + g];
+    w[k] = a.cfg.render_f16 ? f16_round(v) : v;
+  }
+  float gsh[48];
+#pragma unroll
+  for (int k = 0; k < 48; ++k) gsh[k] = 0.0f;
+  bool any = false;
+  for (int s = 0; s < a.nv; ++s) {
+    const size_t vg = (size_t)s * a.N + g;
+    if (a.counts[vg] == 0u) continue;
+    any = true;
+    const float r0 = a.vrec[vg], r1 = a.vrec[M + vg], r2 = a.vrec[2 * M + vg];
+    if (r0 == 0.0f && r1 == 0.0f && r2 == 0.0f) continue;
+    const DevView &V = a.views[s];
+    float pc[4];
+    m4_mul_pos(V.view, w[0], w[1], w[2], pc);
+    const float nx = -pc[0], ny = -pc[1], nz = -pc[2];
+    const float len = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+    float b[16];
+    sh_basis(nx / len, ny / len, nz / len, a.cfg.sh_deg, b);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      gsh[3 * k + 0] = gsh[3 * k + 0] + b[k] * r0;
+      gsh[3 * k + 1] = gsh[3 * k + 1] + b[k] * r1;
+      gsh[3 * k + 2] = gsh[3 * k + 2] + b[k] * r2;
+    }
+  }
+  if (a.G) {
+    if (!any) return;
+#pragma unroll
+    for (int k = 0; k < 48; ++k) {
+      float *dst = a.G + (size_t)(P_SH + k) * a.cap + g;
+      *dst = *dst + gsh[k];
+    }
+    return;
+  }
+  const AdamParams &H = *a.adam;
+  if (!H.update_sh) return;
+  const float lr = H.lr[4];
+#pragma unroll
+  for (int k = 0; k < 48; ++k) {
+    const size_t o = (size_t)(P_SH + k) * a.cap + g;
+    float p = a.Pw[o], m1 = a.M1[o], m2 = a.M2[o];
+    adam_elem(p, m1, m2, gsh[k], lr, H.beta1, H.beta2, H.eps, H.bc1_rcp, H.bc2_sqrt_rcp);
+    a.Pw[o] = p;
+    a.M1[o] = m1;
+    a.M2[o] = m2;
+  }
+}
+
 void launch_vjp(hipStream_t st, const VjpArgs &a) {
   if (a.N == 0) return;
-  hipLaunchKernelGGL(k_vjp, dim3((a.N + 255) / 256), dim3(256), 0, st, a);
+  // the SH kernel reads the positions (view directions) that the fused Adam
+  // of k_vjp_geom updates: it runs first
+  hipLaunchKernelGGL(k_vjp_sh, dim3((a.N + 255) / 256), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_vjp_geom, dim3((a.N + 255) / 256), dim3(256), 0, st, a);
 }
 
 // run_adam_updates (gaussian-renderer.ts:2455-2545) over the gradient

@@ -35,7 +35,8 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
                                                  float2 *__restrict__ aux,
                                                  uint32_t *__restrict__ counts,
                                                  uint32_t *__restrict__ max_dq,
-                                                 uint32_t *__restrict__ scounts) {
+                                                 uint32_t *__restrict__ scounts,
+                                                 uint2 *__restrict__ rects) {
   // Records of one view for the block's 256 Gaussians are staged in LDS
   // and stored as whole contiguous lines (each store instruction of a wave
   // writes 1 KB), instead of 64-byte pieces 64 bytes apart per lane.
@@ -93,6 +94,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
       float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0, r3 = r0;
       float2 ax = make_float2(0.f, 0.f);
       uint32_t count = 0, scount = 0;
+      uint2 rect = make_uint2(0u, 0u);
       if (alive) {
         const float tx = pc[0], ty = pc[1], tz = pc[2];
         const float fx = V.focal[0], fy = V.focal[1];
@@ -143,6 +145,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           count = (nx <= 0 || ny <= 0) ? 0u : (uint32_t)(nx * ny);
           // supertiles of BIN_S x BIN_S tiles the rectangle touches (binned build)
           if (count) scount = ((x1 - 1u) / BIN_S - x0 / BIN_S + 1u) * ((y1 - 1u) / BIN_S - y0 / BIN_S + 1u);
+          rect = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
           // viewDir = normalize(-(view * pos).xyz), training-forward-tiles.wgsl:185
           const float dxv = -pc[0], dyv = -pc[1], dzv = -pc[2];
           const float len = sqrtf(dot3(dxv, dyv, dzv, dxv, dyv, dzv));
@@ -181,6 +184,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
         aux[vg] = ax;
         counts[vg] = count;
         if (scounts) scounts[vg] = scount;
+        if (rects) rects[vg] = rect;
       }
       __syncthreads();
       // piece j of the block's 1024 float4 belongs to Gaussian
@@ -211,11 +215,11 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
 
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq, uint32_t *scounts) {
+                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects) {
   if (N == 0) return;
   const uint32_t blocks = (N + 255u) / 256u;
   hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, aux,
-                     counts, max_dq, scounts);
+                     counts, max_dq, scounts, rects);
 }
 
 // ---------------------------------------------------------------------
@@ -361,7 +365,7 @@ void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t ke
 // positions in LDS. Stores are coalesced whole lines instead of one
 // sequential run per thread.
 template <typename K>
-__global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
+__global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec, const uint2 *__restrict__ rects,
                                               const float2 *__restrict__ aux,
                                               const uint32_t *__restrict__ counts,
                                               const uint32_t *__restrict__ offs, uint32_t N, int nv,
@@ -387,11 +391,10 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
     if (cnt) {
       const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
       const DevView &V = views[s];
-      float4 r3 = rec[vg * 4 + 3];
-      const uint32_t a = __float_as_uint(r3.y), b = __float_as_uint(r3.z);
+      const uint2 rc = rects[vg];
+      const uint32_t a = rc.x, b = rc.y;
       const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu;
-      r3.x = __uint_as_float(lo);
-      rec[vg * 4 + 3] = r3;
+      reinterpret_cast<float *>(rec + vg * 4 + 3)[0] = __uint_as_float(lo);  // q3.x
       s_rect[t] = x0 | ((x1 - x0) << 16);
       s_y0[t] = y0;
       s_kb[t] = (K)(V.tile_base + y0 * V.tiles_x + x0);
@@ -424,18 +427,18 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec,
 }
 
 template <typename K>
-void launch_emit(hipStream_t st, float4 *rec, const float2 *aux, const uint32_t *counts,
+void launch_emit(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
                  const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
                  K *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, aux, counts,
+  hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, rects, aux, counts,
                      offs, N, nv, views, m, keys, vals);
 }
-template void launch_emit<uint32_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
+template void launch_emit<uint32_t>(hipStream_t, float4 *, const uint2 *, const float2 *, const uint32_t *,
                                     const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
                                     uint32_t *, uint32_t *);
-template void launch_emit<uint64_t>(hipStream_t, float4 *, const float2 *, const uint32_t *,
+template void launch_emit<uint64_t>(hipStream_t, float4 *, const uint2 *, const float2 *, const uint32_t *,
                                     const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
                                     uint64_t *, uint32_t *);
 
