@@ -12,7 +12,7 @@
 // (global supertile, depth_q) and, above the key bits, the splat's tile
 // rectangle clipped to the supertile (3 bits per bound); those pairs are
 // radix-sorted (stable). Each supertile's sorted list is cut into chunks
-// of 1024 pairs. A workgroup per chunk counts, per tile of the supertile,
+// of BIN_CH pairs. A workgroup per chunk counts, per tile of the supertile,
 // the pairs whose rectangle covers it (wave ballots); a prefix over the
 // supertile's chunks gives each chunk its offset within every tile list
 // and each tile its count; a scan of the counts gives the tile ranges; a

@@ -49,7 +49,7 @@ struct DevView {
 // carries its tile rectangle (3 bits per bound) above the key bits.
 constexpr uint32_t BIN_S = 8;
 constexpr uint32_t BIN_RECT_BITS = 12;
-constexpr uint32_t BIN_CH = 1024;  // pairs per chunk of a supertile list
+constexpr uint32_t BIN_CH = 512;  // pairs per chunk of a supertile list
 
 // Values of the current tile build that only the device knows; kernels
 // read them so that no launch depends on a host readback (the batched step
