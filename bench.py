@@ -174,9 +174,11 @@ def cpu_baseline(g, cams, targets, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ring16", "uneven64"), default="ring16",
+    ap.add_argument("--workload", choices=("ring16", "uneven64", "cfg0", "cfg1"), default="ring16",
                     help="ring16: 16 ring views at --width x --height (the headline); uneven64: 64 views "
-                         "with seeded sizes 320-1920 x 240-1080 on a sphere of radius 4, means in [-2,2]^3")
+                         "with seeded sizes 320-1920 x 240-1080 on a sphere of radius 4, means in [-2,2]^3; "
+                         "cfg0: 16,384 splats, one 256x256 view from (0,0,3); cfg1: 500,000 splats, 8 ring "
+                         "views at 1280x720")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -213,6 +215,10 @@ def main():
         sys.exit(2)
 
     N, V, W, H = args.n, args.views, args.width, args.height
+    if args.workload == "cfg0":
+        N, V, W, H = 16384, 1, 256, 256
+    elif args.workload == "cfg1":
+        N, V, W, H = 500_000, 8, 1280, 720
     if args.workload == "uneven64":
         V = 64
         g = sv.scene_gaussians(0, N, 2.0)
@@ -220,6 +226,11 @@ def main():
         P = sum(w * h for w, h in sizes)
         workload = (f"{N} splats in [-2,2]^3, {V} views of seeded sizes 320-1920 x 240-1080 "
                     f"({P} pixels) per step, batched adjoint training step")
+    elif args.workload == "cfg0":
+        g = sv.scene_gaussians(0, N, 1.0)
+        cams = [sv.scene_camera([0.0, 0.0, 3.0], [0.0, 0.0, 0.0], W, H, 60.0)]
+        P = W * H
+        workload = f"{N} splats, 1 view {W}x{H} from (0,0,3), training step"
     else:
         g = sv.scene_gaussians(0, N, 1.0)
         cams = sv.ring_cameras(V, 3.0, W, H)
@@ -287,8 +298,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
         "config": {"workload": workload, "splats": N, "views": V,
-                   "width": W if args.workload == "ring16" else None,
-                   "height": H if args.workload == "ring16" else None, "pixels": P,
+                   "width": W if args.workload != "uneven64" else None,
+                   "height": H if args.workload != "uneven64" else None, "pixels": P,
                    "tile": 16, "lds_batch": chosen_b["batch"],
                    "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"],
                    "binned_build": args.binned},

@@ -109,6 +109,9 @@ struct sv_ctx {
   uint32_t *counts = nullptr, *offs = nullptr;
   uint32_t *scounts = nullptr, *soffs = nullptr;  // supertile pairs per (slot, Gaussian)
   uint2 *rects = nullptr;  // tile rectangle per (slot, Gaussian)
+  // projection-block totals of the tile / supertile counts and their scans
+  uint32_t *btot = nullptr, *sbtot = nullptr, *bbase = nullptr, *sbbase = nullptr;
+  size_t btab_cap = 0;
   float *vrec = nullptr;  // 9 planes of slots x N: reached-record sums for the VJP
   size_t vg_cap = 0;
   uint64_t *scan_tmp = nullptr;
@@ -431,6 +434,20 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     if (sv_status s = dalloc(c, &c->rects, M)) return s;
     c->vg_cap = M;
   }
+  // projection-block tables: slots x blocks per slot (+1 for the scan total)
+  const size_t nbk = (size_t)nv * project_blocks(c->N) + 1;
+  if (nbk > c->btab_cap || !c->btot) {
+    dfree(c->btot);
+    dfree(c->sbtot);
+    dfree(c->bbase);
+    dfree(c->sbbase);
+    c->btab_cap = 0;
+    if (sv_status s = dalloc(c, &c->btot, nbk)) return s;
+    if (sv_status s = dalloc(c, &c->sbtot, nbk)) return s;
+    if (sv_status s = dalloc(c, &c->bbase, nbk)) return s;
+    if (sv_status s = dalloc(c, &c->sbbase, nbk)) return s;
+    c->btab_cap = nbk;
+  }
   const size_t scan_n = std::max(M, (size_t)c->tiles_total + 1);
   if (sv_status s = grow(c, &c->scan_tmp, c->scan_tmp_cap, scan_tmp_words(scan_n))) return s;
   key_plan(c);
@@ -525,13 +542,16 @@ sv_status enqueue_build(sv_ctx *c) {
   {
     Scoped t(c, PH_PROJECT);
     launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts, &m->max_dq,
-                   bin ? c->scounts : nullptr, c->rects);
+                   bin ? c->scounts : nullptr, c->rects, c->offs, bin ? c->soffs : nullptr, c->btot,
+                   bin ? c->sbtot : nullptr);
   }
   {
     Scoped t(c, PH_SCAN);
-    launch_scan(c->st, c->counts, c->offs, M, c->scan_tmp, &m->n_isect);
+    // second level of the scans: over the projection blocks' totals
+    const size_t nbk = (size_t)nv * project_blocks(c->N);
+    launch_scan(c->st, c->btot, c->bbase, nbk, c->scan_tmp, &m->n_isect);
     if (bin) {
-      launch_scan(c->st, c->scounts, c->soffs, M, c->scan_tmp, &m[1].n_isect);
+      launch_scan(c->st, c->sbtot, c->sbbase, nbk, c->scan_tmp, &m[1].n_isect);
       launch_keyinfo(c->st, m, c->tile_bits, 64u, c->isect_cap);
       launch_bin_info(c->st, m, c->super_bits, c->super_cap);
     } else {
@@ -541,7 +561,7 @@ sv_status enqueue_build(sv_ctx *c) {
   if (bin) {
     {
       Scoped t(c, PH_EMIT);
-      launch_emit_super(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->scounts, c->soffs, c->N, nv, c->d_views, m,
+      launch_emit_super(c->st, c->rects, c->aux, c->scounts, c->soffs, c->sbbase, c->N, nv, c->d_views, m,
                         c->skeys, c->svals);
     }
     {
@@ -582,10 +602,10 @@ sv_status enqueue_build(sv_ctx *c) {
   {
     Scoped t(c, PH_EMIT);
     if (c->key64)
-      launch_emit<uint64_t>(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
+      launch_emit<uint64_t>(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->bbase, c->N, nv, c->d_views, m,
                             (uint64_t *)c->keys, c->vals);
     else
-      launch_emit<uint32_t>(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->N, nv, c->d_views, m,
+      launch_emit<uint32_t>(c->st, c->rec, c->rects, c->aux, c->counts, c->offs, c->bbase, c->N, nv, c->d_views, m,
                             (uint32_t *)c->keys, c->vals);
   }
   {
@@ -684,7 +704,8 @@ RasterArgs raster_args(sv_ctx *c) {
   a.last = c->last;
   a.target = c->target;
   a.residual = nullptr;
-  a.offs = c->offs;
+  a.bbase = c->bbase;
+  a.nb = project_blocks(c->N);
   a.isect_rec = c->isect_rec;
   a.aux = c->aux;
   a.tile_cut = c->tile_cut;
@@ -780,6 +801,8 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.cfg = dev_config(c->cfg);
   v.counts = c->counts;
   v.offs = c->offs;
+  v.bbase = c->bbase;
+  v.nb = project_blocks(c->N);
   v.rec = c->rec;
   v.rects = c->rects;
   v.aux = c->aux;
@@ -988,7 +1011,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
                         c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
-                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->super_view, c->sranges, c->diff,
+                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
                         c->chunk_super, c->ccount};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
@@ -1290,6 +1313,10 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->scounts);
   dfree(c->soffs);
   dfree(c->rects);
+  dfree(c->btot);
+  dfree(c->sbtot);
+  dfree(c->bbase);
+  dfree(c->sbbase);
   dfree(c->super_view);
   dfree(c->sranges);
   dfree(c->diff);

@@ -57,18 +57,18 @@ __global__ void k_bin_info(StepMeta *m, uint32_t super_bits, uint64_t cap_super)
   m0.overflow = of ? of : m0.overflow;
 }
 
-// Per (slot, Gaussian) item: the emission base in q3.x (as emit does) and
-// its supertile pairs at soffs (supertiles row-major). One workgroup writes the pairs of
+// Per (slot, Gaussian) item: its supertile pairs (supertiles row-major)
+// from its first supertile-pair position on. One workgroup writes the pairs of
 // 256 consecutive items as one contiguous span (binary search of the
 // owning item per position), so the stores are whole lines.
-__global__ __launch_bounds__(BIN_T) void k_emit_super(float4 *__restrict__ rec, const uint2 *__restrict__ rects,
+__global__ __launch_bounds__(BIN_T) void k_emit_super(const uint2 *__restrict__ rects,
                                                       const float2 *__restrict__ aux,
-                                                      const uint32_t *__restrict__ counts,
-                                                      const uint32_t *__restrict__ offs,
                                                       const uint32_t *__restrict__ scounts,
-                                                      const uint32_t *__restrict__ soffs, uint32_t N, int nv,
-                                                      const DevView *__restrict__ views, const StepMeta *m,
-                                                      uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+                                                      const uint32_t *__restrict__ soffs,
+                                                      const uint32_t *__restrict__ sbbase, uint32_t nb,
+                                                      uint32_t N, int nv, const DevView *__restrict__ views,
+                                                      const StepMeta *m, uint32_t *__restrict__ keys,
+                                                      uint32_t *__restrict__ vals) {
   __shared__ uint32_t s_lo[BIN_T];    // first pair position (0xffffffff past the items)
   __shared__ uint32_t s_srect[BIN_T]; // supertile rect: sx0 | sw << 8 | sy0 << 16
   __shared__ uint32_t s_trect0[BIN_T];  // tile rect x0 | y0 << 16
@@ -85,14 +85,13 @@ __global__ __launch_bounds__(BIN_T) void k_emit_super(float4 *__restrict__ rec, 
   uint32_t lo = 0xffffffffu, hi = 0;
   if (vg < M) {
     const uint32_t cnt = scounts[vg];
-    lo = soffs[vg];
+    const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
+    lo = block_pos(sbbase, nb, s, g, soffs[vg]);
     hi = lo + cnt;
     if (cnt) {
-      const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
       const DevView &V = views[s];
       const uint2 rc = rects[vg];
       const uint32_t a = rc.x, b = rc.y;
-      reinterpret_cast<float *>(rec + vg * 4 + 3)[0] = __uint_as_float(offs[vg]);  // q3.x
       const uint32_t x0 = rect_lo(a), y0 = rect_hi(a), x1 = rect_lo(b), y1 = rect_hi(b);
       const uint32_t sx0 = x0 / BIN_S, sy0 = y0 / BIN_S;
       const uint32_t sx1 = (x1 - 1u) / BIN_S + 1u;
@@ -380,13 +379,13 @@ void launch_bin_info(hipStream_t st, StepMeta *m, uint32_t super_bits, uint64_t 
   hipLaunchKernelGGL(k_bin_info, dim3(1), dim3(64), 0, st, m, super_bits, cap_super);
 }
 
-void launch_emit_super(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
-                       const uint32_t *offs, const uint32_t *scounts, const uint32_t *soffs, uint32_t N, int nv,
-                       const DevView *views, const StepMeta *m, uint32_t *keys, uint32_t *vals) {
+void launch_emit_super(hipStream_t st, const uint2 *rects, const float2 *aux, const uint32_t *scounts,
+                       const uint32_t *soffs, const uint32_t *sbbase, uint32_t N, int nv, const DevView *views,
+                       const StepMeta *m, uint32_t *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_emit_super, dim3((uint32_t)((M + BIN_T - 1) / BIN_T)), dim3(BIN_T), 0, st, rec, rects, aux,
-                     counts, offs, scounts, soffs, N, nv, views, m, keys, vals);
+  hipLaunchKernelGGL(k_emit_super, dim3((uint32_t)((M + BIN_T - 1) / BIN_T)), dim3(BIN_T), 0, st, rects, aux,
+                     scounts, soffs, sbbase, project_blocks(N), N, nv, views, m, keys, vals);
 }
 
 size_t bin_chunk_cap(uint64_t super_cap, uint32_t n_super_tiles) {

@@ -15,9 +15,21 @@ namespace sv {
 // colour of training-forward-tiles.wgsl:65-105 evaluated once per view).
 // rec: 64-byte record (4 float4) per (slot, Gaussian); aux: (depth,
 // radius); counts: u32 per (slot, Gaussian).
+// Also the first level of the scans over (slot, Gaussian): offs_loc /
+// soffs_loc = exclusive prefix of the tile / supertile counts within the
+// projection block (256 Gaussians of one slot; q3.x holds the tile one) and
+// btot / sbtot = the block totals, slot-major (project_blocks(N) per slot).
+// A (slot, Gaussian)'s first position = scan of the totals at its block +
+// its local prefix (block_pos).
+inline uint32_t project_blocks(uint32_t N) { return (N + 255u) / 256u; }
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq, uint32_t *scounts = nullptr, uint2 *rects = nullptr);
+                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects, uint32_t *offs_loc,
+                    uint32_t *soffs_loc, uint32_t *btot, uint32_t *sbtot);
+__device__ __forceinline__ uint32_t block_pos(const uint32_t *bbase, uint32_t nb, uint32_t s, uint32_t g,
+                                              uint32_t local) {
+  return bbase[(size_t)s * nb + (g >> 8)] + local;
+}
 // Exclusive scan of u32 counts (M entries) into u32 offsets; total (u64)
 // written to *total. tmp must hold scan_tmp_words(M) u64 words.
 size_t scan_tmp_words(size_t M);
@@ -27,11 +39,12 @@ void launch_scan(hipStream_t st, const uint32_t *counts, uint32_t *offs, size_t 
 // the scan total and the largest depth quantum.
 void launch_keyinfo(hipStream_t st, StepMeta *m, uint32_t tile_bits, uint32_t key_type_bits, uint64_t cap);
 // emit_intersections (training-tiles.wgsl:201-247) for nv slots.
-// rects: per (slot, Gaussian) tile rectangle (x0 | y0 << 16, x1 | y1 << 16).
+// rects: per (slot, Gaussian) tile rectangle (x0 | y0 << 16, x1 | y1 << 16);
+// offs / bbase: local prefixes and scanned block totals of the tile counts.
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
-                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
-                 K *keys, uint32_t *vals);
+                 const uint32_t *offs, const uint32_t *bbase, uint32_t N, int nv, const DevView *views,
+                 const StepMeta *m, K *keys, uint32_t *vals);
 // Tile ranges = lower_bound per global tile (build_tile_offsets +
 // host fix-up, training-tiles.wgsl:249-271, gaussian-renderer.ts:2014-2020).
 // Grid sized for the capacity; n, dbits and the result buffer from m.
@@ -42,9 +55,9 @@ void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap,
 // and m[1].n_isect; supertile pairs of each item; tile counts, ranges and
 // tile lists from the sorted supertile pairs (m[2], m[3]: scan totals).
 void launch_bin_info(hipStream_t st, StepMeta *m, uint32_t super_bits, uint64_t cap_super);
-void launch_emit_super(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
-                       const uint32_t *offs, const uint32_t *scounts, const uint32_t *soffs, uint32_t N, int nv,
-                       const DevView *views, const StepMeta *m, uint32_t *keys, uint32_t *vals);
+void launch_emit_super(hipStream_t st, const uint2 *rects, const float2 *aux, const uint32_t *scounts,
+                       const uint32_t *soffs, const uint32_t *sbbase, uint32_t N, int nv, const DevView *views,
+                       const StepMeta *m, uint32_t *keys, uint32_t *vals);
 struct BinArgs {
   const uint32_t *keys0, *vals0, *keys1, *vals1;  // supertile pairs, sorted into buffer m[1].sel
   const uint32_t *sranges;      // per global supertile + 1: lower bound in the sorted pairs
@@ -100,7 +113,8 @@ struct RasterArgs {
   const float *target;    // 3 planes per view
   const float *residual;  // 3 planes per view, or null for the fused loss
   // backward outputs
-  const uint32_t *offs;   // per (slot, Gaussian) first intersection (emit order)
+  const uint32_t *bbase;  // scanned block totals of the tile counts (emit positions)
+  uint32_t nb;            // projection blocks per slot
   float *isect_rec;       // 9-float adjoint record per intersection (emit order)
   // per (slot, Gaussian) depth and radius; per global tile, the sort key
   // (depth_q << 32 | Gaussian) of the first entry no pixel reaches
@@ -125,7 +139,9 @@ struct VjpArgs {
   int nv;
   DevConfig cfg;
   const uint32_t *counts;  // per (slot, Gaussian)
-  const uint32_t *offs;
+  const uint32_t *offs;    // local prefix of the tile counts in the projection block
+  const uint32_t *bbase;   // scanned block totals: first emit position = block_pos(...)
+  uint32_t nb;
   const float4 *rec;       // projection records
   const uint2 *rects;      // tile rectangle per record
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)

@@ -249,7 +249,7 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
     const uint32_t dq = f2u_sat(clampf(a.aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
     const uint64_t me = ((uint64_t)dq << 32) | g;
-    const float *rp = a.isect_rec + (size_t)a.offs[vg] * REC9;
+    const float *rp = a.isect_rec + (size_t)block_pos(a.bbase, a.nb, s, g, a.offs[vg]) * REC9;
     bool first = true;
     for (uint32_t ty = y0; ty < y1; ++ty) {
       const uint64_t *cut = a.tile_cut + V.tile_base + ty * V.tiles_x;

@@ -28,6 +28,42 @@ namespace sv {
 // 1% + 0.01 px margin for rounding. The rasterizers use both to skip pairs
 // without evaluating them; every skipped pair is one the reference skips,
 // so no result changes.
+// Exclusive prefix of two u32 values over the 256 threads of the block,
+// and their block totals (wave scans, then the four wave totals).
+__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t &pa, uint32_t &pb, uint32_t &ta,
+                                            uint32_t &tb, uint32_t *s_w) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = a, y = b;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t xo = (uint32_t)__shfl_up((int)x, o, 64), yo = (uint32_t)__shfl_up((int)y, o, 64);
+    if (lane >= o) {
+      x += xo;
+      y += yo;
+    }
+  }
+  if (lane == 63) {
+    s_w[wave] = x;
+    s_w[4 + wave] = y;
+  }
+  __syncthreads();
+  uint32_t ba = 0, bb = 0;
+  ta = 0;
+  tb = 0;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    if (w < wave) {
+      ba += s_w[w];
+      bb += s_w[4 + w];
+    }
+    ta += s_w[w];
+    tb += s_w[4 + w];
+  }
+  pa = ba + x - a;
+  pb = bb + y - b;
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, size_t cap, uint32_t N,
                                                  const DevView *__restrict__ views, int nv,
@@ -36,11 +72,16 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
                                                  uint32_t *__restrict__ counts,
                                                  uint32_t *__restrict__ max_dq,
                                                  uint32_t *__restrict__ scounts,
-                                                 uint2 *__restrict__ rects) {
+                                                 uint2 *__restrict__ rects,
+                                                 uint32_t *__restrict__ offs_loc,
+                                                 uint32_t *__restrict__ soffs_loc,
+                                                 uint32_t *__restrict__ btot,
+                                                 uint32_t *__restrict__ sbtot) {
   // Records of one view for the block's 256 Gaussians are staged in LDS
   // and stored as whole contiguous lines (each store instruction of a wave
   // writes 1 KB), instead of 64-byte pieces 64 bytes apart per lane.
   __shared__ float4 s_rec[256 * 4];
+  __shared__ uint32_t s_scan[8];
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = g < N;
   const uint32_t gl = live ? g : N - 1u;  // lanes past N compute on a valid Gaussian, store nothing
@@ -176,6 +217,12 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           ax = make_float2(depth, radius);
         }
       }
+      // first level of the scans over (view, Gaussian): exclusive prefix of
+      // the tile and supertile counts within the block; the block totals
+      // are scanned afterwards, and a position is block base + prefix
+      uint32_t pa, pb, ta, tb;
+      block_scan2(live ? count : 0u, live ? scount : 0u, pa, pb, ta, tb, s_scan);
+      r3.x = __uint_as_float(pa);  // emit base within the block
       s_rec[threadIdx.x * 4 + 0] = r0;
       s_rec[threadIdx.x * 4 + 1] = r1;
       s_rec[threadIdx.x * 4 + 2] = r2;
@@ -184,7 +231,13 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
         aux[vg] = ax;
         counts[vg] = count;
         if (scounts) scounts[vg] = scount;
-        if (rects) rects[vg] = rect;
+        rects[vg] = rect;
+        offs_loc[vg] = pa;
+        if (soffs_loc) soffs_loc[vg] = pb;
+      }
+      if (threadIdx.x == 0) {
+        btot[(size_t)s * gridDim.x + blockIdx.x] = ta;
+        if (sbtot) sbtot[(size_t)s * gridDim.x + blockIdx.x] = tb;
       }
       __syncthreads();
       // piece j of the block's 1024 float4 belongs to Gaussian
@@ -215,11 +268,12 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
 
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects) {
+                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects, uint32_t *offs_loc,
+                    uint32_t *soffs_loc, uint32_t *btot, uint32_t *sbtot) {
   if (N == 0) return;
-  const uint32_t blocks = (N + 255u) / 256u;
+  const uint32_t blocks = project_blocks(N);
   hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, aux,
-                     counts, max_dq, scounts, rects);
+                     counts, max_dq, scounts, rects, offs_loc, soffs_loc, btot, sbtot);
 }
 
 // ---------------------------------------------------------------------
@@ -368,7 +422,8 @@ template <typename K>
 __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec, const uint2 *__restrict__ rects,
                                               const float2 *__restrict__ aux,
                                               const uint32_t *__restrict__ counts,
-                                              const uint32_t *__restrict__ offs, uint32_t N, int nv,
+                                              const uint32_t *__restrict__ offs,
+                                              const uint32_t *__restrict__ bbase, uint32_t nb, uint32_t N, int nv,
                                               const DevView *__restrict__ views, const StepMeta *m,
                                               K *__restrict__ keys, uint32_t *__restrict__ vals) {
   __shared__ uint32_t s_lo[256];   // first output position (0xffffffff past the items)
@@ -386,15 +441,14 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec, const ui
   uint32_t lo = 0xffffffffu, hi = 0;
   if (vg < M) {
     const uint32_t cnt = counts[vg];
-    lo = offs[vg];
+    const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
+    lo = bbase[(size_t)s * nb + (g >> 8)] + offs[vg];
     hi = lo + cnt;
     if (cnt) {
-      const uint32_t s = (uint32_t)(vg / N), g = (uint32_t)(vg - (size_t)s * N);
       const DevView &V = views[s];
       const uint2 rc = rects[vg];
       const uint32_t a = rc.x, b = rc.y;
       const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu;
-      reinterpret_cast<float *>(rec + vg * 4 + 3)[0] = __uint_as_float(lo);  // q3.x
       s_rect[t] = x0 | ((x1 - x0) << 16);
       s_y0[t] = y0;
       s_kb[t] = (K)(V.tile_base + y0 * V.tiles_x + x0);
@@ -428,19 +482,19 @@ __global__ __launch_bounds__(256) void k_emit(float4 *__restrict__ rec, const ui
 
 template <typename K>
 void launch_emit(hipStream_t st, float4 *rec, const uint2 *rects, const float2 *aux, const uint32_t *counts,
-                 const uint32_t *offs, uint32_t N, int nv, const DevView *views, const StepMeta *m,
-                 K *keys, uint32_t *vals) {
+                 const uint32_t *offs, const uint32_t *bbase, uint32_t N, int nv, const DevView *views,
+                 const StepMeta *m, K *keys, uint32_t *vals) {
   const size_t M = (size_t)nv * N;
   if (M == 0) return;
   hipLaunchKernelGGL(k_emit<K>, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, rec, rects, aux, counts,
-                     offs, N, nv, views, m, keys, vals);
+                     offs, bbase, project_blocks(N), N, nv, views, m, keys, vals);
 }
 template void launch_emit<uint32_t>(hipStream_t, float4 *, const uint2 *, const float2 *, const uint32_t *,
-                                    const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
-                                    uint32_t *, uint32_t *);
+                                    const uint32_t *, const uint32_t *, uint32_t, int, const DevView *,
+                                    const StepMeta *, uint32_t *, uint32_t *);
 template void launch_emit<uint64_t>(hipStream_t, float4 *, const uint2 *, const float2 *, const uint32_t *,
-                                    const uint32_t *, uint32_t, int, const DevView *, const StepMeta *,
-                                    uint64_t *, uint32_t *);
+                                    const uint32_t *, const uint32_t *, uint32_t, int, const DevView *,
+                                    const StepMeta *, uint64_t *, uint32_t *);
 
 // ---------------------------------------------------------------------
 // Tile ranges: ranges[t] = first sorted index whose tile is >= t, which is

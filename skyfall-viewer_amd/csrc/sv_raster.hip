@@ -153,12 +153,13 @@ __device__ __forceinline__ uint32_t pix_y(const TileGeom &t, int j) {
 }
 
 // Position of an intersection in emit order: the Gaussian's first
-// intersection plus its row-major index inside the Gaussian's tile
-// rectangle (training-tiles.wgsl:238-246).
-__device__ __forceinline__ uint32_t emit_pos(const float4 &q3, uint32_t tx, uint32_t ty) {
+// intersection (its projection block's base + q3.x, the prefix within the
+// block) plus its row-major index inside the Gaussian's tile rectangle
+// (training-tiles.wgsl:238-246).
+__device__ __forceinline__ uint32_t emit_pos(const float4 &q3, uint32_t base, uint32_t tx, uint32_t ty) {
   const uint32_t u = __float_as_uint(q3.y), w = __float_as_uint(q3.z);
   const uint32_t x0 = u & 0xffffu, y0 = u >> 16, x1 = w & 0xffffu;
-  return __float_as_uint(q3.x) + (ty - y0) * (x1 - x0) + (tx - x0);
+  return base + __float_as_uint(q3.x) + (ty - y0) * (x1 - x0) + (tx - x0);
 }
 
 // Load up to B entries of a tile list from `first` (count cnt) into LDS,
@@ -190,7 +191,7 @@ template <int B>
 __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
                                                uint32_t cnt, const QuadBounds &qb,
                                                float4 *s_q0, float4 *s_q1, float4 *s_q2, uint32_t *s_k,
-                                               uint32_t *s_p, uint32_t tx, uint32_t ty) {
+                                               uint32_t *s_p, uint32_t slot, uint32_t tx, uint32_t ty) {
   const int lane = threadIdx.x;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t kept = 0;
@@ -200,7 +201,8 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
     uint32_t qm = 0;
     float4 q0, q1, q2, q3;
     if (k < cnt) {
-      const float4 *q = a.rec + (vbase + min(a.vals[first + k], a.N - 1u)) * 4;
+      const uint32_t id = min(a.vals[first + k], a.N - 1u);
+      const float4 *q = a.rec + (vbase + id) * 4;
       q0 = q[0];
       q1 = q[1];
       q2 = q[2];
@@ -215,7 +217,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       touch = qm != 0;
       if (s_p) {
         q3 = q[3];
-        const uint32_t p = emit_pos(q3, tx, ty);
+        const uint32_t p = emit_pos(q3, a.bbase[(size_t)slot * a.nb + (id >> 8)], tx, ty);
         s_p[k] = p;
         // an entry that touches no pixel of the tile contributes nothing
         if (!touch && (uint64_t)p < a.n_isect) {
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
       }
       for (uint32_t b0 = start; b0 < end; b0 += B) {
         const uint32_t cnt = min((uint32_t)B, end - b0);
-        const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0);
+        const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0, 0);
         __syncthreads();
         // Two entries per iteration: their alphas are independent (only the
         // compositing that follows is ordered), so the two exp chains
@@ -498,7 +500,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        const uint32_t kept = load_batch<B>(a, vbase, lo, cnt, qb, s_q0, s_q1, s_q2, s_k, s_p, t.tx, t.ty);
+        const uint32_t kept = load_batch<B>(a, vbase, lo, cnt, qb, s_q0, s_q1, s_q2, s_k, s_p, t.s, t.tx, t.ty);
         __syncthreads();
         for (int e = (int)kept - 1; e >= 0; --e) {
           // same word on every lane: the quadrant tests below are uniform
