@@ -171,14 +171,76 @@ def cpu_baseline(g, cams, targets, args):
     }
 
 
+def run_cfg4(args):
+    """The growth run: 1M splats, 16 ring views at 1920x1080, batched steps;
+    every 100 steps prune sigmoid(logit) < 0.005 and densify (split above 8
+    px where a tile's L1 error is above its view's 90th percentile, else
+    clone), the added splats capped so the count stays <= 4M (actions past
+    the budget are dropped in index order). The timed region holds every
+    step and every prune / densify event."""
+    N0, V, W, H, NMAX, EVERY = 1_000_000, 16, 1920, 1080, 4_000_000, 100
+    g = sv.scene_gaussians(0, N0, 1.0)
+    cams = sv.ring_cameras(V, 3.0, W, H)
+    tg = dict(g)
+    tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
+    c = sv.Context(capacity=NMAX)
+    configure(c, args)
+    c.set_config(use_graph=args.graph)
+    c.set_gaussians(tg)
+    c.set_views(cams)
+    targets = []
+    for v in range(V):
+        c.forward(v)
+        targets.append(c.image(v))
+    c.set_gaussians(g)
+    for v in range(V):
+        c.set_target(v, targets[v])
+    views = list(range(V))
+    log(f"cfg4: {N0} splats, {V} views {W}x{H}, {args.warmup} warmup + {args.steps} steps")
+    for _ in range(args.warmup):
+        c.step_views(views, sv.MODE_BATCHED)
+    c.sync()
+    events, isects, st = [], 0, None
+    t0 = time.perf_counter()
+    for it in range(1, args.steps + 1):
+        st = c.step_views(views, sv.MODE_BATCHED, stats=True)
+        isects += int(st.n_isect)
+        if it % EVERY == 0:
+            removed = c.prune_opacity(0.005)
+            act, nc, ns = c.densify_decide(views, 0.9, 8.0)
+            budget = NMAX - c.count
+            if nc + ns > budget:
+                idx = np.flatnonzero(act)
+                act[idx[max(budget, 0):]] = 0
+            added = c.densify(act, seed=it)
+            events.append({"step": it, "pruned": removed, "added": added, "count": c.count})
+    c.sync()
+    dt = time.perf_counter() - t0
+    P = V * W * H
+    out = {
+        "metric": "train steps/s, growth run (config 4)", "value": args.steps / dt, "unit": "steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
+        "config": {"workload": f"{N0} splats growing to <= {NMAX}, {V} views {W}x{H}, prune/densify every "
+                               f"{EVERY} steps", "splats_start": N0, "splats_end": c.count, "views": V,
+                   "width": W, "height": H},
+        "intersections_per_s": isects / dt, "final_mean_l1": float(st.l1_sum) / (3.0 * P) if st else None,
+        "events": events,
+    }
+    c.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ring16", "uneven64", "cfg0", "cfg1"), default="ring16",
+    ap.add_argument("--workload", choices=("ring16", "uneven64", "cfg0", "cfg1", "cfg4"), default="ring16",
                     help="ring16: 16 ring views at --width x --height (the headline); uneven64: 64 views "
                          "with seeded sizes 320-1920 x 240-1080 on a sphere of radius 4, means in [-2,2]^3; "
                          "cfg0: 16,384 splats, one 256x256 view from (0,0,3); cfg1: 500,000 splats, 8 ring "
-                         "views at 1280x720")
+                         "views at 1280x720; cfg4: the growth run (1M splats growing to at most 4M, 16 ring "
+                         "views at 1080p, prune / densify every 100 steps; give --steps, e.g. 2000)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -214,6 +276,9 @@ def main():
               "(multi-GPU sharding by views is a design only)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "cfg4":
+        run_cfg4(args)
+        return
     N, V, W, H = args.n, args.views, args.width, args.height
     if args.workload == "cfg0":
         N, V, W, H = 16384, 1, 256, 256

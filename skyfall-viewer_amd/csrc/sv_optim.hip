@@ -250,18 +250,41 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     const uint32_t dq = f2u_sat(clampf(a.aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
     const uint64_t me = ((uint64_t)dq << 32) | g;
     const float *rp = a.isect_rec + (size_t)block_pos(a.bbase, a.nb, s, g, a.offs[vg]) * REC9;
+    const uint32_t w = x1 - x0, n = w * (y1 - y0);
+    const uint64_t *cut0 = a.tile_cut + V.tile_base + y0 * V.tiles_x + x0;
     bool first = true;
-    for (uint32_t ty = y0; ty < y1; ++ty) {
-      const uint64_t *cut = a.tile_cut + V.tile_base + ty * V.tiles_x;
-      for (uint32_t tx = x0; tx < x1; ++tx, rp += REC9) {
-        if (me >= cut[tx]) continue;
+    // four rectangle entries per round: their cut and record loads are
+    // issued together, then folded in emit order
+    constexpr uint32_t U = 4;
+    for (uint32_t k0 = 0; k0 < n; k0 += U) {
+      bool ok[U];
+      float rr[U][REC9];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t k = k0 + u;
+        ok[u] = false;
+        if (k < n) {
+          const uint32_t dy = k / w, dx = k - dy * w;
+          ok[u] = me < cut0[dy * V.tiles_x + dx];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const float *src = rp + (size_t)(k0 + u) * REC9;
+#pragma unroll
+        for (int q = 0; q < REC9; ++q) rr[u][q] = src[q];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
         if (first) {
 #pragma unroll
-          for (int q = 0; q < REC9; ++q) r[q] = rp[q];
+          for (int q = 0; q < REC9; ++q) r[q] = rr[u][q];
           first = false;
         } else {
 #pragma unroll
-          for (int q = 0; q < REC9; ++q) r[q] = r[q] + rp[q];
+          for (int q = 0; q < REC9; ++q) r[q] = r[q] + rr[u][q];
         }
       }
     }
