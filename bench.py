@@ -214,6 +214,8 @@ def run_cfg4(args):
                 act[idx[max(budget, 0):]] = 0
             added = c.densify(act, seed=it)
             events.append({"step": it, "pruned": removed, "added": added, "count": c.count})
+            log(f"cfg4 step {it}: pruned {removed}, added {added}, {c.count} splats, "
+                f"mean L1 {float(st.l1_sum) / (3.0 * V * W * H):.5f}")
     c.sync()
     dt = time.perf_counter() - t0
     P = V * W * H
