@@ -125,6 +125,31 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t m) {
   return m;
 }
 
+// 0.0f from a register move the optimiser does not fold.
+__device__ __forceinline__ float opaque_zero() {
+  float z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
+// a / b correctly rounded, for a = 0 or a in [2^-100, 2^10] and b in
+// [1e-4, 1] (the backward's T / max(1 - alpha, 1e-4), T in (1e-7, 1]):
+// the refinement that the IEEE division sequence (v_div_scale, v_rcp, three
+// fma corrections, v_div_fmas, v_div_fixup) performs, without the scaling
+// and fix-up steps, which leave operands in these ranges unchanged
+// (numerator, reciprocal and quotient far from the denormal and overflow
+// ranges). Same bits as a / b.
+__device__ __forceinline__ float div_unscaled(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, r, 1.0f);
+  r = fmaf(e, r, r);
+  float q = a * r;
+  const float e2 = fmaf(-b, q, a);
+  q = fmaf(e2, r, q);
+  const float e3 = fmaf(-b, q, a);
+  return fmaf(e3, r, q);
+}
+
 __device__ __forceinline__ float sign_ref(float x) {
   return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
 }
@@ -510,25 +535,28 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
           const float4 x0 = s_q0[e];
           const float4 x1 = s_q1[e];
           const float4 x2 = s_q2[e];
-          float v0 = 0.f, v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f, v5 = 0.f, v6 = 0.f, v7 = 0.f, v8 = 0.f;
+          // the nine sums start at a zero the compiler cannot see through:
+          // with a literal 0 it rematerialises the zeros on every skip path
+          float v0 = opaque_zero(), v1 = v0, v2 = v0, v3 = v0, v4 = v0, v5 = v0, v6 = v0, v7 = v0, v8 = v0;
           bool hit = false;
           {
 #pragma unroll
             for (int j = 0; j < PPL; ++j) {
               if (!((ke >> (16 + j)) & 1u)) continue;
-              if (!(valid[j] && i <= lastp[j])) continue;
+              // one divergent branch per slot: the alpha is computed on every
+              // lane (out-of-range lanes' values are discarded)
               const float dx = fxh[j & 1] - x0.x;
               const float dy = fyh[j >> 1] - x0.y;
-              if (fabsf(dx) > x2.z || fabsf(dy) > x2.w) continue;
               const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-              if (sg < 0.0f || sg > x2.y) continue;
               const float gauss = sv_expf_neg(-sg);
               const float op = x1.y;
               const float as = fminf(op * gauss, SV_MAX_ALPHA);
-              if (as < SV_MIN_ALPHA) continue;
+              const bool take = (valid[j] & (i <= lastp[j])) & !(fabsf(dx) > x2.z || fabsf(dy) > x2.w) &
+                                !(sg < 0.0f || sg > x2.y) & !(as < SV_MIN_ALPHA);
+              if (!take) continue;
               hit = true;
               const float oma = fmaxf(1.0f - as, 0.0001f);
-              const float Tb = T[j] / oma;
+              const float Tb = div_unscaled(T[j], oma);
               const float contrib = as * Tb;
               v0 = v0 + r0[j] * contrib;
               v1 = v1 + r1[j] * contrib;
