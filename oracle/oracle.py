@@ -284,6 +284,27 @@ def prune_keep(logit: np.ndarray, min_opacity: float) -> np.ndarray:
     return keep
 
 
+def accumulate_importance(proj: np.ndarray, stats4: np.ndarray, decay: float) -> np.ndarray:
+    """accumulate_importance (importance-accum.wgsl:19-40) of one view's
+    projections into a copy of stats4 (n, 4): importance_sum, visibility,
+    max screen radius, pad."""
+    st = np.array(stats4, np.float32, copy=True).reshape(-1, 4)
+    pr = np.ascontiguousarray(proj)
+    lib().orc_accumulate_importance(C.c_size_t(st.shape[0]), pr.ctypes.data_as(C.c_void_p), C.c_float(decay),
+                                    _fp(st))
+    return st
+
+
+def accumulate_grads(geom: np.ndarray, accum4: np.ndarray) -> np.ndarray:
+    """accumulate_grads (densify-accum.wgsl:19-46) of one view's geometric
+    gradients geom (n, 5: conic a, b, c, mean x, y) into a copy of accum4
+    (n, 4): |dL/dmean2d|, 0, |dL/dconic|, count."""
+    acc = np.array(accum4, np.float32, copy=True).reshape(-1, 4)
+    gg = np.ascontiguousarray(geom, np.float32).reshape(-1, 5)
+    lib().orc_accumulate_grads(C.c_size_t(acc.shape[0]), _fp(gg), _fp(acc))
+    return acc
+
+
 def knn_init(pos: np.ndarray, extent: float):
     """(density, logscale) per point; pos (n, 3)."""
     p = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
