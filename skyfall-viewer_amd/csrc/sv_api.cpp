@@ -109,6 +109,7 @@ struct sv_ctx {
   uint32_t *counts = nullptr, *offs = nullptr;
   uint32_t *scounts = nullptr, *soffs = nullptr;  // supertile pairs per (slot, Gaussian)
   uint2 *rects = nullptr;  // tile rectangle per (slot, Gaussian)
+  uint32_t *tins = nullptr;  // tight-rectangle insets per (slot, Gaussian)
   // projection-block totals of the tile / supertile counts and their scans
   uint32_t *btot = nullptr, *sbtot = nullptr, *bbase = nullptr, *sbbase = nullptr;
   size_t btab_cap = 0;
@@ -422,6 +423,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     dfree(c->scounts);
     dfree(c->soffs);
     dfree(c->rects);
+    dfree(c->tins);
     dfree(c->vrec);
     c->vg_cap = 0;
     if (sv_status s = dalloc(c, &c->vrec, (size_t)REC9 * M)) return s;
@@ -432,6 +434,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     if (sv_status s = dalloc(c, &c->scounts, M)) return s;
     if (sv_status s = dalloc(c, &c->soffs, M)) return s;
     if (sv_status s = dalloc(c, &c->rects, M)) return s;
+    if (sv_status s = dalloc(c, &c->tins, M)) return s;
     c->vg_cap = M;
   }
   // projection-block tables: slots x blocks per slot (+1 for the scan total)
@@ -542,7 +545,7 @@ sv_status enqueue_build(sv_ctx *c) {
   {
     Scoped t(c, PH_PROJECT);
     launch_project(c->st, c->P, c->cap, c->N, c->d_views, nv, dc, c->rec, c->aux, c->counts, &m->max_dq,
-                   bin ? c->scounts : nullptr, c->rects, c->offs, bin ? c->soffs : nullptr, c->btot,
+                   bin ? c->scounts : nullptr, c->rects, c->tins, c->offs, bin ? c->soffs : nullptr, c->btot,
                    bin ? c->sbtot : nullptr);
   }
   {
@@ -805,6 +808,7 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.nb = project_blocks(c->N);
   v.rec = c->rec;
   v.rects = c->rects;
+  v.tins = c->tins;
   v.aux = c->aux;
   v.isect_rec = c->isect_rec;
   v.tile_cut = c->tile_cut;
@@ -1011,7 +1015,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
                         c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
-                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
+                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
                         c->chunk_super, c->ccount};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
@@ -1313,6 +1317,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->scounts);
   dfree(c->soffs);
   dfree(c->rects);
+  dfree(c->tins);
   dfree(c->btot);
   dfree(c->sbtot);
   dfree(c->bbase);
@@ -1764,12 +1769,14 @@ sv_status sv_read_projections(sv_ctx *c, int v, float *out) {
   const size_t N = c->N;
   std::vector<float4> r(4 * N);
   std::vector<float2> ax(N);
+  std::vector<uint32_t> cnt(N);
   HIP_OK(c, hipMemcpy(r.data(), c->rec, 4 * N * sizeof(float4), hipMemcpyDeviceToHost));
   HIP_OK(c, hipMemcpy(ax.data(), c->aux, N * sizeof(float2), hipMemcpyDeviceToHost));
+  HIP_OK(c, hipMemcpy(cnt.data(), c->counts, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
   std::vector<float> pos(3 * N), lg(N);
   if (sv_status s = get_params(c, c->P, pos.data(), nullptr, nullptr, lg.data(), nullptr)) return s;
   for (size_t g = 0; g < N; ++g) {
-    const float4 r0 = r[4 * g], r1 = r[4 * g + 1], r3 = r[4 * g + 3];
+    const float4 r0 = r[4 * g], r1 = r[4 * g + 1];
     float *o = out + 12 * g;
     if (ax[g].y == 0.0f) {  // culled (radius 0): the record is all zero
       for (int k = 0; k < 12; ++k) o[k] = 0.0f;
@@ -1782,7 +1789,7 @@ sv_status sv_read_projections(sv_ctx *c, int v, float *out) {
     o[4] = r0.z;
     o[5] = r0.w;
     o[6] = r1.x;
-    std::memcpy(&o[7], &r3.w, 4);
+    std::memcpy(&o[7], &cnt[g], 4);
     o[8] = pos[3 * g];
     o[9] = pos[3 * g + 1];
     o[10] = pos[3 * g + 2];
@@ -2076,7 +2083,7 @@ sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, 
   if (hipMemcpy(d_thr, thr.data(), nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     st = fail(c, SV_E_HIP, "threshold upload failed");
   if (!st) {
-    launch_densify_decide(c->st, c->rec, c->aux, c->N, c->d_views, nv, c->tile_l1, d_thr, split_px, d_act);
+    launch_densify_decide(c->st, c->rec, c->aux, c->counts, c->N, c->d_views, nv, c->tile_l1, d_thr, split_px, d_act);
     st = check_launch(c, "densify_decide");
   }
   if (!st && hipMemcpy(actions, d_act, c->N, hipMemcpyDeviceToHost) != hipSuccess)

@@ -24,8 +24,46 @@ namespace sv {
 inline uint32_t project_blocks(uint32_t N) { return (N + 255u) / 256u; }
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects, uint32_t *offs_loc,
+                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects, uint32_t *tins, uint32_t *offs_loc,
                     uint32_t *soffs_loc, uint32_t *btot, uint32_t *sbtot);
+
+// Tight tile rectangle of a (view, Gaussian): the tiles of its rectangle
+// whose pixel centres come within one pixel plus its extent (ext_x, ext_y)
+// of the mean. Every tile where a quadrant test can pass lies inside, so
+// the backward writes, and the record gather reads, adjoint records only
+// there. Stored as four 8-bit insets from the rectangle's sides (left | top
+// << 8 | right << 16 | bottom << 24); TIGHT_EMPTY (all insets 255) leaves
+// nothing, as rectangles are at most 66 tiles wide.
+constexpr uint32_t TIGHT_EMPTY = 0xffffffffu;
+__device__ __forceinline__ uint32_t tight_insets(float m, float e, float mm, float ee, uint32_t x0, uint32_t y0,
+                                                 uint32_t x1, uint32_t y1) {
+  if (!(e >= 0.0f && ee >= 0.0f)) return TIGHT_EMPTY;
+  // tile t is kept when 16 t + 0.5 - 1 <= m + e and 16 t + 15.5 + 1 >= m - e
+  const auto lo = [](float c, float r) {
+    return (int32_t)ceilf(fminf(fmaxf((c - r - 16.5f) * 0.0625f, -1e6f), 1e6f));
+  };
+  const auto hi = [](float c, float r) {
+    return (int32_t)floorf(fminf(fmaxf((c + r + 0.5f) * 0.0625f, -1e6f), 1e6f)) + 1;
+  };
+  const int32_t tx0 = max(lo(m, e), (int32_t)x0), tx1 = min(hi(m, e), (int32_t)x1);
+  const int32_t ty0 = max(lo(mm, ee), (int32_t)y0), ty1 = min(hi(mm, ee), (int32_t)y1);
+  if (tx0 >= tx1 || ty0 >= ty1) return TIGHT_EMPTY;
+  const auto c8 = [](int32_t v) { return (uint32_t)min(v, 254); };
+  return c8(tx0 - (int32_t)x0) | (c8(ty0 - (int32_t)y0) << 8) | (c8((int32_t)x1 - tx1) << 16) |
+         (c8((int32_t)y1 - ty1) << 24);
+}
+// Bounds [tx0, tx1) x [ty0, ty1) of the tight rectangle (empty: tx0 >= tx1).
+struct TightRect {
+  int32_t tx0, ty0, tx1, ty1;
+};
+__device__ __forceinline__ TightRect tight_rect(uint32_t ins, uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1) {
+  TightRect t;
+  t.tx0 = (int32_t)x0 + (int32_t)(ins & 255u);
+  t.ty0 = (int32_t)y0 + (int32_t)((ins >> 8) & 255u);
+  t.tx1 = (int32_t)x1 - (int32_t)((ins >> 16) & 255u);
+  t.ty1 = (int32_t)y1 - (int32_t)(ins >> 24);
+  return t;
+}
 __device__ __forceinline__ uint32_t block_pos(const uint32_t *bbase, uint32_t nb, uint32_t s, uint32_t g,
                                               uint32_t local) {
   return bbase[(size_t)s * nb + (g >> 8)] + local;
@@ -144,6 +182,7 @@ struct VjpArgs {
   uint32_t nb;
   const float4 *rec;       // projection records
   const uint2 *rects;      // tile rectangle per record
+  const uint32_t *tins;    // tight-rectangle insets per record (tight_insets)
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
   const float *isect_rec;  // 9-float adjoint record per intersection
   const uint64_t *tile_cut; // per global tile: records at or past this (depth_q, id) were not written
@@ -190,7 +229,7 @@ void launch_clamp_logit(hipStream_t st, float *P, size_t cap, uint32_t N, float 
 void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_view, uint32_t n_tiles,
                     const float *color, const float *target, float *tile_l1);
 // Densify rule from the batch's records, tile L1 and per-view thresholds.
-void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, uint32_t N,
+void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, const uint32_t *counts, uint32_t N,
                            const DevView *views, int nv, const float *tile_l1, const float *thr,
                            float split_px, uint8_t *actions);
 void launch_densify(hipStream_t st, const float *P, const float *M1, const float *M2, size_t cap,

@@ -231,7 +231,9 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 // ---------------------------------------------------------------------
 // Sum of the adjoint records of the reached pairs of one (slot, Gaussian),
 // in emit order (ty-major over its tile rectangle): a pair at or past its
-// tile's cut (depth_q, id) was reached by no pixel and has no record.
+// tile's cut (depth_q, id) was reached by no pixel and has no record, nor
+// has a pair outside the Gaussian's tight rectangle (no pixel there can
+// take it); the walk covers the tight rectangle only.
 __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   if (a.meta && a.meta->overflow) return;
   const size_t M = (size_t)a.nv * a.N;
@@ -249,11 +251,17 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
     const uint32_t dq = f2u_sat(clampf(a.aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
     const uint64_t me = ((uint64_t)dq << 32) | g;
-    const float *rp = a.isect_rec + (size_t)block_pos(a.bbase, a.nb, s, g, a.offs[vg]) * REC9;
-    const uint32_t w = x1 - x0, n = w * (y1 - y0);
-    const uint64_t *cut0 = a.tile_cut + V.tile_base + y0 * V.tiles_x + x0;
+    const TightRect tr = tight_rect(a.tins[vg], x0, y0, x1, y1);
+    const uint32_t w = x1 - x0;
+    const uint32_t tw = tr.tx1 > tr.tx0 ? (uint32_t)(tr.tx1 - tr.tx0) : 0u;
+    const uint32_t n = tr.ty1 > tr.ty0 ? tw * (uint32_t)(tr.ty1 - tr.ty0) : 0u;
+    // entry (dx, dy) of the tight rectangle is record (oy + dy) w + ox + dx
+    // of the Gaussian's rectangle
+    const uint32_t ox = n ? (uint32_t)tr.tx0 - x0 : 0u, oy = n ? (uint32_t)tr.ty0 - y0 : 0u;
+    const float *rp = a.isect_rec + ((size_t)block_pos(a.bbase, a.nb, s, g, a.offs[vg]) + oy * w + ox) * REC9;
+    const uint64_t *cut0 = a.tile_cut + V.tile_base + (size_t)(y0 + oy) * V.tiles_x + (x0 + ox);
     bool first = true;
-    // four rectangle entries per round: their cut and record loads are
+    // four tight-rectangle entries per round: their cut and record loads are
     // issued together, then folded in emit order
     constexpr uint32_t U = 4;
     for (uint32_t k0 = 0; k0 < n; k0 += U) {
@@ -264,14 +272,15 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
         const uint32_t k = k0 + u;
         ok[u] = false;
         if (k < n) {
-          const uint32_t dy = k / w, dx = k - dy * w;
+          const uint32_t dy = k / tw, dx = k - dy * tw;
           ok[u] = me < cut0[dy * V.tiles_x + dx];
         }
       }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         if (!ok[u]) continue;
-        const float *src = rp + (size_t)(k0 + u) * REC9;
+        const uint32_t k = k0 + u, dy = k / tw, dx = k - dy * tw;
+        const float *src = rp + (size_t)(dy * w + dx) * REC9;
 #pragma unroll
         for (int q = 0; q < REC9; ++q) rr[u][q] = src[q];
       }
@@ -731,7 +740,8 @@ void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_v
 // split_px, else it is cloned. Replaces the decision block of
 // performDensification (gaussian-renderer.ts:3249-3326).
 __global__ __launch_bounds__(256) void k_densify_decide(const float4 *__restrict__ rec,
-                                                        const float2 *__restrict__ aux, uint32_t N,
+                                                        const float2 *__restrict__ aux,
+                                                        const uint32_t *__restrict__ counts, uint32_t N,
                                                         const DevView *__restrict__ views, int nv,
                                                         const float *__restrict__ tile_l1,
                                                         const float *__restrict__ thr, float split_px,
@@ -742,8 +752,8 @@ __global__ __launch_bounds__(256) void k_densify_decide(const float4 *__restrict
   float rmax = 0.0f;
   for (int s = 0; s < nv; ++s) {
     const size_t vg = (size_t)s * N + g;
+    if (counts[vg] == 0u) continue;
     const float4 q3 = rec[vg * 4 + 3];
-    if (__float_as_uint(q3.w) == 0u) continue;
     rmax = fmaxf(rmax, aux[vg].y);
     if (cand) continue;
     const DevView &V = views[s];
@@ -760,11 +770,11 @@ __global__ __launch_bounds__(256) void k_densify_decide(const float4 *__restrict
   actions[g] = !cand ? 0u : (rmax > split_px ? 2u : 1u);
 }
 
-void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, uint32_t N,
+void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, const uint32_t *counts, uint32_t N,
                            const DevView *views, int nv, const float *tile_l1, const float *thr,
                            float split_px, uint8_t *actions) {
   if (N == 0) return;
-  hipLaunchKernelGGL(k_densify_decide, dim3((N + 255) / 256), dim3(256), 0, st, rec, aux, N, views, nv,
+  hipLaunchKernelGGL(k_densify_decide, dim3((N + 255) / 256), dim3(256), 0, st, rec, aux, counts, N, views, nv,
                      tile_l1, thr, split_px, actions);
 }
 

@@ -19,8 +19,8 @@ namespace sv {
 //   q0 = (mean.x, mean.y, conic.a, conic.b)
 //   q1 = (conic.c, opacity, r, g)
 //   q2 = (b, power_max, ext_x, ext_y)
-//   q3 = (first intersection (set by emit), rect x0|y0<<16, rect x1|y1<<16,
-//         tiles_count)
+//   q3 = (emit base within the projection block, rect x0|y0<<16,
+//         rect x1|y1<<16, tight-rectangle insets)
 // and aux = (depth, radius). power_max = ln(255 opacity) + 0.02: a pair with
 // a larger power has opacity * exp(-power) < 1/255 even with the ulp error
 // of exp, so it would be skipped. ext_x, ext_y bound the region power <=
@@ -73,6 +73,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
                                                  uint32_t *__restrict__ max_dq,
                                                  uint32_t *__restrict__ scounts,
                                                  uint2 *__restrict__ rects,
+                                                 uint32_t *__restrict__ tins,
                                                  uint32_t *__restrict__ offs_loc,
                                                  uint32_t *__restrict__ soffs_loc,
                                                  uint32_t *__restrict__ btot,
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
       float2 ax = make_float2(0.f, 0.f);
       uint32_t count = 0, scount = 0;
       uint2 rect = make_uint2(0u, 0u);
+      uint32_t tin = TIGHT_EMPTY;
       if (alive) {
         const float tx = pc[0], ty = pc[1], tz = pc[2];
         const float fx = V.focal[0], fy = V.focal[1];
@@ -212,8 +214,9 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           r0 = make_float4(mx, my, aa, bb);
           r1 = make_float4(cc, opac, rgb[0], rgb[1]);
           r2 = make_float4(rgb[2], pmax, ex, ey);
+          tin = count ? tight_insets(mx, ex, my, ey, x0, y0, x1, y1) : TIGHT_EMPTY;
           r3 = make_float4(0.0f, __uint_as_float(x0 | (y0 << 16)), __uint_as_float(x1 | (y1 << 16)),
-                           __uint_as_float(count));
+                           __uint_as_float(tin));
           ax = make_float2(depth, radius);
         }
       }
@@ -232,6 +235,7 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
         counts[vg] = count;
         if (scounts) scounts[vg] = scount;
         rects[vg] = rect;
+        tins[vg] = tin;
         offs_loc[vg] = pa;
         if (soffs_loc) soffs_loc[vg] = pb;
       }
@@ -268,12 +272,12 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
 
 void launch_project(hipStream_t st, const float *P, size_t cap, uint32_t N, const DevView *views,
                     int nv, const DevConfig &cfg, float4 *rec, float2 *aux, uint32_t *counts,
-                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects, uint32_t *offs_loc,
+                    uint32_t *max_dq, uint32_t *scounts, uint2 *rects, uint32_t *tins, uint32_t *offs_loc,
                     uint32_t *soffs_loc, uint32_t *btot, uint32_t *sbtot) {
   if (N == 0) return;
   const uint32_t blocks = project_blocks(N);
   hipLaunchKernelGGL(k_project, dim3(blocks), dim3(256), 0, st, P, cap, N, views, nv, cfg, rec, aux,
-                     counts, max_dq, scounts, rects, offs_loc, soffs_loc, btot, sbtot);
+                     counts, max_dq, scounts, rects, tins, offs_loc, soffs_loc, btot, sbtot);
 }
 
 // ---------------------------------------------------------------------

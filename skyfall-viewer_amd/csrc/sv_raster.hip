@@ -212,6 +212,71 @@ __device__ __forceinline__ QuadBounds quad_bounds(const TileGeom &t, const DevVi
   return b;
 }
 
+// Minimum over the quadrant's pixel-centre rectangle of the power p(d) =
+// 0.5 (a dx^2 + c dy^2) + b dx dy (a convex quadratic with its minimum 0 at
+// the mean). With the mean outside the rectangle the minimum lies on an edge
+// that faces the mean (on the segment from any point of the rectangle to the
+// mean, p decreases, and the segment leaves the rectangle through such an
+// edge); along the edge x = xe the minimum is at dy = -b dx / c, clamped to
+// the edge, and likewise for y = ye. Also returns the sum of the absolute
+// terms at the evaluated points (the scale of their rounding error).
+__device__ __forceinline__ float rect_min_power(float mx, float my, float a, float b, float c, float inv_a,
+                                                float inv_c, float x0, float x1, float y0, float y1,
+                                                float *scale) {
+  const float dx = fminf(fmaxf(mx, x0), x1) - mx;  // 0 when mx is inside [x0, x1]
+  const float dy = fminf(fmaxf(my, y0), y1) - my;
+  float best = 0.0f, s = 0.0f;
+  if (dx != 0.0f || dy != 0.0f) {
+    best = INFINITY;
+    if (dx != 0.0f) {
+      const float t = fminf(fmaxf(-b * dx * inv_c, y0 - my), y1 - my);
+      best = fminf(best, 0.5f * (a * dx * dx + c * t * t) + b * dx * t);
+      s = 0.5f * (a * dx * dx + c * t * t) + fabsf(b * dx * t);
+    }
+    if (dy != 0.0f) {
+      const float t = fminf(fmaxf(-b * dy * inv_a, x0 - mx), x1 - mx);
+      best = fminf(best, 0.5f * (a * t * t + c * dy * dy) + b * t * dy);
+      s = fmaxf(s, 0.5f * (a * t * t + c * dy * dy) + fabsf(b * t * dy));
+    }
+  }
+  *scale = s;
+  return best;
+}
+
+// Quadrants of the tile that an entry can colour: bit h + 2 v for the
+// quadrant of column h, row v. First the box |dx| <= ext_x, |dy| <= ext_y
+// around the mean; then the power over the quadrant's pixel centres must
+// come within power_max. That bound is kept conservative: power_max already
+// sits 0.02 above ln(255 opacity), and the minimum must exceed it by more
+// than 2e-6 times the size of the terms (the rounding of this estimate and
+// of the per-pixel power, a few float ulps of the largest term, on any pixel
+// inside the box). A quadrant dropped here holds no pixel that takes the
+// entry.
+__device__ __forceinline__ uint32_t quad_mask(const float4 &q0, float c, const float4 &q2, const QuadBounds &qb) {
+  bool cx[2], cy[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    cx[h] = qb.x1[h] >= qb.x0[h] && fabsf(q0.x - fminf(fmaxf(q0.x, qb.x0[h]), qb.x1[h])) <= q2.z;
+    cy[h] = qb.y1[h] >= qb.y0[h] && fabsf(q0.y - fminf(fmaxf(q0.y, qb.y0[h]), qb.y1[h])) <= q2.w;
+  }
+  uint32_t qm = (uint32_t)(cx[0] && cy[0]) | ((uint32_t)(cx[1] && cy[0]) << 1) |
+                ((uint32_t)(cx[0] && cy[1]) << 2) | ((uint32_t)(cx[1] && cy[1]) << 3);
+  if (qm == 0u) return 0u;
+  const float a = q0.z, b = q0.w;
+  const float inv_a = __builtin_amdgcn_rcpf(a), inv_c = __builtin_amdgcn_rcpf(c);
+  // the largest sum of absolute terms at a pixel inside the box
+  const float sbox = 0.5f * (a * q2.z * q2.z + c * q2.w * q2.w) + fabsf(b) * q2.z * q2.w;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (!((qm >> j) & 1u)) continue;
+    float s;
+    const float mn = rect_min_power(q0.x, q0.y, a, b, c, inv_a, inv_c, qb.x0[j & 1], qb.x1[j & 1], qb.y0[j >> 1],
+                                    qb.y1[j >> 1], &s);
+    if (mn > q2.y + 2e-6f * (sbox + s)) qm &= ~(1u << j);
+  }
+  return qm;
+}
+
 template <int B>
 __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
                                                uint32_t cnt, const QuadBounds &qb,
@@ -231,21 +296,20 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       q0 = q[0];
       q1 = q[1];
       q2 = q[2];
-      bool cx[2], cy[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        cx[h] = qb.x1[h] >= qb.x0[h] && fabsf(q0.x - fminf(fmaxf(q0.x, qb.x0[h]), qb.x1[h])) <= q2.z;
-        cy[h] = qb.y1[h] >= qb.y0[h] && fabsf(q0.y - fminf(fmaxf(q0.y, qb.y0[h]), qb.y1[h])) <= q2.w;
-      }
-      qm = (uint32_t)(cx[0] && cy[0]) | ((uint32_t)(cx[1] && cy[0]) << 1) | ((uint32_t)(cx[0] && cy[1]) << 2) |
-           ((uint32_t)(cx[1] && cy[1]) << 3);
+      qm = quad_mask(q0, q1.x, q2, qb);
       touch = qm != 0;
       if (s_p) {
         q3 = q[3];
         const uint32_t p = emit_pos(q3, a.bbase[(size_t)slot * a.nb + (id >> 8)], tx, ty);
         s_p[k] = p;
-        // an entry that touches no pixel of the tile contributes nothing
-        if (!touch && (uint64_t)p < a.n_isect) {
+        // an entry that touches no pixel of the tile contributes nothing:
+        // a zero record inside the Gaussian's tight rectangle (the record
+        // gather reads only there), no record outside it
+        const uint32_t ru = __float_as_uint(q3.y), rw = __float_as_uint(q3.z);
+        const TightRect tr = tight_rect(__float_as_uint(q3.w), ru & 0xffffu, ru >> 16, rw & 0xffffu, rw >> 16);
+        const bool in_tight = (int32_t)tx >= tr.tx0 && (int32_t)tx < tr.tx1 && (int32_t)ty >= tr.ty0 &&
+                              (int32_t)ty < tr.ty1;
+        if (!touch && in_tight && (uint64_t)p < a.n_isect) {
           float *o = a.isect_rec + (size_t)p * REC9;
 #pragma unroll
           for (int q9 = 0; q9 < REC9; ++q9) o[q9] = 0.0f;
@@ -281,6 +345,21 @@ __device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0,
   return in && !(*as < SV_MIN_ALPHA);
 }
 
+// Front-to-back compositing of one taken entry into one pixel; the pixel
+// is done, with the entry as its last index, once 1 - A < 1e-4.
+__device__ __forceinline__ void composite(float as, float r, float g, float b, uint32_t idx, float &c0, float &c1,
+                                          float &c2, float &A, bool &done, uint32_t &last) {
+  const float contrib = as * (1.0f - A);
+  c0 = c0 + r * contrib;
+  c1 = c1 + g * contrib;
+  c2 = c2 + b * contrib;
+  A = A + contrib;
+  if (1.0f - A < SV_T_EPS) {
+    done = true;
+    last = idx;
+  }
+}
+
 // ---------------------------------------------------------------------
 // training_tiled_forward (training-forward-tiles.wgsl:107-215). Per pixel
 // (centre +0.5) over the tile list front to back: power = 0.5(a dx^2 +
@@ -291,7 +370,7 @@ __device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0,
 // Empty tile: bg, A = 0, last = NO_LAST.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
+__global__ __launch_bounds__(WT, 5) void k_forward(RasterArgs ain) {
   RasterArgs a = ain;
   if (!resolve_meta(a)) return;
   __shared__ float4 s_q0[B];
@@ -358,28 +437,9 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
             float asa, asb;
             const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, a2, &asa);
             const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, b2, &asb);
-            if (va) {
-              const float contrib = asa * (1.0f - A[j]);
-              c0[j] = c0[j] + a1.z * contrib;
-              c1[j] = c1[j] + a1.w * contrib;
-              c2[j] = c2[j] + a2.x * contrib;
-              A[j] = A[j] + contrib;
-              if (1.0f - A[j] < SV_T_EPS) {
-                done[j] = true;
-                last[j] = b0 + (ka & 0xFFFFu);
-              }
-            }
-            if (vb && !done[j]) {
-              const float contrib = asb * (1.0f - A[j]);
-              c0[j] = c0[j] + b1.z * contrib;
-              c1[j] = c1[j] + b1.w * contrib;
-              c2[j] = c2[j] + b2.x * contrib;
-              A[j] = A[j] + contrib;
-              if (1.0f - A[j] < SV_T_EPS) {
-                done[j] = true;
-                last[j] = b0 + (kb & 0xFFFFu);
-              }
-            }
+            if (va) composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+            if (vb && !done[j])
+              composite(asb, b1.z, b1.w, b2.x, b0 + (kb & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
           }
           if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
         }
@@ -392,15 +452,7 @@ __global__ __launch_bounds__(WT) void k_forward(RasterArgs ain) {
             if (done[j]) continue;
             float asa;
             if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, a2, &asa)) continue;
-            const float contrib = asa * (1.0f - A[j]);
-            c0[j] = c0[j] + a1.z * contrib;
-            c1[j] = c1[j] + a1.w * contrib;
-            c2[j] = c2[j] + a2.x * contrib;
-            A[j] = A[j] + contrib;
-            if (1.0f - A[j] < SV_T_EPS) {
-              done[j] = true;
-              last[j] = b0 + (ka & 0xFFFFu);
-            }
+            composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
           }
         }
         if (__syncthreads_and((done[0] && done[1]) && (done[2] && done[3]))) break;
