@@ -609,24 +609,28 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               hit = true;
               const float oma = fmaxf(1.0f - as, 0.0001f);
               const float Tb = div_unscaled(T[j], oma);
+              // the gradient terms in fused multiply-adds (they feed only
+              // sums; the alpha and T chain above stays the forward's)
               const float contrib = as * Tb;
-              v0 = v0 + r0[j] * contrib;
-              v1 = v1 + r1[j] * contrib;
-              v2 = v2 + r2[j] * contrib;
+              v0 = fmaf(r0[j], contrib, v0);
+              v1 = fmaf(r1[j], contrib, v1);
+              v2 = fmaf(r2[j], contrib, v2);
               const float cr = x1.z, cg = x1.w, cb = x2.x;
-              const float fg = (r0[j] * cr + r1[j] * cg) + r2[j] * cb;
-              const float bgt = (r0[j] * acc0[j] + r1[j] * acc1[j]) + r2[j] * acc2[j];
+              const float fg = fmaf(r2[j], cb, fmaf(r1[j], cg, r0[j] * cr));
+              const float bgt = fmaf(r2[j], acc2[j], fmaf(r1[j], acc1[j], r0[j] * acc0[j]));
               const float dL_dalpha = (fg - bgt) * Tb;
-              v3 = v3 + (dL_dalpha * as) * (1.0f - op);
-              const float dL_ds = (dL_dalpha * op) * (-gauss);
-              v4 = v4 + ((dL_ds * 0.5f) * dx) * dx;
-              v5 = v5 + (dL_ds * dx) * dy;
-              v6 = v6 + ((dL_ds * 0.5f) * dy) * dy;
-              v7 = v7 - dL_ds * (x0.z * dx + x0.w * dy);
-              v8 = v8 - dL_ds * (x0.w * dx + x1.x * dy);
-              acc0[j] = as * cr + (1.0f - as) * acc0[j];
-              acc1[j] = as * cg + (1.0f - as) * acc1[j];
-              acc2[j] = as * cb + (1.0f - as) * acc2[j];
+              v3 = fmaf(dL_dalpha * as, 1.0f - op, v3);
+              const float nds = dL_dalpha * (op * gauss);  // -dL/dpower
+              const float hds = 0.5f * nds;
+              v4 = fmaf(-hds * dx, dx, v4);
+              v5 = fmaf(-nds * dx, dy, v5);
+              v6 = fmaf(-hds * dy, dy, v6);
+              v7 = fmaf(nds, fmaf(x0.z, dx, x0.w * dy), v7);
+              v8 = fmaf(nds, fmaf(x0.w, dx, x1.x * dy), v8);
+              const float om = 1.0f - as;
+              acc0[j] = fmaf(as, cr, om * acc0[j]);
+              acc1[j] = fmaf(as, cg, om * acc1[j]);
+              acc2[j] = fmaf(as, cb, om * acc2[j]);
               T[j] = Tb;
             }
           }
