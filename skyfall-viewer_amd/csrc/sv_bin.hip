@@ -340,15 +340,30 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
   }
   __syncthreads();
   // groups of tiles [t0, t1) whose entries fit the stage (one tile has at
-  // most BIN_CH <= BIN_STAGE entries, so every group holds a tile)
-  for (uint32_t t0 = 0; t0 < 64;) {
-    uint32_t t1 = t0, sum = 0;
-    while (t1 < 64 && sum + s_tot[t1] <= BIN_STAGE) sum += s_tot[t1++];
-    if (tid < 64) {
-      uint32_t o = 0;
-      for (uint32_t t = t0; t < tid; ++t) o += s_tot[t];
-      s_soff[tid] = tid < t0 ? 0u : (tid < t1 ? o : 0xffffffffu);
+  // most BIN_CH <= BIN_STAGE entries, so every group holds a tile). The
+  // first wave scans the tile totals; with all of them within the stage
+  // (the usual case) there is one group.
+  __shared__ uint32_t s_incl[64];
+  if (tid < 64) {
+    uint32_t x = s_tot[tid];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+      if (lane >= (uint32_t)o) x += y;
     }
+    s_incl[tid] = x;
+  }
+  __syncthreads();
+  for (uint32_t t0 = 0; t0 < 64;) {
+    // the group is the longest run of tiles from t0 whose entries fit
+    const uint32_t before = t0 ? s_incl[t0 - 1] : 0u;
+    uint32_t t1 = t0 + 1;
+    if (s_incl[63] - before <= BIN_STAGE) {
+      t1 = 64;
+    } else {
+      while (t1 < 64 && s_incl[t1] - before <= BIN_STAGE) ++t1;
+    }
+    if (tid < 64) s_soff[tid] = (tid >= t0 && tid < t1) ? (tid ? s_incl[tid - 1] : 0u) - before : 0xffffffffu;
     __syncthreads();
     // this lane's tiles in the group, in order; rank = covering lanes below
     const uint64_t gmask = (t1 == 64 ? ~0ull : ((1ull << t1) - 1ull)) & ~((1ull << t0) - 1ull);
@@ -358,15 +373,14 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
       if (idx < BIN_STAGE) s_stage[idx] = g;
     }
     __syncthreads();
-    for (uint32_t e = tid; e < sum; e += BIN_CH) {
-      // the last tile whose stage offset is <= e (empty tiles share the
-      // next tile's offset, so the last one holding e wins)
-      uint32_t t = 0;
-#pragma unroll
-      for (uint32_t step = 32; step > 0; step >>= 1)
-        if (s_soff[t + step] <= e) t += step;
-      const uint32_t pos = s_base[t] + (e - s_soff[t]);
-      if (pos < s_end[t]) out[pos] = s_stage[e];  // the counts bound every list
+    // each wave stores whole tiles: a tile's entries are one contiguous run
+    // of its list
+    for (uint32_t t = t0 + wave; t < t1; t += BIN_CH / 64) {
+      const uint32_t n = s_tot[t], so = s_soff[t], b = s_base[t], e = s_end[t];
+      for (uint32_t i = lane; i < n; i += 64) {
+        const uint32_t pos = b + i;
+        if (pos < e) out[pos] = s_stage[so + i];  // the counts bound every list
+      }
     }
     __syncthreads();
     t0 = t1;

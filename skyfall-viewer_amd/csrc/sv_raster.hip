@@ -132,22 +132,15 @@ __device__ __forceinline__ float opaque_zero() {
   return z;
 }
 
-// a / b correctly rounded, for a = 0 or a in [2^-100, 2^10] and b in
-// [1e-4, 1] (the backward's T / max(1 - alpha, 1e-4), T in (1e-7, 1]):
-// the refinement that the IEEE division sequence (v_div_scale, v_rcp, three
-// fma corrections, v_div_fmas, v_div_fixup) performs, without the scaling
-// and fix-up steps, which leave operands in these ranges unchanged
-// (numerator, reciprocal and quotient far from the denormal and overflow
-// ranges). Same bits as a / b.
-__device__ __forceinline__ float div_unscaled(float a, float b) {
+// a / b within about one ulp, for b in [1e-4, 1] (the backward's
+// T / max(1 - alpha, 1e-4)): the hardware reciprocal refined by one Newton
+// step, times a. The transmittance it rebuilds feeds only gradient sums
+// (which pairs a pixel takes is decided by alpha alone), so the division
+// need not be the correctly rounded one.
+__device__ __forceinline__ float div_approx(float a, float b) {
   float r = __builtin_amdgcn_rcpf(b);
-  const float e = fmaf(-b, r, 1.0f);
-  r = fmaf(e, r, r);
-  float q = a * r;
-  const float e2 = fmaf(-b, q, a);
-  q = fmaf(e2, r, q);
-  const float e3 = fmaf(-b, q, a);
-  return fmaf(e3, r, q);
+  r = fmaf(fmaf(-b, r, 1.0f), r, r);
+  return a * r;
 }
 
 __device__ __forceinline__ float sign_ref(float x) {
@@ -608,7 +601,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               if (!take) continue;
               hit = true;
               const float oma = fmaxf(1.0f - as, 0.0001f);
-              const float Tb = div_unscaled(T[j], oma);
+              const float Tb = div_approx(T[j], oma);
               // the gradient terms in fused multiply-adds (they feed only
               // sums; the alpha and T chain above stays the forward's)
               const float contrib = as * Tb;
