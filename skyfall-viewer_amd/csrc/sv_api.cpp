@@ -473,7 +473,10 @@ void key_plan(sv_ctx *c) {
   // binned build: supertile key + depth bits (the device checks the live
   // width against the 32-bit key with the rectangle bits)
   c->super_bits = bitlen(c->super_total > 0 ? c->super_total - 1 : 0);
-  c->max_super_passes = (int)std::min<uint32_t>(4u, (c->super_bits + bitlen(dq_max) + 7) / 8);
+  // a binned step's key has at most 32 - BIN_RECT_BITS bits (wider keys
+  // fall back to the pair sort), so three 8-bit passes cover it
+  c->max_super_passes =
+      (int)std::min<uint32_t>((32u - BIN_RECT_BITS + 7u) / 8u, (c->super_bits + bitlen(dq_max) + 7) / 8);
   c->binned = c->cfg.binned_build && !c->bin_unfit && c->super_bits + BIN_RECT_BITS < 32;
 }
 
