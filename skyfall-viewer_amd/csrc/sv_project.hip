@@ -25,9 +25,9 @@ namespace sv {
 // a larger power has opacity * exp(-power) < 1/255 even with the ulp error
 // of exp, so it would be skipped. ext_x, ext_y bound the region power <=
 // power_max: max |dx| = sqrt(2 power_max (Q^-1)_xx) for the conic Q, with a
-// 1% + 0.01 px margin for rounding. The rasterizers use both to skip pairs
-// without evaluating them; every skipped pair is one the reference skips,
-// so no result changes.
+// 1% + 0.01 px margin for rounding. The rasterizers' batch loads use both
+// to skip whole quadrants of a tile without evaluating their pixels; every
+// skipped pair is one the reference skips, so no result changes.
 // Exclusive prefix of two u32 values over the 256 threads of the block,
 // and their block totals (wave scans, then the four wave totals).
 __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t &pa, uint32_t &pb, uint32_t &ta,

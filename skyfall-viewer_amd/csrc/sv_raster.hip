@@ -322,20 +322,19 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
   return kept;
 }
 
-// Alpha of one splat record (q0, q1, q2) at pixel centre (fx, fy), and
+// Alpha of one splat record (q0, q1) at pixel centre (fx, fy), and
 // whether the pixel takes it (training-forward-tiles.wgsl:183-197): power
 // = 0.5 (a dx^2 + c dy^2) + b dx dy, skipped if < 0; alpha = min(opacity
-// e^-power, 0.999), skipped if < 1/255. The extent and power_max tests
-// before the exp only drop pairs whose alpha is below 1/255.
-__device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0, const float4 &x1,
-                                           const float4 &x2, float *as) {
+// e^-power, 0.999), skipped if < 1/255. The power goes into the exp
+// clamped to 80 (the exp's argument range): e^-80 opacity is far below
+// 1/255, so a clamped pair is skipped as the unclamped one is.
+__device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0, const float4 &x1, float *as) {
   const float dx = fx - x0.x;
   const float dy = fy - x0.y;
   const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-  const bool in = !(fabsf(dx) > x2.z || fabsf(dy) > x2.w) && !(sg < 0.0f || sg > x2.y);
-  const float gauss = sv_expf_neg(-sg);
+  const float gauss = sv_expf_neg(-fminf(sg, 80.0f));
   *as = fminf(x1.y * gauss, SV_MAX_ALPHA);
-  return in && !(*as < SV_MIN_ALPHA);
+  return !(sg < 0.0f) && !(*as < SV_MIN_ALPHA);
 }
 
 // Front-to-back compositing of one taken entry into one pixel; the pixel
@@ -428,8 +427,8 @@ __global__ __launch_bounds__(WT, 5) void k_forward(RasterArgs ain) {
             if (!(ua || ub)) continue;
             if (done[j]) continue;
             float asa, asb;
-            const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, a2, &asa);
-            const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, b2, &asb);
+            const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa);
+            const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, &asb);
             if (va) composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
             if (vb && !done[j])
               composite(asb, b1.z, b1.w, b2.x, b0 + (kb & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
@@ -444,7 +443,7 @@ __global__ __launch_bounds__(WT, 5) void k_forward(RasterArgs ain) {
             if (!((ka >> (16 + j)) & 1u)) continue;
             if (done[j]) continue;
             float asa;
-            if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, a2, &asa)) continue;
+            if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa)) continue;
             composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
           }
         }
@@ -593,11 +592,10 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               const float dx = fxh[j & 1] - x0.x;
               const float dy = fyh[j >> 1] - x0.y;
               const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-              const float gauss = sv_expf_neg(-sg);
+              const float gauss = sv_expf_neg(-fminf(sg, 80.0f));  // as frag_alpha
               const float op = x1.y;
               const float as = fminf(op * gauss, SV_MAX_ALPHA);
-              const bool take = (valid[j] & (i <= lastp[j])) & !(fabsf(dx) > x2.z || fabsf(dy) > x2.w) &
-                                !(sg < 0.0f || sg > x2.y) & !(as < SV_MIN_ALPHA);
+              const bool take = (valid[j] & (i <= lastp[j])) & !(sg < 0.0f) & !(as < SV_MIN_ALPHA);
               if (!take) continue;
               hit = true;
               const float oma = fmaxf(1.0f - as, 0.0001f);
