@@ -237,14 +237,15 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   if (a.meta && a.meta->overflow) return;
   const size_t M = (size_t)a.nv * a.N;
-  const size_t vg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (vg >= M) return;
+  // grid: x over Gaussians, y over view slots
+  const uint32_t s = blockIdx.y, g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.N) return;
+  const size_t vg = (size_t)s * a.N + g;
   float r[REC9];
 #pragma unroll
   for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
   const uint32_t cnt = a.counts[vg];
   if (cnt) {
-    const uint32_t s = (uint32_t)(vg / a.N), g = (uint32_t)(vg - (size_t)s * a.N);
     const DevView &V = a.views[s];
     const uint2 rc = a.rects[vg];
     const uint32_t ra = rc.x, rb = rc.y;
@@ -264,23 +265,26 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     // four tight-rectangle entries per round: their cut and record loads are
     // issued together, then folded in emit order
     constexpr uint32_t U = 4;
+    // (cx, cy): column and row of entry k0, stepped without divisions
+    uint32_t cx = 0, cy = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += U) {
       bool ok[U];
+      uint32_t ex[U], ey[U];
       float rr[U][REC9];
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
-        const uint32_t k = k0 + u;
-        ok[u] = false;
-        if (k < n) {
-          const uint32_t dy = k / tw, dx = k - dy * tw;
-          ok[u] = me < cut0[dy * V.tiles_x + dx];
+        ex[u] = cx;
+        ey[u] = cy;
+        if (++cx == tw) {
+          cx = 0;
+          ++cy;
         }
+        ok[u] = k0 + u < n && me < cut0[ey[u] * V.tiles_x + ex[u]];
       }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {
         if (!ok[u]) continue;
-        const uint32_t k = k0 + u, dy = k / tw, dx = k - dy * tw;
-        const float *src = rp + (size_t)(dy * w + dx) * REC9;
+        const float *src = rp + (size_t)(ey[u] * w + ex[u]) * REC9;
 #pragma unroll
         for (int q = 0; q < REC9; ++q) rr[u][q] = src[q];
       }
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
 void launch_gather_records(hipStream_t st, const VjpArgs &a) {
   const size_t M = (size_t)a.nv * a.N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_gather_records, dim3((uint32_t)((M + 255) / 256)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_gather_records, dim3((a.N + 255) / 256, (uint32_t)a.nv), dim3(256), 0, st, a);
 }
 
 // The per-Gaussian backward runs as two kernels with the same arithmetic
