@@ -117,7 +117,11 @@ __global__ __launch_bounds__(BIN_T) void k_emit_super(const uint2 *__restrict__ 
       if (s_lo[j + step] <= k) j += step;
     const uint32_t local = k - s_lo[j];
     const uint32_t sr = s_srect[j], sw = (sr >> 8) & 0xffu;
-    const uint32_t dy = local / sw, dx = local - dy * sw;
+    // local < 2^16 and sw < 2^8: (local + 0.5) / sw in float is at least
+    // 0.5 / sw from an integer, far beyond its rounding error, so the
+    // truncation is the integer quotient
+    const uint32_t dy = (uint32_t)(((float)local + 0.5f) * __builtin_amdgcn_rcpf((float)sw));
+    const uint32_t dx = local - dy * sw;
     const uint32_t sx = (sr & 0xffu) + dx, sy = (sr >> 16) + dy;
     // the tile rectangle clipped to supertile (sx, sy), local to it
     const uint32_t a = s_trect0[j], b = s_trect1[j];
@@ -273,11 +277,17 @@ __global__ __launch_bounds__(64) void k_tile_totals(const uint32_t *__restrict__
   const uint32_t tx = (ls % V.super_x) * BIN_S + (t & 7u), ty = (ls / V.super_x) * BIN_S + (t >> 3);
   const uint32_t b = cbase[s], c = nch[s];
   uint32_t run = 0;
-  for (uint32_t k = 0; k < c; ++k) {
-    uint32_t *p = &ccount[(size_t)(b + k) * 64 + t];
-    const uint32_t v = *p;
-    *p = run;
-    run += v;
+  // eight chunks per round: their counts load together, then the prefix
+  constexpr uint32_t R = 8;
+  for (uint32_t k0 = 0; k0 < c; k0 += R) {
+    uint32_t v[R];
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) v[u] = k0 + u < c ? ccount[(size_t)(b + k0 + u) * 64 + t] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < R; ++u) {
+      if (k0 + u < c) ccount[(size_t)(b + k0 + u) * 64 + t] = run;
+      run += v[u];
+    }
   }
   if (tx < V.tiles_x && ty < V.tiles_y) tcount[V.tile_base + ty * V.tiles_x + tx] = run;
 }
