@@ -362,7 +362,7 @@ __device__ __forceinline__ void composite(float as, float r, float g, float b, u
 // Empty tile: bg, A = 0, last = NO_LAST.
 // ---------------------------------------------------------------------
 template <int B>
-__global__ __launch_bounds__(WT, 6) void k_forward(RasterArgs ain) {
+__global__ __launch_bounds__(WT, 7) void k_forward(RasterArgs ain) {
   RasterArgs a = ain;
   if (!resolve_meta(a)) return;
   __shared__ float4 s_q0[B];
