@@ -312,7 +312,7 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
                                                        const uint32_t *__restrict__ ranges,
                                                        uint32_t *__restrict__ out) {
   __shared__ uint32_t s_pre[BIN_CH / 64][64];  // per (wave, tile): covering pairs of earlier waves
-  __shared__ uint64_t s_col[BIN_CH / 64][64];  // per (wave, tile): the wave's pairs covering the tile
+  __shared__ uint32_t s_gid[BIN_CH / 64][64];  // per (wave, lane): the pair's Gaussian
   __shared__ uint32_t s_tot[64], s_base[64], s_end[64], s_soff[64];
   __shared__ uint32_t s_stage[BIN_STAGE];
   const Chunk ch = chunk_of(blockIdx.x, m, chunk_super, cbase, sranges);
@@ -321,11 +321,10 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
   uint32_t r = 0, g = 0;
   const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, kb, &r, &g);
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint64_t mine = ok ? cover_mask(r) : 0ull;
   const uint64_t col = wave_transpose64(mine);
-  s_col[wave][lane] = col;
   s_pre[wave][lane] = (uint32_t)__popcll(col);
+  s_gid[wave][lane] = g;
   __syncthreads();
   if (tid < 64) {
     uint32_t run = 0;
@@ -375,12 +374,15 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
     }
     if (tid < 64) s_soff[tid] = (tid >= t0 && tid < t1) ? (tid ? s_incl[tid - 1] : 0u) - before : 0xffffffffu;
     __syncthreads();
-    // this lane's tiles in the group, in order; rank = covering lanes below
-    const uint64_t gmask = (t1 == 64 ? ~0ull : ((1ull << t1) - 1ull)) & ~((1ull << t0) - 1ull);
-    for (uint64_t mm = mine & gmask; mm; mm &= mm - 1ull) {
-      const uint32_t t = (uint32_t)__ffsll((unsigned long long)mm) - 1u;
-      const uint32_t idx = s_soff[t] + s_pre[wave][t] + (uint32_t)__popcll(s_col[wave][t] & lt_mask);
-      if (idx < BIN_STAGE) s_stage[idx] = g;
+    // lane t stages tile t's entries from this wave's pairs: the pairs
+    // covering it (bits of its transposed column), in pair order, at the
+    // tile's stage offset plus the covering pairs of earlier waves
+    if (lane >= t0 && lane < t1) {
+      uint32_t idx = s_soff[lane] + s_pre[wave][lane];
+      for (uint64_t mm = col; mm; mm &= mm - 1ull, ++idx) {
+        const uint32_t p = (uint32_t)__ffsll((unsigned long long)mm) - 1u;
+        if (idx < BIN_STAGE) s_stage[idx] = s_gid[wave][p];
+      }
     }
     __syncthreads();
     // each wave stores whole tiles: a tile's entries are one contiguous run
