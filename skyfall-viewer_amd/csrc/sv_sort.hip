@@ -55,12 +55,19 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
   // there, lanes with equal digits are matched by ballots and counted with
   // one LDS add by the lowest of them (per-lane atomics would serialise on
   // the few addresses); otherwise one LDS atomic per key.
+  // the tile's keys load together first
+  K kk[SORT_ITEMS];
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 64 + lane;
+    kk[j] = i < n ? keys[i] : (K)0;
+  }
   if (m->key_bits <= shift + 4u) {
-#pragma unroll 4
+#pragma unroll
     for (int j = 0; j < SORT_ITEMS; ++j) {
       const uint64_t i = base + (uint64_t)j * 64 + lane;
       const bool valid = i < n;
-      const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & dmask : 0u;
+      const uint32_t d = valid ? (uint32_t)(kk[j] >> shift) & dmask : 0u;
       uint64_t mk = __ballot(valid);
 #pragma unroll
       for (int bit = 0; bit < 4; ++bit) {
@@ -74,7 +81,7 @@ __global__ __launch_bounds__(SORT_T) void k_sort_count(const K *__restrict__ key
 #pragma unroll
     for (int j = 0; j < SORT_ITEMS; ++j) {
       const uint64_t i = base + (uint64_t)j * 64 + lane;
-      if (i < n) atomicAdd(&s_h[wave][(uint32_t)(keys[i] >> shift) & dmask], 1u);
+      if (i < n) atomicAdd(&s_h[wave][(uint32_t)(kk[j] >> shift) & dmask], 1u);
     }
   }
   __syncthreads();
@@ -179,12 +186,19 @@ __global__ __launch_bounds__(SORT_T) void k_sort_scatter(const K *__restrict__ k
   K key[SORT_ITEMS];
   uint32_t val[SORT_ITEMS];
   uint32_t rank[SORT_ITEMS];
+  // all the tile's loads first, so they are in flight together (the
+  // ranking below is ordered through LDS, item by item)
 #pragma unroll
   for (int j = 0; j < SORT_ITEMS; ++j) {
     const uint64_t i = base + (uint64_t)j * 64 + lane;
     const bool valid = i < n;
     key[j] = valid ? kin[i] : (K)0;
     val[j] = valid ? vin[i] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < SORT_ITEMS; ++j) {
+    const uint64_t i = base + (uint64_t)j * 64 + lane;
+    const bool valid = i < n;
     const uint32_t d = (uint32_t)(key[j] >> shift) & dmask;
     // lanes holding the same digit: AND of ballots over the 8 digit bits
     uint64_t m = __ballot(valid);
