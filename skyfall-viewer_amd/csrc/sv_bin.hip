@@ -244,21 +244,26 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_count(const uint32_t *__restric
                                                        const uint32_t *__restrict__ chunk_super, const StepMeta *m,
                                                        uint32_t *__restrict__ ccount) {
   __shared__ uint32_t s_cnt[BIN_CH / 64][64];
-  const Chunk ch = chunk_of(blockIdx.x, m, chunk_super, cbase, sranges);
-  if (!ch.live) return;
   const uint32_t sel = m[1].sel, kb = m[1].key_bits;
-  uint32_t r = 0, g = 0;
-  const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, kb, &r, &g);
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // lane t: the wave's pairs covering tile t
-  const uint64_t col = wave_transpose64(ok ? cover_mask(r) : 0ull);
-  s_cnt[wave][lane] = (uint32_t)__popcll(col);
-  __syncthreads();
-  if (threadIdx.x < 64) {
-    uint32_t tot = 0;
+  // a fixed grid strides over the chunks (one short workgroup per chunk
+  // made the launch the cost)
+  for (uint32_t c = blockIdx.x;; c += gridDim.x) {
+    const Chunk ch = chunk_of(c, m, chunk_super, cbase, sranges);
+    if (!ch.live) return;
+    uint32_t r = 0, g = 0;
+    const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, kb, &r, &g);
+    // lane t: the wave's pairs covering tile t
+    const uint64_t col = wave_transpose64(ok ? cover_mask(r) : 0ull);
+    s_cnt[wave][lane] = (uint32_t)__popcll(col);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      uint32_t tot = 0;
 #pragma unroll
-    for (int w = 0; w < BIN_CH / 64; ++w) tot += s_cnt[w][threadIdx.x];
-    ccount[(size_t)blockIdx.x * 64 + threadIdx.x] = tot;
+      for (int w = 0; w < BIN_CH / 64; ++w) tot += s_cnt[w][threadIdx.x];
+      ccount[(size_t)c * 64 + threadIdx.x] = tot;
+    }
+    __syncthreads();
   }
 }
 
@@ -427,8 +432,8 @@ void launch_fill_tiles(hipStream_t st, const BinArgs &b) {
   hipLaunchKernelGGL(k_chunk_map, dim3((S + BIN_T - 1) / BIN_T), dim3(BIN_T), 0, st, b.nch, b.cbase, S,
                      b.chunk_super);
   const uint32_t grid = (uint32_t)b.chunk_cap;
-  hipLaunchKernelGGL(k_fill_count, dim3(grid), dim3(BIN_CH), 0, st, b.keys0, b.vals0, b.keys1, b.vals1, b.sranges,
-                     b.cbase, b.chunk_super, b.m, b.ccount);
+  hipLaunchKernelGGL(k_fill_count, dim3(std::min<uint32_t>(grid, 2048u)), dim3(BIN_CH), 0, st, b.keys0, b.vals0,
+                     b.keys1, b.vals1, b.sranges, b.cbase, b.chunk_super, b.m, b.ccount);
   launch_fill_u32(st, b.tcount, 0u, (size_t)b.n_tiles + 1);
   hipLaunchKernelGGL(k_tile_totals, dim3(S), dim3(64), 0, st, b.nch, b.cbase, b.super_view, b.views, b.ccount,
                      b.tcount);

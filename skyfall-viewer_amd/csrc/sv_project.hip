@@ -512,23 +512,27 @@ template <typename K>
 __global__ __launch_bounds__(256) void k_ranges(const K *__restrict__ keys0, const K *__restrict__ keys1,
                                                 const StepMeta *m, uint32_t n_tiles,
                                                 uint32_t *__restrict__ ranges) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m->overflow) return;
   const uint64_t I = m->n_isect;
+  // a fixed grid strides over the live pairs (a thread per pair of the
+  // capacity would launch mostly idle blocks)
+  const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   if (I == 0) {
-    if (i <= n_tiles) ranges[i] = 0u;
+    for (uint64_t i = i0; i <= n_tiles; i += stride) ranges[i] = 0u;
     return;
   }
-  if (i >= I) return;
   const K *__restrict__ keys = m->sel ? keys1 : keys0;
   const uint32_t dbits = m->dbits;
   // bits above the key bits carry a payload (binned build)
   const K kmask = m->key_bits >= 8 * sizeof(K) ? ~(K)0 : (((K)1 << m->key_bits) - 1);
-  const uint32_t t = (uint32_t)((keys[i] & kmask) >> dbits);
-  const uint32_t lo = (i == 0) ? 0u : (uint32_t)((keys[i - 1] & kmask) >> dbits) + 1u;
-  for (uint32_t u = lo; u <= t && u <= n_tiles; ++u) ranges[u] = (uint32_t)i;
-  if (i == I - 1)
-    for (uint32_t u = t + 1u; u <= n_tiles; ++u) ranges[u] = (uint32_t)I;
+  for (uint64_t i = i0; i < I; i += stride) {
+    const uint32_t t = (uint32_t)((keys[i] & kmask) >> dbits);
+    const uint32_t lo = (i == 0) ? 0u : (uint32_t)((keys[i - 1] & kmask) >> dbits) + 1u;
+    for (uint32_t u = lo; u <= t && u <= n_tiles; ++u) ranges[u] = (uint32_t)i;
+    if (i == I - 1)
+      for (uint32_t u = t + 1u; u <= n_tiles; ++u) ranges[u] = (uint32_t)I;
+  }
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
@@ -552,8 +556,8 @@ template <typename K>
 void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap, uint32_t n_tiles,
                    const StepMeta *m, uint32_t *ranges) {
   const uint64_t threads = std::max<uint64_t>(cap, (uint64_t)n_tiles + 1);
-  hipLaunchKernelGGL(k_ranges<K>, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st, keys0, keys1,
-                     m, n_tiles, ranges);
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((threads + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_ranges<K>, dim3(blocks), dim3(256), 0, st, keys0, keys1, m, n_tiles, ranges);
 }
 template void launch_ranges<uint32_t>(hipStream_t, const uint32_t *, const uint32_t *, uint64_t, uint32_t,
                                       const StepMeta *, uint32_t *);
