@@ -171,7 +171,7 @@ def cpu_baseline(g, cams, targets, args):
     }
 
 
-def run_cfg4(args):
+def run_growth4m(args):
     """The growth run: 1M splats, 16 ring views at 1920x1080, batched steps;
     every 100 steps prune sigmoid(logit) < 0.005 and densify (split above 8
     px where a tile's L1 error is above its view's 90th percentile, else
@@ -196,7 +196,7 @@ def run_cfg4(args):
     for v in range(V):
         c.set_target(v, targets[v])
     views = list(range(V))
-    log(f"cfg4: {N0} splats, {V} views {W}x{H}, {args.warmup} warmup + {args.steps} steps")
+    log(f"growth4m: {N0} splats, {V} views {W}x{H}, {args.warmup} warmup + {args.steps} steps")
     for _ in range(args.warmup):
         c.step_views(views, sv.MODE_BATCHED)
     c.sync()
@@ -214,13 +214,13 @@ def run_cfg4(args):
                 act[idx[max(budget, 0):]] = 0
             added = c.densify(act, seed=it)
             events.append({"step": it, "pruned": removed, "added": added, "count": c.count})
-            log(f"cfg4 step {it}: pruned {removed}, added {added}, {c.count} splats, "
+            log(f"growth4m step {it}: pruned {removed}, added {added}, {c.count} splats, "
                 f"mean L1 {float(st.l1_sum) / (3.0 * V * W * H):.5f}")
     c.sync()
     dt = time.perf_counter() - t0
     P = V * W * H
     out = {
-        "metric": "train steps/s, growth run (config 4)", "value": args.steps / dt, "unit": "steps/s",
+        "metric": "train steps/s, growth run (1M to 4M splats, 16 views at 1080p)", "value": args.steps / dt, "unit": "steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
@@ -237,11 +237,11 @@ def run_cfg4(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ring16", "uneven64", "cfg0", "cfg1", "cfg4"), default="ring16",
+    ap.add_argument("--workload", choices=("ring16", "uneven64", "small256", "ring8_720p", "growth4m"), default="ring16",
                     help="ring16: 16 ring views at --width x --height (the headline); uneven64: 64 views "
                          "with seeded sizes 320-1920 x 240-1080 on a sphere of radius 4, means in [-2,2]^3; "
-                         "cfg0: 16,384 splats, one 256x256 view from (0,0,3); cfg1: 500,000 splats, 8 ring "
-                         "views at 1280x720; cfg4: the growth run (1M splats growing to at most 4M, 16 ring "
+                         "small256: 16,384 splats, one 256x256 view from (0,0,3); ring8_720p: 500,000 splats, 8 ring "
+                         "views at 1280x720; growth4m: the growth run (1M splats growing to at most 4M, 16 ring "
                          "views at 1080p, prune / densify every 100 steps; give --steps, e.g. 2000)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
@@ -274,17 +274,17 @@ def main():
         print(json.dumps({"copy_bytes_per_dispatch": x.numel() * 4}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported "
               "(multi-GPU sharding by views is a design only)", file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "cfg4":
-        run_cfg4(args)
+    if args.workload == "growth4m":
+        run_growth4m(args)
         return
     N, V, W, H = args.n, args.views, args.width, args.height
-    if args.workload == "cfg0":
+    if args.workload == "small256":
         N, V, W, H = 16384, 1, 256, 256
-    elif args.workload == "cfg1":
+    elif args.workload == "ring8_720p":
         N, V, W, H = 500_000, 8, 1280, 720
     if args.workload == "uneven64":
         V = 64
@@ -293,7 +293,7 @@ def main():
         P = sum(w * h for w, h in sizes)
         workload = (f"{N} splats in [-2,2]^3, {V} views of seeded sizes 320-1920 x 240-1080 "
                     f"({P} pixels) per step, batched adjoint training step")
-    elif args.workload == "cfg0":
+    elif args.workload == "small256":
         g = sv.scene_gaussians(0, N, 1.0)
         cams = [sv.scene_camera([0.0, 0.0, 3.0], [0.0, 0.0, 0.0], W, H, 60.0)]
         P = W * H

@@ -179,7 +179,7 @@ size_t orc_densify(size_t n, const float *rows, const float *m1, const float *m2
  * color / target: H x W x 3. */
 void orc_tile_l1(const orc_view *v, const float *color, const float *target, float *tile_l1);
 
-/* Densify rule (config 1): a Gaussian is a candidate when a tile of its
+/* Densify rule (the 300-step training run): a Gaussian is a candidate when a tile of its
  * rectangle in a view where it is listed has L1 error above that view's
  * threshold; candidates split when their largest projected radius over
  * those views exceeds split_px, else clone. proj[v] / tile_l1[v] per view. */

@@ -2280,7 +2280,7 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
                              hipMemcpyDeviceToDevice, c->st));
     return sync(c);
   };
-  const sv_config cfg0 = c->cfg;
+  const sv_config saved_cfg = c->cfg;
   const uint32_t t0 = c->step_t;
   const bool timing0 = c->timing;
   c->timing = false;
@@ -2289,7 +2289,7 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
     st = fail(c, SV_E_HIP, "hipEventCreate failed");
   if (!st) st = copy_all(true);
   for (size_t i = 0; i < cand.size() && !st; ++i) {
-    c->cfg = cfg0;
+    c->cfg = saved_cfg;
     c->cfg.batch = cand[i].batch;
     c->cfg.persistent = cand[i].persistent;
     c->cfg.xcd_remap = cand[i].xcd_remap;
@@ -2316,7 +2316,7 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
   dfree(sAcc);
   c->step_t = t0;
   c->timing = timing0;
-  c->cfg = cfg0;
+  c->cfg = saved_cfg;
   if (st) return st;
   size_t best = 0;
   for (size_t i = 1; i < cand.size(); ++i)

@@ -30,7 +30,7 @@ def orc():
     return _orc
 
 
-def cfg0_scene(sv, n=16384, w=256, h=256, seed=0, jitter_seed=1):
+def small_scene(sv, n=16384, w=256, h=256, seed=0, jitter_seed=1):
     """The seeded small scene: n Gaussians in [-1,1]^3, one camera at
     (0,0,3) looking at the origin with a 60 degree vertical FOV."""
     g = sv.scene_gaussians(seed, n, 1.0)

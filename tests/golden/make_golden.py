@@ -7,10 +7,10 @@ Inputs come from the seeded scene generator, restated here in numpy
 (`gen_gaussians`, `gen_jitter`, `gen_camera`) and checked bit for bit
 against the library's C++ generator by tests/test_golden.py. Outputs are
 the oracle's (oracle/liboracle.so): the full tile build, image and per-pixel
-state of the config-2 scene (4,096 Gaussians, 64x64), gradients of 64
+state of the adjoint-check scene (4,096 Gaussians, 64x64), gradients of 64
 sampled Gaussians in full, and SHA-256 digests of every array of both the
-config-2 and the config-0 scene (16,384 Gaussians, 256x256), so a change of
-one bit anywhere shows up.
+adjoint-check scene and the small scene (16,384 Gaussians, 256x256), so a
+change of one bit anywhere shows up.
 """
 from __future__ import annotations
 
@@ -121,10 +121,10 @@ def digest(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-CONFIGS = {
+SCENES = {
     # name: (n, w, h, eye)
-    "cfg0": (16384, 256, 256, (0.0, 0.0, 3.0)),
-    "cfg2": (4096, 64, 64, (0.0, 0.0, 3.0)),
+    "small256": (16384, 256, 256, (0.0, 0.0, 3.0)),
+    "adjoint64": (4096, 64, 64, (0.0, 0.0, 3.0)),
 }
 ARRAYS = ("keys", "vals", "offsets", "color", "alpha", "last_ids", "residual")
 GRADS = ("pos", "logscale", "quat", "logit", "sh", "geom")
@@ -133,7 +133,7 @@ GRADS = ("pos", "logscale", "quat", "logit", "sh", "geom")
 def run_oracle(name: str):
     import oracle as orc
 
-    n, w, h, eye = CONFIGS[name]
+    n, w, h, eye = SCENES[name]
     g = gen_gaussians(0, n, 1.0)
     tg = dict(g)
     tg["pos"] = gen_jitter(1, g["pos"], 0.02)
@@ -146,15 +146,15 @@ def run_oracle(name: str):
 
 def main():
     manifest = {}
-    for name in CONFIGS:
+    for name in SCENES:
         g, target, r = run_oracle(name)
         ent = {"n_isect": r["n_isect"], "inputs": {k: digest(v) for k, v in g.items()},
                "target": digest(target), "outputs": {k: digest(r[k]) for k in ARRAYS},
                "grads": {k: digest(r["grads"][k]) for k in GRADS}}
         manifest[name] = ent
-        if name == "cfg2":
+        if name == "adjoint64":
             pick = np.random.default_rng(3).choice(np.flatnonzero(r["proj"]["tiles_count"] > 0), 64, replace=False)
-            np.savez_compressed(os.path.join(HERE, "cfg2_4096_64x64.npz"),
+            np.savez_compressed(os.path.join(HERE, "adjoint64_4096_64x64.npz"),
                                 target=target, pick=pick.astype(np.int64),
                                 **{k: r[k] for k in ARRAYS},
                                 **{"grad_" + k: r["grads"][k][pick] for k in GRADS})

@@ -39,7 +39,7 @@ def test_numpy_camera_equals_library_camera(sv, eye, w, h):
 
 @pytest.mark.parametrize("name", sorted(MANIFEST))
 def test_generator_inputs_match_digests(name):
-    n = mg.CONFIGS[name][0]
+    n = mg.SCENES[name][0]
     g = mg.gen_gaussians(0, n, 1.0)
     for k, d in MANIFEST[name]["inputs"].items():
         assert mg.digest(g[k]) == d, k
@@ -55,7 +55,7 @@ def test_oracle_reproduces_golden(orc, name):
         assert mg.digest(r[k]) == ent["outputs"][k], k
     for k in mg.GRADS:
         assert mg.digest(r["grads"][k]) == ent["grads"][k], k
-    if name == "cfg2":
-        z = np.load(os.path.join(HERE, "golden", "cfg2_4096_64x64.npz"))
+    if name == "adjoint64":
+        z = np.load(os.path.join(HERE, "golden", "adjoint64_4096_64x64.npz"))
         for k in mg.ARRAYS:
             assert np.array_equal(z[k], r[k]), k

@@ -7,7 +7,7 @@ the rectangle bits exceed 32 bits."""
 import numpy as np
 import pytest
 
-from conftest import cfg0_scene
+from conftest import small_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +26,7 @@ def _cfg(ctx, **kw):
 
 
 def test_binned_lists_match_oracle(sv, orc):
-    g, _, cam = cfg0_scene(sv)
+    g, _, cam = small_scene(sv)
     view = orc.view_params(256, 256, sh_deg=3)
     ref = orc.render_and_grad(g, cam, view)
     with sv.Context(capacity=32768) as c:

@@ -1,4 +1,4 @@
-"""The device path against the committed golden fixtures of the config-2
+"""The device path against the committed golden fixtures of the adjoint-check
 scene (4,096 Gaussians, 64x64, one view; tests/golden): sort keys, values,
 tile offsets, alpha and last indices bit-exact, image within 1e-5, the
 gradients of the 64 recorded Gaussians within 1e-4 of max |g|."""
@@ -19,8 +19,8 @@ import make_golden as mg  # noqa: E402
 
 
 def test_config2_against_golden(sv):
-    z = np.load(os.path.join(HERE, "golden", "cfg2_4096_64x64.npz"))
-    n, w, h, eye = mg.CONFIGS["cfg2"]
+    z = np.load(os.path.join(HERE, "golden", "adjoint64_4096_64x64.npz"))
+    n, w, h, eye = mg.SCENES["adjoint64"]
     g = sv.scene_gaussians(0, n, 1.0)
     cam = sv.scene_camera(list(eye), [0, 0, 0], w, h, 60.0)
     c = sv.Context(capacity=n)
@@ -42,6 +42,6 @@ def test_config2_against_golden(sv):
     pick = z["pick"]
     for k in mg.GRADS:
         e = close_rel(gg[k][pick], z["grad_" + k], 1e-4)
-        print(f"[golden] cfg2 grad {k}: {e:.2e}", flush=True)
+        print(f"[golden] adjoint64 grad {k}: {e:.2e}", flush=True)
         assert e <= 1e-4, k
     c.close()

@@ -12,7 +12,7 @@ it in well under a second). Bars:
 import numpy as np
 import pytest
 
-from conftest import cfg0_scene, close_rel
+from conftest import small_scene, close_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -35,11 +35,11 @@ def _cfg(sv, ctx, **kw):
 
 @pytest.fixture(scope="module")
 def scene(sv, orc):
-    g, tg, cam = cfg0_scene(sv)
+    g, tg, cam = small_scene(sv)
     view = orc.view_params(256, 256, sh_deg=3)
     target = orc.render_and_grad(tg, cam, view)["color"]
     ref = orc.render_and_grad(g, cam, view, target=target, w_l2=0.8, w_l1=0.2)
-    print(f"[parity] cfg0 scene: N={g['pos'].shape[0]} I={ref['n_isect']}", flush=True)
+    print(f"[parity] small scene: N={g['pos'].shape[0]} I={ref['n_isect']}", flush=True)
     return g, cam, view, target, ref
 
 
@@ -122,7 +122,7 @@ def test_fixed_point_tile_grads(sv, orc, scene, ctx, flags):
 
 
 def test_one_training_step(sv, orc, scene):
-    """Config-0 step: forward, L2+L1 loss, backward, geometry backward and
+    """Small-scene step: forward, L2+L1 loss, backward, geometry backward and
     Adam (lr 1e-3, betas 0.9/0.999, eps 1e-15, t = 1)."""
     g, cam, view, target, ref = scene
     c = sv.Context(capacity=32768)

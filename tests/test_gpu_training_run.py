@@ -1,4 +1,4 @@
-"""Config 1 on the device: 500,000 Gaussians, 8 ring views at 1280x720,
+"""The 300-step training run on the device: 500,000 Gaussians, 8 ring views at 1280x720,
 300 plain-method Adam iterations cycling the views, prune at
 sigmoid(logit) < 0.005 and densify (split if the projected radius exceeds
 8 px and a tile's L1 error is above its view's 90th percentile, else clone)

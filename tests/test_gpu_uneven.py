@@ -1,4 +1,4 @@
-"""Views of uneven sizes in one batched pipeline (config 3 at reduced size:
+"""Views of uneven sizes in one batched pipeline (the 64-view workload at reduced size:
 seeded sizes, cameras on a sphere of radius 4, means in [-2,2]^3): every
 image matches the oracle within 1e-5 (exactly, in practice), and the
 batched step — captured as a graph — is bit-identical to per-view

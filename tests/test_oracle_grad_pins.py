@@ -266,7 +266,7 @@ def test_geometry_backward_matches_autograd_of_shader_model(sv, orc):
 
 # ------------------------------------------------ finite differences (FP64)
 def test_sh_and_logit_gradients_match_central_fd_of_fp64_forward(sv, orc):
-    """Config-2 scene: 4096 Gaussians, 64x64, one view, SH degree 3. For 64
+    """Adjoint-check scene: 4096 Gaussians, 64x64, one view, SH degree 3. For 64
     Gaussians picked with seed 3, the logit derivative (h = 1e-3) and the
     derivative along a random direction of the 48 SH coefficients are
     compared with the oracle's float32 gradient. A pair or sign switch
