@@ -30,7 +30,8 @@ __device__ __forceinline__ bool next_tile(const RasterArgs &a, uint32_t &tg, uin
   __syncthreads();
   if (threadIdx.x == 0) *s_tg = atomicAdd(a.work_counter, 1u);
   __syncthreads();
-  tg = *s_tg;
+  // uniform over the workgroup: keep it (and what derives from it) scalar
+  tg = __builtin_amdgcn_readfirstlane(*s_tg);
   return tg < a.n_tiles_total;
 }
 
@@ -49,7 +50,7 @@ __device__ __forceinline__ uint32_t first_tile(const RasterArgs &a, uint32_t *s_
   if (a.work_counter == nullptr) return a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
   if (threadIdx.x == 0) *s_tg = atomicAdd(a.work_counter, 1u);
   __syncthreads();
-  return *s_tg;
+  return __builtin_amdgcn_readfirstlane(*s_tg);
 }
 
 // DPP move of a float (0 in lanes disabled by the row mask).
@@ -447,7 +448,11 @@ __global__ __launch_bounds__(WT, 7) void k_forward(RasterArgs ain) {
             composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
           }
         }
-        if (__syncthreads_and((done[0] && done[1]) && (done[2] && done[3]))) break;
+        // the workgroup is one wave: a wave vote, then the barrier that
+        // protects the LDS batch
+        const bool all_done = __all((done[0] && done[1]) && (done[2] && done[3]));
+        __syncthreads();
+        if (all_done) break;
       }
 #pragma unroll
       for (int j = 0; j < PPL; ++j) {
