@@ -347,10 +347,9 @@ __device__ __forceinline__ void composite(float as, float r, float g, float b, u
   c1 = c1 + g * contrib;
   c2 = c2 + b * contrib;
   A = A + contrib;
-  if (1.0f - A < SV_T_EPS) {
-    done = true;
-    last = idx;
-  }
+  const bool fin = 1.0f - A < SV_T_EPS;  // selects, not a branch
+  done = done | fin;
+  last = fin ? idx : last;
 }
 
 // ---------------------------------------------------------------------
