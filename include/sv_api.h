@@ -349,8 +349,9 @@ sv_status sv_points_to_gaussians(size_t n, const float *xyz, const uint8_t *rgb,
  * of everything before it. The random draws of densify come from caller
  * seeds, so there is no generator state to save. */
 sv_status sv_save_checkpoint(sv_ctx *ctx, const char *path);
-/* Restores a checkpoint written by sv_save_checkpoint (N must fit the
- * context's capacity); views and targets are not part of it. */
+/* Restores a checkpoint written by sv_save_checkpoint; a file holding more
+ * Gaussians than the context's capacity grows it to the next power of two
+ * (as sv_densify does). Views and targets are not part of it. */
 sv_status sv_load_checkpoint(sv_ctx *ctx, const char *path);
 
 /* --- autotuner ------------------------------------------------------ */

@@ -196,9 +196,12 @@ def adam(param, m1, m2, grad, lr, beta1, beta2, eps, bc1, bc2s):
 
 
 def render_and_grad(g: dict, cam, view: OrcView, target=None, w_l2=0.8, w_l1=0.2, w_ssim=0.0,
-                    wavefront=False, wf_flags=0, cap_isect=None) -> dict:
+                    wavefront=False, wf_flags=0, cap_isect=None, grads=None) -> dict:
     """Project, sort, forward; with a target also loss, backward and the
-    geometry backward. Returns every intermediate as numpy arrays."""
+    geometry backward. Returns every intermediate as numpy arrays. With
+    `grads` (a zero_grads dict) the gradients are added into it in place,
+    after its per-view grad_geom planes are cleared, as a training step
+    accumulates views."""
     p = Params(g)
     n = p.n
     proj = project(g, cam, view)
@@ -217,7 +220,10 @@ def render_and_grad(g: dict, cam, view: OrcView, target=None, w_l2=0.8, w_l1=0.2
     so = OrcStepOut(out["proj"].ctypes.data_as(C.c_void_p), _u32p(out["keys"]), _u32p(out["vals"]), cap,
                     _u32p(out["offsets"]), _fp(out["color"]), _fp(out["alpha"]), _u32p(out["last_ids"]),
                     _fp(out["residual"]), _i32p(fixed) if fixed is not None else None)
-    grads = zero_grads(n)
+    if grads is None:
+        grads = zero_grads(n)
+    else:
+        grads["geom"][...] = 0.0
     gs = grads_struct(grads)
     oc = camera(cam)
     tgt = None if target is None else np.ascontiguousarray(target, np.float32)
@@ -381,3 +387,64 @@ def percentile_lower(x: np.ndarray, p: float) -> float:
     """The value at index floor(p (n - 1)) of the sorted values."""
     s = np.sort(np.asarray(x, np.float32))
     return float(s[int(np.floor(p * (s.size - 1)))])
+
+
+def position_lr(lr_pos: float, t: int) -> float:
+    """computePositionLearningRate (gaussian-renderer.ts:284-298) in float64,
+    stored as float32: log-linear decay to 1.6e-6 over 30000 steps with a
+    sine warm-up from 0.01 x over the first 3000."""
+    import math
+
+    if not lr_pos > 0.0:
+        return float(np.float32(lr_pos))
+    lr_final, delay_mult, max_steps = 0.0000016, 0.01, 30000.0
+    step = max(float(t), 0.0)
+    tt = min(1.0, step / max_steps)
+    loglerp = math.exp(math.log(float(np.float32(lr_pos))) * (1 - tt) + math.log(lr_final) * tt)
+    phase = min(1.0, step / max(1.0, round(0.1 * max_steps)))
+    delay = delay_mult + (1.0 - delay_mult) * math.sin(0.5 * math.pi * phase)
+    return float(np.float32(loglerp * delay))
+
+
+def adam_update(params: dict, m1: dict, m2: dict, grads: dict, t: int, lrs: dict, beta1: float, beta2: float,
+                eps: float, sh_update_interval: int = 1, pos_lr_schedule: bool = False):
+    """run_adam_updates (gaussian-renderer.ts:2455-2545) in place: adam_step
+    (adam.wgsl:18-38) per tensor with its learning rate, t = step index; the
+    position rate follows the schedule when asked and is skipped when not
+    positive; SH only when t % sh_update_interval == 0 (:2482-2484)."""
+    bc1, bc2 = adam_bias(t, beta1, beta2)
+    for k in PARAM_KEYS:
+        lr = lrs[k]
+        if k == "pos":
+            lr = position_lr(lr, t) if pos_lr_schedule else lr
+            if not np.float32(lr) > 0:
+                continue
+        if k == "sh" and sh_update_interval and t % sh_update_interval != 0:
+            continue
+        adam(params[k], m1[k], m2[k], grads[k], lr, beta1, beta2, eps, bc1, bc2)
+
+
+def train(g: dict, cams, views, targets, schedule, lrs: dict, beta1=0.9, beta2=0.999, eps=1e-15, w_l2=0.8,
+          w_l1=0.2, w_ssim=0.0, sh_update_interval=1, pos_lr_schedule=False, wavefront=False, wf_flags=0,
+          t0=0, state=None):
+    """trainStep (gaussian-renderer.ts:3638-3660) for every entry of
+    `schedule` (a list of view-index lists): zero the gradients, render,
+    loss and backward each view in order (gradients summed; grad_geom per
+    view), then one Adam step. Returns (params, m1, m2, per-step list of
+    (params copy, grads)). `state` = (params, m1, m2) continues a run."""
+    if state is None:
+        p = {k: np.array(g[k], np.float32, copy=True) for k in PARAM_KEYS}
+        m1 = {k: np.zeros_like(p[k]) for k in PARAM_KEYS}
+        m2 = {k: np.zeros_like(p[k]) for k in PARAM_KEYS}
+    else:
+        p, m1, m2 = ({k: np.array(d[k], np.float32, copy=True) for k in PARAM_KEYS} for d in state)
+    hist = []
+    n = p["pos"].shape[0]
+    for i, vs in enumerate(schedule):
+        gr = zero_grads(n)
+        for v in vs:
+            render_and_grad(p, cams[v], views[v], target=targets[v], w_l2=w_l2, w_l1=w_l1, w_ssim=w_ssim,
+                            wavefront=wavefront, wf_flags=wf_flags, grads=gr)
+        adam_update(p, m1, m2, gr, t0 + i + 1, lrs, beta1, beta2, eps, sh_update_interval, pos_lr_schedule)
+        hist.append(({k: p[k].copy() for k in PARAM_KEYS}, gr))
+    return p, m1, m2, hist

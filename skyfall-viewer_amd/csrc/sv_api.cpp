@@ -130,6 +130,7 @@ struct sv_ctx {
   size_t sort_tmp_cap = 0;
   float *isect_rec = nullptr;
   int32_t *fixed = nullptr;
+  uint32_t *fixed_inv = nullptr;  // emit position -> sorted position (plain mode)
   uint64_t fixed_cap = 0;
   uint64_t *h_small = nullptr;  // pinned
 
@@ -957,9 +958,11 @@ sv_status loss_views(sv_ctx *c) {
 sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
   if (c->I > c->fixed_cap || !c->fixed) {
     dfree(c->fixed);
+    dfree(c->fixed_inv);
     c->fixed_cap = 0;
     const uint64_t want = std::max<uint64_t>(c->I + c->I / 4, 1024);
     if (sv_status s = dalloc(c, &c->fixed, (size_t)want * 54)) return s;
+    if (sv_status s = dalloc(c, &c->fixed_inv, (size_t)want)) return s;
     c->fixed_cap = want;
   }
   RasterArgs a = raster_args(c);
@@ -967,8 +970,9 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
   launch_fill_f32(c->st, c->geom, 0.f, (size_t)5 * c->cap);
   {
     Scoped t(c, PH_FIXED);
-    launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed);
-    launch_reduce_fixed(c->st, c->fixed, c->vals, c->vals2, c->d_meta, c->I, c->cap, c->G, c->geom);
+    launch_backward_fixed(c->st, a, c->P, c->cap, flags, c->fixed, c->fixed_inv);
+    launch_reduce_fixed(c->st, c->fixed, c->fixed_inv, c->counts, c->offs, c->bbase, c->N, c->d_meta, c->cap, c->G,
+                        c->geom);
     launch_geom(c->st, c->P, c->cap, c->N, c->d_views, dev_config(c->cfg), c->geom, c->G,
                 stats ? c->grad_accum : nullptr, c->aux, c->counts, stats ? c->importance : nullptr);
   }
@@ -1029,7 +1033,21 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
   g.push_back(c->chunk_cap);
   g.push_back(c->binned ? 1u : 0u);
   g.push_back(c->tiles_total);
-  for (int v : c->slot_views) g.push_back((uint64_t)(uint32_t)v);
+  g.push_back(c->super_total);
+  g.push_back(c->super_bits);
+  g.push_back((uint64_t)c->max_super_passes);
+  g.push_back((uint64_t)c->max_passes);
+  g.push_back(c->key64 ? 1u : 0u);
+  g.push_back(c->max_key_bits);
+  g.push_back(c->tile_bits);
+  // per slot: the view, its size and pixel offset (the loss launches take
+  // them by value)
+  for (size_t s = 0; s < c->slot_views.size(); ++s) {
+    const int v = c->slot_views[s];
+    g.push_back((uint64_t)(uint32_t)v);
+    g.push_back(((uint64_t)c->views[v].W << 32) | c->views[v].H);
+    g.push_back(c->views[v].pix_off);
+  }
   uint64_t w[(sizeof(sv_config) + 7) / 8] = {0};
   std::memcpy(w, &c->cfg, sizeof(sv_config));
   for (uint64_t x : w) g.push_back(x);
@@ -1341,6 +1359,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->chunk_super);
   dfree(c->ccount);
   dfree(c->fixed);
+  dfree(c->fixed_inv);
   if (c->h_small) (void)hipHostFree(c->h_small);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -1375,6 +1394,7 @@ sv_status sv_set_gaussians(sv_ctx *c, size_t n, const float *pos, const float *l
   if (n > c->cap) return fail(c, SV_E_CAPACITY, "%zu Gaussians exceed capacity %zu", n, c->cap);
   if (n > 0xffffffffull) return fail(c, SV_E_CAPACITY, "too many Gaussians");
   HIP_OK(c, hipStreamSynchronize(c->st));
+  drop_graph(c);
   c->N = (uint32_t)n;
   if (sv_status s = set_params(c, c->P, pos, ls, q, lg, sh)) return s;
   clear_state(c);
@@ -1422,6 +1442,7 @@ sv_status sv_set_views(sv_ctx *c, int nv, const sv_camera *cams) {
   c->bin_unfit = false;
   if (nv <= 0 || !cams) return fail(c, SV_E_ARG, "need at least one camera");
   HIP_OK(c, hipStreamSynchronize(c->st));
+  drop_graph(c);
   std::vector<HostView> vs(nv);
   uint64_t off = 0;
   const uint32_t ts = c->cfg.tile_size;
@@ -2204,10 +2225,15 @@ sv_status sv_load_checkpoint(sv_ctx *c, const char *path) {
     std::fclose(f);
     return fail(c, SV_E_ARG, "%s is not a version-1 checkpoint of this library", path);
   }
-  if (hd.n > c->cap) {
+  // the file's size must match its count before anything is allocated
+  const uint64_t per = (uint64_t)(3 * P_NUM + 8) * sizeof(float);
+  long here = std::ftell(f);
+  long size = -1;
+  if (here >= 0 && std::fseek(f, 0, SEEK_END) == 0) size = std::ftell(f);
+  if (size < 0 || std::fseek(f, here, SEEK_SET) != 0 || hd.n > 0xffffffffull ||
+      (uint64_t)size != (uint64_t)here + sizeof(sv_config) + hd.n * per + sizeof(uint64_t)) {
     std::fclose(f);
-    return fail(c, SV_E_CAPACITY, "checkpoint holds %llu Gaussians, capacity is %zu", (unsigned long long)hd.n,
-                c->cap);
+    return fail(c, SV_E_ARG, "%s: size does not match its %llu Gaussians", path, (unsigned long long)hd.n);
   }
   sv_config cfg;
   get(&cfg, sizeof(cfg));
@@ -2226,6 +2252,25 @@ sv_status sv_load_checkpoint(sv_ctx *c, const char *path) {
     return s;
   }
   HIP_OK(c, hipStreamSynchronize(c->st));
+  drop_graph(c);
+  if (n > c->cap) {
+    // a checkpoint saved after densify grew the scene: grow to the next
+    // power of two, as densify does (ensureCapacity, gaussian-renderer.ts:2044-2064)
+    size_t cap_n = 1;
+    while (cap_n < n) cap_n <<= 1;
+    float *Pn = nullptr, *M1n = nullptr, *M2n = nullptr;
+    if (sv_status s = dalloc(c, &Pn, (size_t)P_NUM * cap_n)) return s;
+    if (sv_status s = dalloc(c, &M1n, (size_t)P_NUM * cap_n)) {
+      dfree(Pn);
+      return s;
+    }
+    if (sv_status s = dalloc(c, &M2n, (size_t)P_NUM * cap_n)) {
+      dfree(Pn);
+      dfree(M1n);
+      return s;
+    }
+    if (sv_status s = swap_in(c, Pn, M1n, M2n, cap_n, (uint32_t)n)) return s;
+  }
   clear_state(c);
   float *arrays[3] = {c->P, c->M1, c->M2};
   const float *src = data.data();

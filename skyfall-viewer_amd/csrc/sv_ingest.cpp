@@ -123,7 +123,9 @@ sv_status parse_ply(const std::vector<unsigned char> &buf, Ply &p) {
   }
   if (!binary_le) return ingest_fail(SV_E_ARG, "only binary_little_endian PLY is read");
   p.data = end + std::strlen("end_header") + 1;
-  if (p.data + p.n * p.stride > buf.size()) return ingest_fail(SV_E_ARG, "PLY vertex data is truncated");
+  // n * stride may not fit in size_t for a crafted count: compare by division
+  if (p.data > buf.size() || (p.stride != 0 && p.n > (buf.size() - p.data) / p.stride))
+    return ingest_fail(SV_E_ARG, "PLY vertex data is truncated");
   return SV_OK;
 }
 
@@ -148,10 +150,11 @@ struct Reader {
   const std::vector<unsigned char> &b;
   size_t off = 0;
   bool ok = true;
+  // off never passes b.size(), so b.size() - off does not wrap
   template <typename T>
   T get() {
     T v{};
-    if (off + sizeof(T) > b.size()) {
+    if (!ok || sizeof(T) > b.size() - off) {
       ok = false;
       return v;
     }
@@ -159,9 +162,14 @@ struct Reader {
     off += sizeof(T);
     return v;
   }
-  void skip(size_t n) {
-    if (off + n > b.size()) ok = false;
-    off += n;
+  // skip `count` records of `each` bytes (count comes from the file)
+  void skip(uint64_t count, size_t each) {
+    if (!ok || (each != 0 && count > (uint64_t)((b.size() - off) / each))) {
+      ok = false;
+      off = b.size();
+      return;
+    }
+    off += (size_t)count * each;
   }
 };
 
@@ -399,7 +407,7 @@ sv_status sv_colmap_load(const char *sparse_dir, size_t n_images, sv_camera *cam
       name.push_back(ch);
     }
     const uint64_t n2 = ri.get<uint64_t>();
-    ri.skip((size_t)n2 * 24);
+    ri.skip(n2, 24);
     const ColmapCam *cam = nullptr;
     for (auto &c : cmap)
       if (c.id == cam_id) cam = &c;
@@ -445,7 +453,7 @@ sv_status sv_colmap_load(const char *sparse_dir, size_t n_images, sv_camera *cam
     for (int k = 0; k < 3; ++k) rgb[3 * i + k] = rp.get<uint8_t>();
     rp.get<double>();
     const uint64_t track = rp.get<uint64_t>();
-    rp.skip((size_t)track * 8);
+    rp.skip(track, 8);
   }
   if (!rp.ok) return ingest_fail(SV_E_ARG, "points3D.bin is truncated");
   return SV_OK;

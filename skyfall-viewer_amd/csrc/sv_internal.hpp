@@ -164,9 +164,10 @@ struct RasterArgs {
 void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 // backward_tiles_local (backward-tiles-local.wgsl:185-382) for one slot:
-// i32 sums per (intersection, slot of 54).
+// i32 sums per (intersection, slot of 54) in sorted order; inv[emit
+// position] = sorted position of every intersection.
 void launch_backward_fixed(hipStream_t st, const RasterArgs &a, const float *P, size_t cap,
-                           uint32_t flags, int32_t *fixed);
+                           uint32_t flags, int32_t *fixed, uint32_t *inv);
 
 // sv_optim.hip ----------------------------------------------------------
 struct VjpArgs {
@@ -207,10 +208,13 @@ void launch_adam(hipStream_t st, float *P, float *M1, float *M2, const float *G,
 // Copy the step's Adam parameters to the device: from mapped host memory
 // (src, re-read on every graph replay) or, with src null, from `value`.
 void launch_load_adam(hipStream_t st, const AdamParams &value, const AdamParams *src, AdamParams *dst);
-// backward_reduce (backward-reduce.wgsl:35-71) + geometry backward input.
-void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals0,
-                         const uint32_t *vals1, const StepMeta *m, uint64_t I, size_t cap, float *G,
-                         float *geom);
+// backward_reduce (backward-reduce.wgsl:35-71) + geometry backward input
+// for one slot: one thread per Gaussian adds its intersections' values in
+// sorted order (the Gaussian's tiles in ascending order, found through inv
+// from its emit positions), so the sums do not depend on scheduling.
+void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *inv, const uint32_t *counts,
+                         const uint32_t *offs, const uint32_t *bbase, uint32_t N, const StepMeta *m, size_t cap,
+                         float *G, float *geom);
 // training_backward_geom (training-backward-geom.wgsl:36-299) from the
 // per-Gaussian grad_geom planes of one view.
 // Also accumulate_importance (importance-accum.wgsl:19-40) for that view

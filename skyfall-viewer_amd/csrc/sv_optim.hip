@@ -533,34 +533,62 @@ void launch_load_adam(hipStream_t st, const AdamParams &value, const AdamParams 
 // to grad_opacity, grad_geom and grad_sh (order not deterministic, as in
 // the reference).
 __global__ __launch_bounds__(256) void k_reduce_fixed(const int32_t *__restrict__ fixed,
-                                                      const uint32_t *__restrict__ vals0,
-                                                      const uint32_t *__restrict__ vals1,
+                                                      const uint32_t *__restrict__ inv,
+                                                      const uint32_t *__restrict__ counts,
+                                                      const uint32_t *__restrict__ offs,
+                                                      const uint32_t *__restrict__ bbase, uint32_t N,
                                                       const StepMeta *m, size_t cap,
                                                       float *__restrict__ G, float *__restrict__ geom) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m->overflow || i >= m->n_isect) return;
-  const uint32_t g = (m->sel ? vals1 : vals0)[i];
-  if ((size_t)g >= cap) return;
-  const int32_t *s = fixed + i * 54;
-  const float inv = 1.0f / 65536.0f;
-  float v = (float)s[0] * inv;
-  if (fabsf(v) > 1e-10f) atomicAdd(&G[(size_t)P_LOGIT * cap + g], v);
-  for (int k = 0; k < 5; ++k) {
-    v = (float)s[1 + k] * inv;
-    if (fabsf(v) > 1e-10f) atomicAdd(&geom[(size_t)k * cap + g], v);
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m->overflow || g >= N) return;
+  const uint32_t cnt = counts[g];
+  if (cnt == 0u) return;
+  const uint32_t e0 = bbase[g >> 8] + offs[g];
+  const uint64_t nI = m->n_isect;  // every sorted position is below it
+  const float inv65536 = 1.0f / 65536.0f;
+  // the Gaussian's values, added in sorted order as the shader's atomics
+  // would in a sequential schedule; skip |v| <= 1e-10 (backward-reduce.wgsl:44-68)
+  float acc[6];
+  acc[0] = G[(size_t)P_LOGIT * cap + g];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) acc[1 + k] = geom[(size_t)k * cap + g];
+  for (uint32_t e = e0; e < e0 + cnt; ++e) {
+    if (inv[e] >= nI) continue;
+    const int32_t *s = fixed + (size_t)inv[e] * 54;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float v = (float)s[k] * inv65536;
+      if (fabsf(v) > 1e-10f) acc[k] = acc[k] + v;
+    }
   }
-  for (int k = 0; k < 48; ++k) {
-    v = (float)s[6 + k] * inv;
-    if (fabsf(v) > 1e-10f) atomicAdd(&G[(size_t)(P_SH + k) * cap + g], v);
+  G[(size_t)P_LOGIT * cap + g] = acc[0];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) geom[(size_t)k * cap + g] = acc[1 + k];
+  // SH: 48 sums, 12 at a time to bound registers
+  for (int k0 = 0; k0 < 48; k0 += 12) {
+    float sh[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sh[k] = G[(size_t)(P_SH + k0 + k) * cap + g];
+    for (uint32_t e = e0; e < e0 + cnt; ++e) {
+      if (inv[e] >= nI) continue;
+      const int32_t *s = fixed + (size_t)inv[e] * 54 + 6 + k0;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        const float v = (float)s[k] * inv65536;
+        if (fabsf(v) > 1e-10f) sh[k] = sh[k] + v;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) G[(size_t)(P_SH + k0 + k) * cap + g] = sh[k];
   }
 }
 
-void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *vals0,
-                         const uint32_t *vals1, const StepMeta *m, uint64_t I, size_t cap, float *G,
-                         float *geom) {
-  if (I == 0) return;
-  hipLaunchKernelGGL(k_reduce_fixed, dim3((uint32_t)((I + 255) / 256)), dim3(256), 0, st, fixed, vals0,
-                     vals1, m, cap, G, geom);
+void launch_reduce_fixed(hipStream_t st, const int32_t *fixed, const uint32_t *inv, const uint32_t *counts,
+                         const uint32_t *offs, const uint32_t *bbase, uint32_t N, const StepMeta *m, size_t cap,
+                         float *G, float *geom) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_reduce_fixed, dim3((N + 255) / 256), dim3(256), 0, st, fixed, inv, counts, offs, bbase, N,
+                     m, cap, G, geom);
 }
 
 // training_backward_geom from the grad_geom planes (wavefront path) +

@@ -668,7 +668,8 @@ constexpr int WF_S = 54;
 
 __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs ain, const float *__restrict__ P,
                                                        size_t cap, uint32_t flags,
-                                                       int32_t *__restrict__ fixed) {
+                                                       int32_t *__restrict__ fixed,
+                                                       uint32_t *__restrict__ inv) {
   RasterArgs a = ain;
   if (!resolve_meta(a)) return;
   __shared__ int32_t s_acc[WF_B * WF_S];
@@ -720,6 +721,9 @@ __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs ain, const flo
       s_x0[threadIdx.x] = q[0];
       s_x1[threadIdx.x] = q[1];
       s_x2[threadIdx.x] = q[2].x;
+      // sorted position of this (Gaussian, tile) pair, filed under its emit
+      // position: the reduce walks each Gaussian's pairs in tile order
+      inv[emit_pos(q[3], a.bbase[(size_t)s * a.nb + (g >> 8)], tx, ty)] = bs + threadIdx.x;
       float wp[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -802,10 +806,10 @@ void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32
 }
 
 void launch_backward_fixed(hipStream_t st, const RasterArgs &a, const float *P, size_t cap,
-                           uint32_t flags, int32_t *fixed) {
+                           uint32_t flags, int32_t *fixed, uint32_t *inv) {
   if (a.n_tiles_total == 0) return;
   hipLaunchKernelGGL(k_backward_fixed, dim3(a.n_tiles_total), dim3(RT), 0, st, a, P, cap, flags,
-                     fixed);
+                     fixed, inv);
 }
 
 }  // namespace sv

@@ -1,0 +1,144 @@
+"""Training trajectories over several Adam steps against the oracle's own run
+(the small scene: 16,384 splats, two 256x256 views at (0,0,3) and (3,0,0)).
+
+Adam's update m_hat / (sqrt(v_hat) + eps) is a normalised step: wherever a
+gradient is within rounding noise of zero it moves the parameter by up to
+lr in either direction. On the oracle itself, random perturbations of the
+gradients of 1e-7 (1e-6) of max |g| per step, the size of float
+reordering, move 0.3 % (2.3 %) of the SH parameters by more than 1e-4 of
+max |p| within 10 steps, so a free-running comparison holds to 1e-4 only
+where the gradients agree bit for bit. Hence three checks:
+
+1. plain mode (the reference's fixed-point method): the device reduces each
+   Gaussian's integer sums in sorted order, the oracle's order, so the whole
+   run is bit-exact: every parameter after every step equals the oracle's
+   (10 steps alternating the views; and 16 steps with the position-lr
+   schedule, gaussian-renderer.ts:284-298, and SH updated every 16th step,
+   :2482-2484, both views per step);
+2. batched adjoint mode, teacher-forced: at every step the device's
+   gradient at its current parameters is within 1e-4 of max |ref| of the
+   oracle's gradient at those parameters, and the new parameters and
+   moments equal the oracle's Adam (adam.wgsl:18-38) applied to the device's
+   gradient bit for bit (10 steps, both views per step, with and without
+   the schedules);
+3. batched adjoint mode, free-running with the fused Adam of the product
+   path (graph launch): after 10 steps the parameters follow the oracle's
+   own 10-step run, for each tensor 97 % of the elements within 1e-4 of
+   max |ref| and none further apart than two runs can drift under Adam's
+   step bound (Cauchy-Schwarz on m and v: |m_hat| / sqrt(v_hat) <=
+   (1 - b1) / sqrt(1 - b2) * sqrt(sum_k<t (b1^2 / b2)^k) * sqrt(1 - b2^t) /
+   (1 - b1^t) at step t).
+"""
+import numpy as np
+import pytest
+
+from conftest import close_rel
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("pos", "logscale", "quat", "logit", "sh")
+LR = 1e-3
+LRS = {k: LR for k in KEYS}
+
+
+@pytest.fixture(scope="module")
+def scene(sv, orc):
+    g = sv.scene_gaussians(0, 16384, 1.0)
+    tg = dict(g)
+    tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
+    cams = [sv.scene_camera([0.0, 0.0, 3.0], [0, 0, 0], 256, 256, 60.0),
+            sv.scene_camera([3.0, 0.0, 0.0], [0, 0, 0], 256, 256, 60.0)]
+    views = [orc.view_params(256, 256, sh_deg=3) for _ in cams]
+    targets = [orc.render_and_grad(tg, c, v)["color"] for c, v in zip(cams, views)]
+    return g, cams, views, targets
+
+
+def _ctx(sv, g, cams, targets, sched, **kw):
+    c = sv.Context(capacity=32768)
+    c.set_config(sh_deg=3, w_l2=0.8, w_l1=0.2, w_ssim=0.0, lr_pos=LR, lr_sh=LR, lr_opacity=LR, lr_scale=LR,
+                 lr_rot=LR, beta1=0.9, beta2=0.999, eps=1e-15, sh_update_interval=sched[0],
+                 pos_lr_schedule=sched[1], plain_wf_flags=0, **kw)
+    c.set_gaussians(g)
+    c.set_views(cams)
+    for v, t in enumerate(targets):
+        c.set_target(v, t)
+    return c
+
+
+SCHEDULES = {"const": (1, 0), "sched": (16, 1)}
+
+
+@pytest.mark.parametrize("case", [("alternate", "const", 10), ("both", "sched", 16)])
+def test_plain_trajectory_bit_exact(sv, orc, scene, case):
+    order, sch, steps = case
+    g, cams, views, targets = scene
+    schedule = [[s % 2] if order == "alternate" else [0, 1] for s in range(steps)]
+    si, pl = SCHEDULES[sch]
+    _, _, _, hist = orc.train(g, cams, views, targets, schedule, LRS, sh_update_interval=si,
+                              pos_lr_schedule=bool(pl), wavefront=True, wf_flags=0)
+    c = _ctx(sv, g, cams, targets, SCHEDULES[sch])
+    try:
+        for s, vs in enumerate(schedule):
+            c.step_views(vs, sv.MODE_PLAIN)
+            got = c.get_gaussians()
+            ref = hist[s][0]
+            diffs = {k: int(np.count_nonzero(got[k] != ref[k])) for k in KEYS}
+            errs = {k: close_rel(got[k], ref[k], 0) for k in KEYS}
+            print(f"[traj] plain {order}/{sch} step {s + 1}: differing elements {diffs} max rel {errs}", flush=True)
+            for k in KEYS:
+                assert np.array_equal(got[k], ref[k]), (s + 1, k)
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("sch", ["const", "sched"])
+def test_batched_teacher_forced(sv, orc, scene, sch):
+    g, cams, views, targets = scene
+    si, pl = SCHEDULES[sch]
+    c = _ctx(sv, g, cams, targets, SCHEDULES[sch], keep_grads=1, use_graph=1)
+    worst = {k: 0.0 for k in KEYS}
+    try:
+        for s in range(10):
+            p0, m10, m20 = c.get_gaussians(), c.get_moments(1), c.get_moments(2)
+            c.step_views([0, 1], sv.MODE_BATCHED)
+            p1, m11, m21, gr = c.get_gaussians(), c.get_moments(1), c.get_moments(2), c.grads()
+            ref = orc.zero_grads(p0["pos"].shape[0])
+            for v in (0, 1):
+                orc.render_and_grad(p0, cams[v], views[v], target=targets[v], w_l2=0.8, w_l1=0.2, grads=ref)
+            for k in KEYS:
+                e = close_rel(gr[k], ref[k], 1e-4)
+                worst[k] = max(worst[k], e)
+                assert e <= 1e-4, (s + 1, k, e)
+            # the device's Adam on its own gradient = the oracle's Adam, bit for bit
+            q, a1, a2 = ({k: d[k].copy() for k in KEYS} for d in (p0, m10, m20))
+            orc.adam_update(q, a1, a2, gr, s + 1, LRS, 0.9, 0.999, 1e-15, si, bool(pl))
+            for k in KEYS:
+                assert np.array_equal(p1[k], q[k]), (s + 1, k)
+                assert np.array_equal(m11[k], a1[k]) and np.array_equal(m21[k], a2[k]), (s + 1, k)
+        print(f"[traj] batched teacher-forced {sch}: worst gradient error per tensor {worst}", flush=True)
+    finally:
+        c.close()
+
+
+def test_batched_free_running(sv, orc, scene):
+    g, cams, views, targets = scene
+    steps = 10
+    p_ref, _, _, _ = orc.train(g, cams, views, targets, [[0, 1]] * steps, LRS)
+    c = _ctx(sv, g, cams, targets, SCHEDULES["const"], keep_grads=0, use_graph=1)
+    try:
+        for _ in range(steps):
+            c.step_views([0, 1], sv.MODE_BATCHED)
+        got = c.get_gaussians()
+    finally:
+        c.close()
+    b1, b2 = 0.9, 0.999
+    bound = 2.0 * sum(LR * (1 - b1) / np.sqrt(1 - b2) * np.sqrt(sum((b1 * b1 / b2) ** k for k in range(t)))
+                      * np.sqrt(1 - b2 ** t) / (1 - b1 ** t) for t in range(1, steps + 1))
+    for k in KEYS:
+        d = np.abs(got[k].astype(np.float64) - p_ref[k])
+        scale = np.abs(p_ref[k]).max()
+        frac = float(np.mean(d > 1e-4 * scale))
+        print(f"[traj] batched free-running {k}: beyond 1e-4 of max {100 * frac:.3f} %, "
+              f"median {np.median(d) / scale:.2e}, max {d.max():.2e}", flush=True)
+        assert frac <= 0.03, k
+        assert d.max() <= bound, k
