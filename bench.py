@@ -57,13 +57,18 @@ def configure(ctx: sv.Context, args, keep_grads=0):
 
 
 def kernel_bytes(phase: str, N: int, V: int, I: int, P: int, key_bytes: int, passes: int) -> float:
-    """Algorithmic HBM bytes of one launch of a phase (per-unit figures x units)."""
-    if phase == "forward":  # per intersection: id 4 + record 36; per pixel: rgb 12 + A 4 + last 4
+    """Algorithmic HBM bytes of one launch of a phase: the per-unit figures of
+    the step's byte model (per intersection: a 40 B gather of the id and the
+    36 B record in each rasterizer, a 36 B 2-D gradient contribution; per
+    pixel: 12 B image + 8 B (alpha, last index) written by the forward, 12 B
+    target + 8 B (alpha, last index) read by the backward) x the units the
+    launch processes."""
+    if phase == "forward":
         return I * 40.0 + P * 20.0
-    if phase == "backward":  # per isect: id 4 + record 36 + rect/offset 12 + adjoint record 36; per pixel 32
-        return I * 88.0 + P * 32.0
-    if phase == "vjp_adam":  # per Gaussian: 59 x (p, m, v) read + write + stats; per (G, view) 24; per isect 36
-        return N * 1480.0 + N * V * 24.0 + I * 36.0
+    if phase == "backward":
+        return I * 76.0 + P * 20.0
+    if phase == "vjp_adam":  # per Gaussian: 59 x (p, m, v) read + write; per (G, view) the 36 B 2-D grad record
+        return N * 1416.0 + N * V * 36.0
     if phase == "sort":
         return passes * I * (key_bytes + 2 * (key_bytes + 4.0))
     return 0.0
@@ -75,6 +80,23 @@ def step_b_alg(N: int, V: int, I: int, P: int, d: int) -> float:
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1M_16x1080p.json")
+SQ_FILE = os.path.join(ROOT, "profiles", "sq_counters_1M_16x1080p.json")
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its 16-lane
+# SIMD for 4 cycles (unpacked FP32: 78.6 TFLOP/s of the 157.3 packed peak);
+# peak engine clock 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4.0
+
+
+def measured_sq(phase: str, N: int, V: int, W: int, H: int, workload: str):
+    """Shader-sequencer counters of `phase` from the committed rocprofv3
+    summary, when it was measured on this workload; else None."""
+    if workload != "ring16" or not os.path.exists(SQ_FILE):
+        return None
+    with open(SQ_FILE) as fh:
+        t = json.load(fh)
+    if t["workload"] != {"splats": N, "views": V, "width": W, "height": H} or phase not in t["phases"]:
+        return None
+    return t["phases"][phase]
 
 
 def measured_traffic(phase: str, N: int, V: int, W: int, H: int, workload: str):
@@ -141,12 +163,26 @@ def run_mode(mode, g, cams, targets, args, graph=0):
     return (t1 - t0) / args.steps, timings, phase_steps, st, eager, tune, chosen
 
 
+def cpu_threads():
+    """Threads of the CPU baseline: OMP_NUM_THREADS when set (the host's
+    share of cores for this GPU), else every core this process may run on."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return env or avail, avail
+
+
 def cpu_baseline(g, cams, targets, args):
+    """The oracle (C, OpenMP; parallel LSD radix sort) on one of the step's
+    views, scaled to a step: steps/s, and keys/s of sorting that view's
+    emitted (tile, depth) pairs."""
+    threads, avail = cpu_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     orc.lib()
     w, h = int(cams[0].viewport[0]), int(cams[0].viewport[1])
     view = orc.view_params(w, h, sh_deg=3)
@@ -159,15 +195,23 @@ def cpu_baseline(g, cams, targets, args):
         p = np.array(g[k], np.float32, copy=True)
         orc.adam(p, np.zeros_like(p), np.zeros_like(p), r["grads"][k], 1e-3, 0.9, 0.999, 1e-15, bc1, bc2)
     t_adam = time.perf_counter() - t0
+    # keys/s: the emitted pairs of this view, unsorted, through the radix sort
+    keys, vals = orc.emit(r["proj"], view)
+    t0 = time.perf_counter()
+    orc.sort_pairs_radix(keys, vals)
+    t_sort = time.perf_counter() - t0
     V = len(cams)
     return {
         "value": 1.0 / (V * t_view + t_adam),
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle (C, OpenMP) render+loss+backward+geometry backward of 1 of the {V} views "
-                  f"({g['pos'].shape[0]} splats, {w}x{h}, {r['n_isect']} intersections, {t_view:.2f} s) "
-                  f"+ Adam over all parameters ({t_adam:.2f} s), scaled to {V} views",
+        "keys_per_s": keys.size / t_sort if t_sort > 0 else None,
+        "host_cores_available": avail,
+        "sample": f"CPU oracle (C, OpenMP, {threads} threads) render+loss+backward+geometry backward of 1 of the "
+                  f"{V} views ({g['pos'].shape[0]} splats, {w}x{h}, {r['n_isect']} intersections, {t_view:.2f} s) "
+                  f"+ Adam over all parameters ({t_adam:.2f} s), scaled to {V} views; keys/s = that view's "
+                  f"{keys.size} (tile, depth) pairs through the parallel LSD radix sort ({t_sort:.3f} s)",
     }
 
 
@@ -175,9 +219,12 @@ def run_growth4m(args):
     """The growth run: 1M splats, 16 ring views at 1920x1080, batched steps;
     every 100 steps prune sigmoid(logit) < 0.005 and densify (split above 8
     px where a tile's L1 error is above its view's 90th percentile, else
-    clone), the added splats capped so the count stays <= 4M (actions past
-    the budget are dropped in index order). The timed region holds every
-    step and every prune / densify event."""
+    clone). With --growth-cap 1 (default) an event adds at most
+    max(1, floor(0.1 N)) splats, the best-scored candidates first (the
+    reference's cap, gaussian-renderer.ts:3297-3306), and never more than
+    the 4M budget; with 0 every candidate is taken up to the budget (in
+    index order). The timed region holds every step and every prune /
+    densify event."""
     N0, V, W, H, NMAX, EVERY = 1_000_000, 16, 1920, 1080, 4_000_000, 100
     g = sv.scene_gaussians(0, N0, 1.0)
     cams = sv.ring_cameras(V, 3.0, W, H)
@@ -206,16 +253,19 @@ def run_growth4m(args):
         st = c.step_views(views, sv.MODE_BATCHED, stats=True)
         isects += int(st.n_isect)
         if it % EVERY == 0:
+            l1 = float(st.l1_sum) / (3.0 * V * W * H)
             removed = c.prune_opacity(0.005)
-            act, nc, ns = c.densify_decide(views, 0.9, 8.0)
             budget = NMAX - c.count
+            cap = min(max(1, c.count // 10), budget) if args.growth_cap else 0
+            act, nc, ns = c.densify_decide(views, 0.9, 8.0, max_new=max(cap, 0) if args.growth_cap else 0)
+            if args.growth_cap and cap <= 0:
+                act[:] = 0
             if nc + ns > budget:
                 idx = np.flatnonzero(act)
                 act[idx[max(budget, 0):]] = 0
             added = c.densify(act, seed=it)
-            events.append({"step": it, "pruned": removed, "added": added, "count": c.count})
-            log(f"growth4m step {it}: pruned {removed}, added {added}, {c.count} splats, "
-                f"mean L1 {float(st.l1_sum) / (3.0 * V * W * H):.5f}")
+            events.append({"step": it, "mean_l1": l1, "pruned": removed, "added": added, "count": c.count})
+            log(f"growth4m step {it}: mean L1 {l1:.5f}, pruned {removed}, added {added}, {c.count} splats")
     c.sync()
     dt = time.perf_counter() - t0
     P = V * W * H
@@ -226,12 +276,81 @@ def run_growth4m(args):
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
         "config": {"workload": f"{N0} splats growing to <= {NMAX}, {V} views {W}x{H}, prune/densify every "
                                f"{EVERY} steps", "splats_start": N0, "splats_end": c.count, "views": V,
-                   "width": W, "height": H},
+                   "width": W, "height": H, "growth_cap": "max(1, floor(0.1 N)) per event, best score first"
+                   if args.growth_cap else "none (4M budget only)"},
         "intersections_per_s": isects / dt, "final_mean_l1": float(st.l1_sum) / (3.0 * P) if st else None,
         "events": events,
     }
     c.close()
     print(json.dumps(out), flush=True)
+
+
+def make_workload(name: str, args):
+    """(splats, cameras, sizes, width, height, description) of a workload."""
+    N, V, W, H = args.n, args.views, args.width, args.height
+    if name == "small256":
+        N, V, W, H = 16384, 1, 256, 256
+    elif name == "ring8_720p":
+        N, V, W, H = 500_000, 8, 1280, 720
+    if name == "uneven64":
+        V = 64
+        g = sv.scene_gaussians(0, N, 2.0)
+        cams, sizes = sv.sphere_cameras(2, V, 4.0)
+        desc = (f"{N} splats in [-2,2]^3, {V} views of seeded sizes 320-1920 x 240-1080 "
+                f"({sum(w * h for w, h in sizes)} pixels) per step, batched adjoint training step")
+        return g, cams, sizes, None, None, desc
+    g = sv.scene_gaussians(0, N, 1.0)
+    if name == "small256":
+        cams = [sv.scene_camera([0.0, 0.0, 3.0], [0.0, 0.0, 0.0], W, H, 60.0)]
+        desc = f"{N} splats, 1 view {W}x{H} from (0,0,3), training step"
+    else:
+        cams = sv.ring_cameras(V, 3.0, W, H)
+        desc = f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step"
+    return g, cams, [(W, H)] * V, W, H, desc
+
+
+def render_targets(g, cams, args):
+    tg = dict(g)
+    tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
+    targets = []
+    with sv.Context(capacity=g["pos"].shape[0]) as tc:
+        configure(tc, args)
+        tc.set_gaussians(tg)
+        tc.set_views(cams)
+        for v in range(len(cams)):
+            tc.forward(v)
+            targets.append(tc.image(v))
+    return targets
+
+
+def phase_table(tim, steps):
+    """{phase: (ms per launch, launches per step)}."""
+    return {k: (v[0] / max(v[1], 1), v[1] / steps) for k, v in tim.items() if v[1]}
+
+
+def list_build(tim, steps, I):
+    """(ms per step, keys/s) of building the tile lists from the projections
+    (emit + sort + ranges [+ fill])."""
+    ms = sum(tim.get(k, (0.0, 0))[0] for k in ("emit", "sort", "ranges", "fill")) / steps
+    return ms, (I / (ms * 1e-3) if ms > 0 else None)
+
+
+def secondary(name, args):
+    """Batched and plain steps/s, keys/s and the CPU oracle of a smaller
+    workload (reported beside the headline)."""
+    g, cams, sizes, W, H, desc = make_workload(name, args)
+    targets = render_targets(g, cams, args)
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.autotune = max(3, min(args.steps, 10)), 2, 0
+    sec_b, tim_b, ps_b, st_b, *_ = run_mode(sv.MODE_BATCHED, g, cams, targets, sub, graph=args.graph)
+    sec_p, *_ = run_mode(sv.MODE_PLAIN, g, cams, targets, sub)
+    ms_list, kps = list_build(tim_b, ps_b, int(st_b.n_isect))
+    out = {"workload": desc, "batched_steps_per_s": 1.0 / sec_b, "plain_steps_per_s": 1.0 / sec_p,
+           "keys_per_s": kps, "intersections_per_step": int(st_b.n_isect)}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, cams, targets, args)
+    log(f"{name}: batched {1e3 * sec_b:.3f} ms/step, plain {1e3 * sec_p:.3f} ms/step")
+    return out
 
 
 def main():
@@ -259,6 +378,10 @@ def main():
                     help="1: time the tuner's candidates first and run the batched step with the fastest")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--growth-cap", type=int, default=1,
+                    help="growth4m: 1 = at most max(1, N // 10) new splats per event, best score first")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the small256 and ring8_720p measurements reported beside the headline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (torch) three times, to scale the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -281,39 +404,11 @@ def main():
     if args.workload == "growth4m":
         run_growth4m(args)
         return
-    N, V, W, H = args.n, args.views, args.width, args.height
-    if args.workload == "small256":
-        N, V, W, H = 16384, 1, 256, 256
-    elif args.workload == "ring8_720p":
-        N, V, W, H = 500_000, 8, 1280, 720
-    if args.workload == "uneven64":
-        V = 64
-        g = sv.scene_gaussians(0, N, 2.0)
-        cams, sizes = sv.sphere_cameras(2, V, 4.0)
-        P = sum(w * h for w, h in sizes)
-        workload = (f"{N} splats in [-2,2]^3, {V} views of seeded sizes 320-1920 x 240-1080 "
-                    f"({P} pixels) per step, batched adjoint training step")
-    elif args.workload == "small256":
-        g = sv.scene_gaussians(0, N, 1.0)
-        cams = [sv.scene_camera([0.0, 0.0, 3.0], [0.0, 0.0, 0.0], W, H, 60.0)]
-        P = W * H
-        workload = f"{N} splats, 1 view {W}x{H} from (0,0,3), training step"
-    else:
-        g = sv.scene_gaussians(0, N, 1.0)
-        cams = sv.ring_cameras(V, 3.0, W, H)
-        P = V * W * H
-        workload = f"{N} splats, {V} views {W}x{H} per step, batched adjoint training step"
+    g, cams, sizes, W, H, workload = make_workload(args.workload, args)
+    N, V = g["pos"].shape[0], len(cams)
+    P = sum(w * h for w, h in sizes)
     log(f"scene: {workload}")
-    tg = dict(g)
-    tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
-    targets = []
-    with sv.Context(capacity=N) as tc:
-        configure(tc, args)
-        tc.set_gaussians(tg)
-        tc.set_views(cams)
-        for v in range(V):
-            tc.forward(v)
-            targets.append(tc.image(v))
+    targets = render_targets(g, cams, args)
     log("targets rendered")
 
     sec_b, tim_b, ps_b, st_b, eager_b, tune_b, chosen_b = run_mode(sv.MODE_BATCHED, g, cams, targets, args,
@@ -329,20 +424,29 @@ def main():
         log(f"sequential adjoint: {1e3 * sec_q:.2f} ms/step")
         sec_p, tim_p, ps_p, *_ = run_mode(sv.MODE_PLAIN, g, cams, targets, args)
         plain = {"ms_per_step": 1e3 * sec_p, "steps_per_s": 1.0 / sec_p,
-                 "method": "reference method per view: wavefront fixed-point backward, atomic reduce, "
+                 "method": "reference method per view: wavefront fixed-point backward, per-Gaussian reduce, "
                            "geometry backward, Adam",
                  "phase_ms_per_step": {k: v[0] / ps_p for k, v in tim_p.items() if v[1]}}
         log(f"plain: {1e3 * sec_p:.2f} ms/step")
+    # the same batched step with every (tile, splat) pair sorted: sorted keys/s of the pair sort
+    pair_sort = None
+    if args.binned:
+        ps_args = argparse.Namespace(**vars(args))
+        ps_args.binned, ps_args.autotune, ps_args.steps, ps_args.warmup = 0, 0, 3, 1
+        _, tim_s, ps_s, st_s, *_ = run_mode(sv.MODE_BATCHED, g, cams, targets, ps_args, graph=args.graph)
+        ms_s, kps_s = list_build(tim_s, ps_s, int(st_s.n_isect))
+        sort_ms = tim_s.get("sort", (0.0, 0))[0] / ps_s
+        pair_sort = {"keys_per_s": kps_s, "tile_list_ms_per_step": ms_s, "sort_ms_per_step": sort_ms,
+                     "sort_only_keys_per_s": int(st_s.n_isect) / (sort_ms * 1e-3) if sort_ms > 0 else None,
+                     "key_bits": int(st_s.key_bits), "sort_passes": int(st_s.sort_passes)}
+        log(f"pair sort: {ms_s:.3f} ms/step list build, {sort_ms:.3f} ms sort, {st_s.key_bits}-bit keys")
 
     I = int(st_b.n_isect)
     key_bytes = 8 if st_b.key_bits > 32 else 4
     passes = int(st_b.sort_passes)
     pair_passes = (int(st_b.key_bits) + 7) // 8  # passes of a sort of every (tile, depth) key
-    phase = {k: (v[0] / max(v[1], 1), v[1] / ps_b) for k, v in tim_b.items() if v[1]}
-    # sorted keys/s: intersections per second of building the depth-ordered
-    # tile lists from the projections (emit + sort + ranges [+ fill])
-    list_ms_step = sum(tim_b.get(k, (0.0, 0))[0] for k in ("emit", "sort", "ranges", "fill")) / ps_b
-    keys_per_s = I / (list_ms_step * 1e-3) if list_ms_step > 0 else None
+    phase = phase_table(tim_b, ps_b)
+    list_ms_step, keys_per_s = list_build(tim_b, ps_b, I)
     # dominant single-kernel phase
     cands = {k: phase[k][0] for k in ("forward", "backward", "vjp_adam") if k in phase}
     dom = max(cands, key=cands.get)
@@ -351,6 +455,23 @@ def main():
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     b_alg = step_b_alg(N, V, I, P, pair_passes)
     traffic = measured_traffic(dom, N, V, W, H, args.workload)
+    sq = measured_sq(dom, N, V, W, H, args.workload)
+    valu = None
+    if sq:
+        valu = {"insts_per_launch": sq["waves"] * sq["valu_per_wave"],
+                "issue_frac": sq["waves"] * sq["valu_per_wave"] / (dom_ms * 1e-3 * VALU_ISSUE_PER_S),
+                "active_frac": sq["valu_active_frac"], "waiting_frac": sq["waiting_frac"],
+                "model": "VALU wave-instructions / (launch time x 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles)"}
+    limiter = None
+    if traffic or sq:
+        parts = []
+        if traffic:
+            parts.append(f"measured HBM {traffic / (dom_ms * 1e-3) / 1e9:.0f} GB/s = "
+                         f"{traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS:.2f} of peak")
+        if valu:
+            parts.append(f"VALU issue {valu['issue_frac']:.2f} of peak, waiting {valu['waiting_frac']:.2f} "
+                         f"of wave-cycles")
+        limiter = "latency (" + "; ".join(parts) + ")"
     out = {
         "metric": METRIC,
         "value": 1.0 / sec_b,
@@ -364,9 +485,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
-        "config": {"workload": workload, "splats": N, "views": V,
-                   "width": W if args.workload != "uneven64" else None,
-                   "height": H if args.workload != "uneven64" else None, "pixels": P,
+        "config": {"workload": workload, "splats": N, "views": V, "width": W, "height": H, "pixels": P,
                    "tile": 16, "lds_batch": chosen_b["batch"],
                    "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"],
                    "binned_build": args.binned},
@@ -378,6 +497,7 @@ def main():
         "tile_lists": "binned (supertile pairs sorted, lists filled)" if st_b.binned else "pair sort",
         "sort_key_bits": int(st_b.key_bits),
         "sort_passes": passes,
+        "pair_sort": pair_sort,
         "plain": plain,
         "speedup_vs_plain": (plain["ms_per_step"] / (1e3 * sec_b)) if plain else None,
         "sequential_adjoint": sequential,
@@ -390,12 +510,15 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "traffic_source": "profiles/hbm_traffic_1M_16x1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated)" if traffic else None,
-                     "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms},
+                     "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms,
+                     "limiter": limiter, "valu": valu},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, one view)...")
         out["cpu_baseline"] = cpu_baseline(g, cams, targets, args)
+    if args.workload == "ring16" and not args.no_configs:
+        out["other_workloads"] = {name: secondary(name, args) for name in ("small256", "ring8_720p")}
     print(json.dumps(out), flush=True)
 
 

@@ -294,10 +294,16 @@ sv_status sv_densify(sv_ctx *ctx, const uint8_t *actions, uint64_t seed,
  * tile's L1 error against its target is scored, each view's threshold is
  * its tile errors' `percentile` (value at floor(p (T - 1)) of the sorted
  * errors); a Gaussian listed on a tile above its view's threshold is
- * split (largest projected radius > split_px) or cloned. actions: N bytes
- * (0 none, 1 clone, 2 split), ready for sv_densify. */
+ * split (largest projected radius > split_px) or cloned. max_new > 0
+ * applies the reference's growth cap (:3297-3306, which uses
+ * max(1, floor(0.1 N))): only the max_new candidates of highest score keep
+ * their action, ties in index order; a candidate's score is its largest
+ * tile error over the views where it is listed divided by that view's
+ * threshold (the error itself for a threshold <= 1e-8), as the reference
+ * normalises its scores (:3280-3286). max_new = 0 keeps every candidate.
+ * actions: N bytes (0 none, 1 clone, 2 split), ready for sv_densify. */
 sv_status sv_densify_decide(sv_ctx *ctx, const int *views, int nviews, float percentile,
-                            float split_px, uint8_t *actions, size_t *n_clone,
+                            float split_px, size_t max_new, uint8_t *actions, size_t *n_clone,
                             size_t *n_split);
 /* Per-tile L1 error of view `view` from the last sv_densify_decide
  * (tiles_x * tiles_y floats, row-major tiles). */

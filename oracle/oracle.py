@@ -165,6 +165,14 @@ def sort_pairs(keys: np.ndarray, vals: np.ndarray):
     return k, v
 
 
+def sort_pairs_radix(keys: np.ndarray, vals: np.ndarray):
+    """The parallel LSD radix sort (same output as sort_pairs)."""
+    k = np.array(keys, np.uint32, copy=True)
+    v = np.array(vals, np.uint32, copy=True)
+    lib().orc_sort_pairs_radix(C.c_size_t(k.size), _u32p(k), _u32p(v))
+    return k, v
+
+
 def tile_offsets(keys: np.ndarray, n_tiles: int) -> np.ndarray:
     k = np.ascontiguousarray(keys, np.uint32)
     o = np.zeros(n_tiles + 1, np.uint32)
@@ -369,7 +377,9 @@ def tile_l1(view: OrcView, color: np.ndarray, target: np.ndarray) -> np.ndarray:
     return out
 
 
-def densify_decide(projs, views, tile_l1s, thr, split_px: float) -> np.ndarray:
+def densify_decide(projs, views, tile_l1s, thr, split_px: float, with_score: bool = False):
+    """Actions per Gaussian (0 none, 1 clone, 2 split); with_score also
+    returns the candidates' normalised scores."""
     nv = len(projs)
     n = projs[0].shape[0]
     parr = (C.c_void_p * nv)(*[p.ctypes.data_as(C.c_void_p) for p in projs])
@@ -378,8 +388,23 @@ def densify_decide(projs, views, tile_l1s, thr, split_px: float) -> np.ndarray:
     tarr = (C.POINTER(C.c_float) * nv)(*[_fp(t) for t in tl])
     th = np.ascontiguousarray(thr, np.float32)
     act = np.zeros(n, np.uint8)
+    score = np.zeros(n, np.float32)
     lib().orc_densify_decide(C.c_size_t(n), C.c_size_t(nv), parr, varr, tarr, _fp(th), C.c_float(split_px),
-                             act.ctypes.data_as(C.POINTER(C.c_uint8)))
+                             act.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(score))
+    return (act, score) if with_score else act
+
+
+def densify_cap(actions: np.ndarray, score: np.ndarray, max_new: int) -> np.ndarray:
+    """performDensification's growth cap (gaussian-renderer.ts:3297-3306):
+    the candidates sorted by score, highest first, with a stable sort of the
+    index-ordered list (ties in index order); only the first max_new keep
+    their action."""
+    act = np.array(actions, np.uint8, copy=True)
+    cand = np.flatnonzero(act)
+    if max_new <= 0 or cand.size <= max_new:
+        return act
+    order = cand[np.argsort(-np.asarray(score, np.float64)[cand], kind="stable")]
+    act[order[max_new:]] = 0
     return act
 
 

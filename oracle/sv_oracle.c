@@ -19,6 +19,8 @@
  *                       (Condon-Shortley convention) at random directions
  *   orc_emit/_encode    integer brute force in Python
  *   orc_sort_pairs      numpy stable argsort
+ *   orc_sort_pairs_radix  = orc_sort_pairs and numpy stable argsort (ties,
+ *                       unordered values, every digit width)
  *   orc_tile_offsets    numpy searchsorted (lower_bound)
  *   orc_forward         pure-Python compositing loop on a tiny scene;
  *                       empty tile -> background; single opaque splat
@@ -503,6 +505,70 @@ void orc_sort_pairs(size_t m, uint32_t *keys, uint32_t *vals) {
     free(pos_of);
   }
   free(kv);
+}
+
+/* Parallel stable LSD radix sort of the same pairs (8-bit digits over the
+ * key's significant bytes): each thread counts its contiguous slice, the
+ * per-(digit, thread) counts are scanned digit-major, and each thread
+ * scatters its slice in order, so equal keys keep their input order. The
+ * same output as orc_sort_pairs (a stable sort by key); it is the CPU
+ * baseline's sort. */
+void orc_sort_pairs_radix(size_t m, uint32_t *keys, uint32_t *vals) {
+  if (m < 2) return;
+  uint32_t kmax = 0;
+  for (size_t i = 0; i < m; ++i) kmax |= keys[i];
+  int passes = 0;
+  while (passes < 4 && (kmax >> (8 * passes))) ++passes;
+  uint32_t *k2 = (uint32_t *)malloc(m * sizeof(uint32_t));
+  uint32_t *v2 = (uint32_t *)malloc(m * sizeof(uint32_t));
+  int nt = 1;
+#ifdef _OPENMP
+  nt = omp_get_max_threads();
+#endif
+  size_t *cnt = (size_t *)calloc((size_t)nt * 256, sizeof(size_t));
+  uint32_t *ks = keys, *vs = vals, *kd = k2, *vd = v2;
+  for (int p = 0; p < passes; ++p) {
+    const int sh = 8 * p;
+    memset(cnt, 0, (size_t)nt * 256 * sizeof(size_t));
+#pragma omp parallel num_threads(nt)
+    {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      const size_t lo = m * (size_t)t / (size_t)nt, hi = m * (size_t)(t + 1) / (size_t)nt;
+      size_t *c = cnt + (size_t)t * 256;
+      for (size_t i = lo; i < hi; ++i) c[(ks[i] >> sh) & 255u]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        size_t run = 0;
+        for (int d = 0; d < 256; ++d)
+          for (int u = 0; u < nt; ++u) {
+            const size_t x = cnt[(size_t)u * 256 + d];
+            cnt[(size_t)u * 256 + d] = run;
+            run += x;
+          }
+      }
+      for (size_t i = lo; i < hi; ++i) {
+        const size_t o = c[(ks[i] >> sh) & 255u]++;
+        kd[o] = ks[i];
+        vd[o] = vs[i];
+      }
+    }
+    uint32_t *tk = ks, *tv = vs;
+    ks = kd;
+    vs = vd;
+    kd = tk;
+    vd = tv;
+  }
+  if (ks != keys) {
+    memcpy(keys, ks, m * sizeof(uint32_t));
+    memcpy(vals, vs, m * sizeof(uint32_t));
+  }
+  free(cnt);
+  free(k2);
+  free(v2);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1085,7 +1151,7 @@ int64_t orc_render_and_grad(size_t n, const orc_params *p, const orc_camera *cam
   const uint64_t m = orc_count_isects(n, o->proj);
   if (m > o->cap_isect) return -1;
   orc_emit(n, o->proj, v, o->keys, o->vals);
-  orc_sort_pairs((size_t)m, o->keys, o->vals);
+  orc_sort_pairs_radix((size_t)m, o->keys, o->vals);
   const uint32_t n_tiles = v->tiles_x * v->tiles_y;
   orc_tile_offsets((size_t)m, o->keys, n_tiles, o->offsets);
   orc_forward(p, o->proj, cam, v, o->offsets, o->vals, (size_t)m, o->color, o->alpha,
@@ -1421,24 +1487,26 @@ void orc_tile_l1(const orc_view *v, const float *color, const float *target, flo
 
 void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
                         const orc_view *views, const float *const *tile_l1,
-                        const float *thr, float split_px, uint8_t *actions) {
+                        const float *thr, float split_px, uint8_t *actions, float *score) {
   for (size_t i = 0; i < n; ++i) {
     int cand = 0;
-    float rmax = 0.0f;
+    float rmax = 0.0f, best = 0.0f;
     for (size_t s = 0; s < nviews; ++s) {
       const orc_proj *gp = &proj[s][i];
       if (gp->tiles_count == 0u) continue;
       rmax = fmaxf(rmax, gp->radius);
       uint32_t r[4];
       tile_rect(&views[s], gp->mean2d[0], gp->mean2d[1], gp->radius, r);
-      for (uint32_t ty = r[1]; ty < r[3] && !cand; ++ty)
-        for (uint32_t tx = r[0]; tx < r[2]; ++tx)
-          if (tile_l1[s][ty * views[s].tiles_x + tx] > thr[s]) {
-            cand = 1;
-            break;
-          }
+      for (uint32_t ty = r[1]; ty < r[3]; ++ty)
+        for (uint32_t tx = r[0]; tx < r[2]; ++tx) {
+          const float e = tile_l1[s][ty * views[s].tiles_x + tx];
+          if (e > thr[s]) cand = 1;
+          /* normalised score (gaussian-renderer.ts:3280-3286) */
+          best = fmaxf(best, thr[s] > 1e-8f ? e / thr[s] : e);
+        }
     }
     actions[i] = !cand ? 0u : (rmax > split_px ? 2u : 1u);
+    if (score) score[i] = cand ? best : 0.0f;
   }
 }
 

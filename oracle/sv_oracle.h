@@ -91,6 +91,8 @@ uint64_t orc_count_isects(size_t n, const orc_proj *proj);
 void orc_emit(size_t n, const orc_proj *proj, const orc_view *v,
               uint32_t *keys, uint32_t *vals);
 void orc_sort_pairs(size_t m, uint32_t *keys, uint32_t *vals);
+/* The same stable sort as a parallel LSD radix sort (OpenMP threads). */
+void orc_sort_pairs_radix(size_t m, uint32_t *keys, uint32_t *vals);
 void orc_tile_offsets(size_t m, const uint32_t *keys, uint32_t n_tiles,
                       uint32_t *offsets);
 void orc_forward(const orc_params *p, const orc_proj *proj,
@@ -182,10 +184,13 @@ void orc_tile_l1(const orc_view *v, const float *color, const float *target, flo
 /* Densify rule (the 300-step training run): a Gaussian is a candidate when a tile of its
  * rectangle in a view where it is listed has L1 error above that view's
  * threshold; candidates split when their largest projected radius over
- * those views exceeds split_px, else clone. proj[v] / tile_l1[v] per view. */
+ * those views exceeds split_px, else clone. proj[v] / tile_l1[v] per view.
+ * score (may be NULL): per candidate the largest tile error over its tiles
+ * divided by the view's threshold (the error where the threshold <= 1e-8),
+ * 0 for the others (the normalised score of gaussian-renderer.ts:3280-3286). */
 void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
                         const orc_view *views, const float *const *tile_l1,
-                        const float *thr, float split_px, uint8_t *actions);
+                        const float *thr, float split_px, uint8_t *actions, float *score);
 
 /* knn_init (knn-init.wgsl:40-90; constants gaussian-renderer.ts:77-79):
  * for each of the n points (positions xyz interleaved, rounded to f16 as

@@ -2070,7 +2070,7 @@ sv_status sv_multiview_scores(sv_ctx *c, const int *vs, int nv, uint32_t *import
 }
 
 sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, float split_px,
-                            uint8_t *actions, size_t *n_clone, size_t *n_split) {
+                            size_t max_new, uint8_t *actions, size_t *n_clone, size_t *n_split) {
   if (sv_status s = check_ctx(c)) return s;
   if (!vs || nv <= 0 || !actions) return fail(c, SV_E_ARG, "views and actions are required");
   if (!(percentile >= 0.0f && percentile <= 1.0f)) return fail(c, SV_E_ARG, "percentile must be in [0, 1]");
@@ -2096,24 +2096,41 @@ sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, 
     std::sort(v.begin(), v.end());
     thr[s] = v.empty() ? 0.0f : v[(size_t)std::floor((double)percentile * (double)(v.size() - 1))];
   }
-  float *d_thr = nullptr;
+  float *d_thr = nullptr, *d_score = nullptr;
   uint8_t *d_act = nullptr;
-  if (sv_status s = dalloc(c, &d_thr, (size_t)nv)) return s;
-  if (sv_status s = dalloc(c, &d_act, (size_t)c->N)) {
+  auto release = [&]() {
     dfree(d_thr);
-    return s;
-  }
-  sv_status st = SV_OK;
-  if (hipMemcpy(d_thr, thr.data(), nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    dfree(d_act);
+    dfree(d_score);
+  };
+  sv_status st = dalloc(c, &d_thr, (size_t)nv);
+  if (!st) st = dalloc(c, &d_act, (size_t)c->N);
+  if (!st && max_new) st = dalloc(c, &d_score, (size_t)c->N);
+  if (!st && hipMemcpy(d_thr, thr.data(), nv * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
     st = fail(c, SV_E_HIP, "threshold upload failed");
   if (!st) {
-    launch_densify_decide(c->st, c->rec, c->aux, c->counts, c->N, c->d_views, nv, c->tile_l1, d_thr, split_px, d_act);
+    launch_densify_decide(c->st, c->rec, c->aux, c->counts, c->N, c->d_views, nv, c->tile_l1, d_thr, split_px, d_act,
+                          d_score);
     st = check_launch(c, "densify_decide");
   }
   if (!st && hipMemcpy(actions, d_act, c->N, hipMemcpyDeviceToHost) != hipSuccess)
     st = fail(c, SV_E_HIP, "actions readback failed");
-  dfree(d_thr);
-  dfree(d_act);
+  std::vector<uint32_t> cand;
+  for (uint32_t i = 0; !st && i < c->N; ++i)
+    if (actions[i]) cand.push_back(i);
+  if (!st && max_new && cand.size() > max_new) {
+    // performDensification's growth cap (gaussian-renderer.ts:3297-3306):
+    // candidates by score, highest first, ties in index order (a stable
+    // sort of the index-ordered list); the first max_new keep their action
+    std::vector<float> score(c->N);
+    if (hipMemcpy(score.data(), d_score, c->N * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      st = fail(c, SV_E_HIP, "score readback failed");
+    if (!st) {
+      std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return score[a] > score[b]; });
+      for (size_t k = max_new; k < cand.size(); ++k) actions[cand[k]] = 0;
+    }
+  }
+  release();
   if (st) return st;
   size_t nc = 0, ns = 0;
   for (uint32_t i = 0; i < c->N; ++i) {

@@ -232,10 +232,11 @@ void launch_clamp_logit(hipStream_t st, float *P, size_t cap, uint32_t N, float 
 // Per-tile L1 error of every tile of the batch (scoring for densify).
 void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_view, uint32_t n_tiles,
                     const float *color, const float *target, float *tile_l1);
-// Densify rule from the batch's records, tile L1 and per-view thresholds.
+// Densify rule from the batch's records, tile L1 and per-view thresholds;
+// score (may be null) = the candidates' normalised tile error.
 void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, const uint32_t *counts, uint32_t N,
                            const DevView *views, int nv, const float *tile_l1, const float *thr,
-                           float split_px, uint8_t *actions);
+                           float split_px, uint8_t *actions, float *score);
 void launch_densify(hipStream_t st, const float *P, const float *M1, const float *M2, size_t cap,
                     uint32_t N, const uint8_t *actions, const uint32_t *keep_offs,
                     const uint32_t *child_offs, uint32_t n_keep, uint64_t seed, float *Pn,

@@ -770,44 +770,49 @@ void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_v
 // where it is listed has L1 error above that view's threshold; a candidate
 // splits when its largest projected radius over those views exceeds
 // split_px, else it is cloned. Replaces the decision block of
-// performDensification (gaussian-renderer.ts:3249-3326).
+// performDensification (gaussian-renderer.ts:3249-3326). score (may be
+// null): the largest tile error over the Gaussian's tiles in those views
+// divided by the view's threshold (the error itself where the threshold is
+// <= 1e-8), the normalised score the reference ranks candidates by
+// (:3280-3286); > 1 exactly for candidates with a positive threshold.
 __global__ __launch_bounds__(256) void k_densify_decide(const float4 *__restrict__ rec,
                                                         const float2 *__restrict__ aux,
                                                         const uint32_t *__restrict__ counts, uint32_t N,
                                                         const DevView *__restrict__ views, int nv,
                                                         const float *__restrict__ tile_l1,
                                                         const float *__restrict__ thr, float split_px,
-                                                        uint8_t *__restrict__ actions) {
+                                                        uint8_t *__restrict__ actions, float *__restrict__ score) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= N) return;
   bool cand = false;
-  float rmax = 0.0f;
+  float rmax = 0.0f, best = 0.0f;
   for (int s = 0; s < nv; ++s) {
     const size_t vg = (size_t)s * N + g;
     if (counts[vg] == 0u) continue;
     const float4 q3 = rec[vg * 4 + 3];
     rmax = fmaxf(rmax, aux[vg].y);
-    if (cand) continue;
+    if (cand && !score) continue;
     const DevView &V = views[s];
     const uint32_t a = __float_as_uint(q3.y), b = __float_as_uint(q3.z);
     const uint32_t x0 = a & 0xffffu, y0 = a >> 16, x1 = b & 0xffffu, y1 = b >> 16;
     const float th = thr[s];
-    for (uint32_t ty = y0; ty < y1 && !cand; ++ty)
-      for (uint32_t tx = x0; tx < x1; ++tx)
-        if (tile_l1[V.tile_base + ty * V.tiles_x + tx] > th) {
-          cand = true;
-          break;
-        }
+    for (uint32_t ty = y0; ty < y1; ++ty)
+      for (uint32_t tx = x0; tx < x1; ++tx) {
+        const float e = tile_l1[V.tile_base + ty * V.tiles_x + tx];
+        if (e > th) cand = true;
+        best = fmaxf(best, th > 1e-8f ? e / th : e);
+      }
   }
   actions[g] = !cand ? 0u : (rmax > split_px ? 2u : 1u);
+  if (score) score[g] = cand ? best : 0.0f;
 }
 
 void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, const uint32_t *counts, uint32_t N,
                            const DevView *views, int nv, const float *tile_l1, const float *thr,
-                           float split_px, uint8_t *actions) {
+                           float split_px, uint8_t *actions, float *score) {
   if (N == 0) return;
   hipLaunchKernelGGL(k_densify_decide, dim3((N + 255) / 256), dim3(256), 0, st, rec, aux, counts, N, views, nv,
-                     tile_l1, thr, split_px, actions);
+                     tile_l1, thr, split_px, actions, score);
 }
 
 // Counter-based SplitMix64 stream: value `ctr` of stream `seed`.

@@ -68,7 +68,7 @@ export interface MI355XGaussianRenderer {
   initializeKnnAndSync(sceneExtent?: number): Promise<Float32Array>;
   renderCamera(camera: CameraData): { image: Float32Array; alpha: Float32Array; width: number; height: number };
   getMultiViewScores(views?: number[]): Promise<{ importance: Uint32Array; pruning: Float32Array; sampleCount: number }>;
-  densify(opts?: { views?: number[]; percentile?: number; splitPx?: number; seed?: number }): Promise<{ cloned: number; split: number }>;
+  densify(opts?: { views?: number[]; percentile?: number; splitPx?: number; seed?: number; maxNew?: number }): Promise<{ cloned: number; split: number }>;
   enableProfiling(enabled: boolean): void;
   printProfilingReport(): void;
   profiledTrainStep(): Promise<StepStats>;

@@ -175,8 +175,11 @@ function createGaussianRenderer(pointCloud, options = {}) {
     async getMultiViewScores(views = cameras.map((_, i) => i).slice(0, 10)) {
       return native.multiViewScores(ctx.handle, views);
     },
-    async densify({ views, percentile = 0.9, splitPx = 8, seed = iteration } = {}) {
-      const d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx);
+    // performDensification (gaussian-renderer.ts:3211-3466): at most
+    // max(1, floor(0.1 N)) new splats per event, highest score first (:3297-3306)
+    async densify({ views, percentile = 0.9, splitPx = 8, seed = iteration,
+                    maxNew = Math.max(1, Math.floor(0.1 * native.getCount(ctx.handle))) } = {}) {
+      const d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx, maxNew);
       native.densify(ctx.handle, d.actions, seed);
       return { cloned: d.clones, split: d.splits };
     },

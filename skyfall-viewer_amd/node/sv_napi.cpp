@@ -798,17 +798,20 @@ napi_value Densify(napi_env env, napi_callback_info info) {
 
 napi_value DensifyDecide(napi_env env, napi_callback_info info) {
   Args a;
-  if (!get_args(env, info, a, 4)) return nullptr;
+  if (!get_args(env, info, a, 5)) return nullptr;
   sv_ctx *c = ctx_of(env, a.v[0]);
   if (!c) return nullptr;
   std::vector<int> vs;
-  double pct, split;
-  if (!views_from_js(env, a.v[1], vs) || !get_f64(env, a.v[2], &pct) || !get_f64(env, a.v[3], &split)) return nullptr;
+  double pct, split, max_new;
+  if (!views_from_js(env, a.v[1], vs) || !get_f64(env, a.v[2], &pct) || !get_f64(env, a.v[3], &split) ||
+      !get_f64(env, a.v[4], &max_new))
+    return nullptr;
   uint8_t *act = nullptr;
   napi_value arr = new_typed<uint8_t>(env, napi_uint8_array, sv_get_count(c), &act);
   if (!arr) return nullptr;
   size_t nc = 0, ns = 0;
-  SV(c, sv_densify_decide(c, vs.data(), (int)vs.size(), (float)pct, (float)split, act, &nc, &ns),
+  SV(c, sv_densify_decide(c, vs.data(), (int)vs.size(), (float)pct, (float)split,
+                          max_new > 0 ? (size_t)max_new : 0, act, &nc, &ns),
      "sv_densify_decide");
   napi_value o, v;
   CK(napi_create_object(env, &o));

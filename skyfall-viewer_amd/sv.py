@@ -106,8 +106,8 @@ _SIGS = {
     "sv_reset_opacity": (C.c_int, [C.c_void_p, C.c_float, C.POINTER(C.c_size_t)]),
     "sv_knn_init": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
     "sv_multiview_scores": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, _U32, _P, C.POINTER(C.c_int)]),
-    "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, _U8,
-                                    C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, C.c_size_t,
+                                    _U8, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(TuneEntry),
                               C.POINTER(C.c_int)]),
@@ -556,13 +556,15 @@ class Context:
                                            C.byref(ns)), "multiview_scores")
         return imp, pru, ns.value
 
-    def densify_decide(self, views, percentile: float = 0.9, split_px: float = 8.0):
-        """Returns (actions uint8[N], n_clone, n_split)."""
+    def densify_decide(self, views, percentile: float = 0.9, split_px: float = 8.0, max_new: int = 0):
+        """Returns (actions uint8[N], n_clone, n_split); max_new > 0 keeps only
+        the max_new candidates of highest score (the reference's cap is
+        max(1, N // 10))."""
         vs = (C.c_int * len(views))(*views)
         act = np.zeros(self.count, np.uint8)
         nc, ns = C.c_size_t(), C.c_size_t()
-        self._ck(lib().sv_densify_decide(self.h, vs, len(views), percentile, split_px, act.ctypes.data_as(_U8),
-                                         C.byref(nc), C.byref(ns)), "densify_decide")
+        self._ck(lib().sv_densify_decide(self.h, vs, len(views), percentile, split_px, max_new,
+                                         act.ctypes.data_as(_U8), C.byref(nc), C.byref(ns)), "densify_decide")
         return act, nc.value, ns.value
 
     def tile_l1(self, v: int) -> np.ndarray:

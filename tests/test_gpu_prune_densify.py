@@ -108,6 +108,8 @@ def test_densify_decide_matches_oracle(sv, orc):
     targets = [orc.render_and_grad(tg, cam, v)["color"] for cam, v in zip(cams, views)]
     c = _ctx(sv, g, n, cams, targets)
     act, nc, ns = c.densify_decide([0, 1, 2], 0.9, 8.0)
+    max_new = max(1, n // 10)
+    act_cap, nc_cap, ns_cap = c.densify_decide([0, 1, 2], 0.9, 8.0, max_new=max_new)
     projs, l1s = [], []
     for k, (cam, v) in enumerate(zip(cams, views)):
         r = orc.render_and_grad(g, cam, v)
@@ -116,8 +118,13 @@ def test_densify_decide_matches_oracle(sv, orc):
         projs.append(r["proj"])
         l1s.append(l1)
     thr = np.array([orc.percentile_lower(x, 0.9) for x in l1s], np.float32)
-    want = orc.densify_decide(projs, views, l1s, thr, 8.0)
-    print(f"[decide] clone {nc} split {ns} of {n}", flush=True)
+    want, score = orc.densify_decide(projs, views, l1s, thr, 8.0, with_score=True)
+    print(f"[decide] clone {nc} split {ns} of {n}; capped at {max_new}: clone {nc_cap} split {ns_cap}", flush=True)
     assert np.array_equal(act, want)
     assert nc == int((want == 1).sum()) and ns == int((want == 2).sum()) and nc > 0 and ns > 0
+    # the reference's growth cap: the max_new best-scored candidates, bit for bit
+    assert nc + ns > max_new
+    want_cap = orc.densify_cap(want, score, max_new)
+    assert np.array_equal(act_cap, want_cap)
+    assert nc_cap + ns_cap == max_new
     c.close()

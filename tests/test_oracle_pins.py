@@ -152,6 +152,22 @@ def test_sort_is_stable_sort_by_key(orc, monotone_vals):
     assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
 
 
+@pytest.mark.parametrize("bits", [7, 16, 28, 32])
+@pytest.mark.parametrize("monotone_vals", [True, False])
+def test_radix_sort_is_stable_sort_by_key(orc, bits, monotone_vals):
+    rng = np.random.default_rng(bits)
+    n = 200003
+    keys = rng.integers(0, 2 ** bits, n, dtype=np.uint64).astype(np.uint32)
+    keys[::7] = keys[3]  # many ties
+    vals = np.arange(n, dtype=np.uint32) if monotone_vals else rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    k, v = orc.sort_pairs_radix(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    k2, v2 = orc.sort_pairs(keys, vals)
+    assert np.array_equal(k, k2) and np.array_equal(v, v2)
+    assert orc.sort_pairs_radix(keys[:1], vals[:1])[0].tolist() == keys[:1].tolist()
+
+
 def test_tile_offsets_are_lower_bounds(orc):
     rng = np.random.default_rng(3)
     nt = 700

@@ -122,9 +122,9 @@ def test_densify_decide_brute_force(sv, orc):
     rng = np.random.default_rng(4)
     l1s = [rng.uniform(0, 10, v.tiles_x * v.tiles_y).astype(np.float32) for v in views]
     thr = np.array([orc.percentile_lower(x, 0.9) for x in l1s], np.float32)
-    act = orc.densify_decide(projs, views, l1s, thr, 8.0)
+    act, score = orc.densify_decide(projs, views, l1s, thr, 8.0, with_score=True)
     for i in range(len(g["pos"])):
-        cand, rmax = False, 0.0
+        cand, rmax, best = False, 0.0, 0.0
         for pr, v, l1, th in zip(projs, views, l1s, thr):
             if pr["tiles_count"][i] == 0:
                 continue
@@ -136,6 +136,28 @@ def test_densify_decide_brute_force(sv, orc):
             y1 = min(max(int(np.ceil(m[1] / np.float32(16) + r / np.float32(16))), 0), v.tiles_y)
             tiles = [ty * v.tiles_x + tx for ty in range(y0, y1) for tx in range(x0, x1)]
             cand = cand or any(l1[t] > th for t in tiles)
+            best = max([best] + [float(np.float32(l1[t]) / np.float32(th)) for t in tiles])
         want = 0 if not cand else (2 if rmax > 8.0 else 1)
         assert act[i] == want, i
+        # the normalised score: the worst tile over the threshold, > 1 for candidates
+        assert score[i] == (np.float32(best) if cand else 0.0), i
+        assert (score[i] > 1.0) == cand, i
     assert (act == 1).any() and (act == 2).any() and (act == 0).any()
+
+
+def test_densify_cap_order():
+    """The growth cap keeps the max_new highest scores, ties in index order
+    (gaussian-renderer.ts:3297-3306: Array.sort is stable), by brute force."""
+    import oracle as orc
+
+    rng = np.random.default_rng(9)
+    n = 4000
+    act = rng.choice([0, 1, 2], n, p=[0.5, 0.3, 0.2]).astype(np.uint8)
+    score = np.where(act > 0, rng.integers(1, 40, n) / 8.0, 0.0).astype(np.float32)  # many ties
+    cand = [i for i in range(n) if act[i]]
+    for max_new in (0, 1, 7, 400, len(cand) - 1, len(cand), len(cand) + 5):
+        got = orc.densify_cap(act, score, max_new)
+        keep = set(sorted(cand, key=lambda i: (-float(score[i]), i))[:max_new]) if 0 < max_new < len(cand) else set(cand)
+        for i in range(n):
+            assert got[i] == (act[i] if i in keep else 0), (max_new, i)
+    assert int(np.count_nonzero(orc.densify_cap(act, score, max(1, n // 10)))) == max(1, n // 10)
