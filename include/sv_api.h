@@ -221,6 +221,21 @@ sv_status sv_render_batched(sv_ctx *ctx, const int *views, int nv);
  * floats; alpha: H x W accumulated alpha, or NULL. The training views, their
  * images and targets are untouched. */
 sv_status sv_render_camera(sv_ctx *ctx, const sv_camera *cam, float *rgb, float *alpha);
+/* No counterpart in the reference. sv_render_camera for callers that cannot
+ * allocate device memory (serving loops that render a frame per request,
+ * harnesses that forbid hipMalloc): the same image and alpha, but every
+ * intermediate (projection records, pairs, sort buffers, tile ranges, pixel
+ * planes) lives in one fixed 64 MiB scratch array that each device holds
+ * from library load until the process ends; the call allocates nothing.
+ * Limits: at most 65,536 Gaussians, 1,048,576 pixels, 8,192 tiles and
+ * 1,048,576 (tile, splat) pairs; larger inputs return SV_E_CAPACITY (the
+ * pair count is known only after projection: then the call fails after
+ * running, with no output written). Every call on a device shares that
+ * device's scratch, so calls that overlap on one device (contexts on
+ * several host threads) race; calls in sequence are independent. Uses the
+ * context's Gaussians, configuration and stream; rgb / alpha as in
+ * sv_render_camera (host memory). */
+sv_status sv_render_camera_noalloc(sv_ctx *ctx, const sv_camera *cam, float *rgb, float *alpha);
 
 /* --- readbacks --------------------------------------------------------- */
 sv_status sv_read_image(sv_ctx *ctx, int view, float *rgb);

@@ -24,7 +24,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsv.so")
 ARCH = os.environ.get("SV_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["sv_project.hip", "sv_sort.hip", "sv_raster.hip", "sv_optim.hip", "sv_knn.hip", "sv_bin.hip", "sv_score.hip"]
+HIP_SOURCES = ["sv_project.hip", "sv_sort.hip", "sv_raster.hip", "sv_optim.hip", "sv_knn.hip", "sv_bin.hip", "sv_score.hip", "sv_noalloc.hip"]
 HOST_SOURCES = ["sv_api.cpp", "sv_scene.cpp", "sv_ingest.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]

@@ -166,6 +166,9 @@ struct sv_ctx {
   uint32_t *vw_last = nullptr;
   size_t vw_cap = 0;
 
+  // device address of the fixed scratch of sv_render_camera_noalloc
+  unsigned char *noalloc = nullptr;
+
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -1263,6 +1266,17 @@ sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
     return SV_E_HIP;
   }
   c->cap = capacity;
+  {
+    void *p = nullptr;
+    const hipError_t e = noalloc_scratch(device, &p);
+    if (e != hipSuccess) {
+      fail(nullptr, SV_E_HIP, "scratch address on device %d: %s", device, hipGetErrorString(e));
+      (void)hipStreamDestroy(c->st);
+      delete c;
+      return SV_E_HIP;
+    }
+    c->noalloc = (unsigned char *)p;
+  }
   sv_status s = alloc_gaussian_arrays(c, capacity);
   if (!s) s = dalloc(c, &c->d_small, 8);
   if (!s) s = dalloc(c, &c->d_meta, 4);
@@ -1702,6 +1716,114 @@ sv_status sv_render_camera(sv_ctx *c, const sv_camera *cam, float *rgb, float *a
   c->built = false;
   c->forward_done = false;
   return s;
+}
+
+sv_status sv_render_camera_noalloc(sv_ctx *c, const sv_camera *cam, float *rgb, float *alpha) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!cam || !rgb) return fail(c, SV_E_ARG, "camera and output are required");
+  if (c->N == 0) return fail(c, SV_E_STATE, "no Gaussians");
+  if (!c->noalloc) return fail(c, SV_E_STATE, "no scratch on device %d", c->device);
+  const uint32_t ts = c->cfg.tile_size;
+  const double w = std::round((double)cam->viewport[0]), hh = std::round((double)cam->viewport[1]);
+  if (!(w >= 1.0) || !(hh >= 1.0) || w > 65535.0 * ts || hh > 65535.0 * ts)
+    return fail(c, SV_E_ARG, "camera has an invalid viewport");
+  const uint32_t W = (uint32_t)w, H = (uint32_t)hh, N = c->N;
+  const uint32_t tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
+  const uint64_t P = (uint64_t)W * H, T = (uint64_t)tx * ty;
+  if (N > NOALLOC_MAX_SPLATS)
+    return fail(c, SV_E_CAPACITY, "%u Gaussians exceed the scratch render's maximum of %u", N, NOALLOC_MAX_SPLATS);
+  if (P > NOALLOC_MAX_PIXELS || T > NOALLOC_MAX_TILES)
+    return fail(c, SV_E_CAPACITY, "%ux%u exceeds the scratch render's maximum of %u pixels and %u tiles", W, H,
+                NOALLOC_MAX_PIXELS, NOALLOC_MAX_TILES);
+  // carve the scratch for the maxima (the same offsets on every call)
+  uintptr_t off = 0;
+  auto take = [&](size_t bytes) -> void * {
+    void *p = c->noalloc + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  const size_t NS = NOALLOC_MAX_SPLATS, NP = NOALLOC_MAX_PAIRS, NX = NOALLOC_MAX_PIXELS, NT = NOALLOC_MAX_TILES;
+  const size_t nbmax = project_blocks(NOALLOC_MAX_SPLATS) + 1;
+  DevView *d_view = (DevView *)take(sizeof(DevView));
+  StepMeta *m = (StepMeta *)take(4 * sizeof(StepMeta));
+  float4 *rec = (float4 *)take(4 * NS * sizeof(float4));
+  float2 *aux = (float2 *)take(NS * sizeof(float2));
+  uint32_t *counts = (uint32_t *)take(NS * 4), *offs = (uint32_t *)take(NS * 4), *tins = (uint32_t *)take(NS * 4);
+  uint2 *rects = (uint2 *)take(NS * sizeof(uint2));
+  uint32_t *btot = (uint32_t *)take(nbmax * 4), *bbase = (uint32_t *)take(nbmax * 4);
+  uint64_t *scan_tmp = (uint64_t *)take(scan_tmp_words(std::max(NS, NT + 1)) * 8);
+  uint32_t *keys = (uint32_t *)take(NP * 4), *keys2 = (uint32_t *)take(NP * 4);
+  uint32_t *vals = (uint32_t *)take(NP * 4), *vals2 = (uint32_t *)take(NP * 4);
+  uint32_t *sort_tmp = (uint32_t *)take(sort_tmp_words<uint32_t>(NP) * 4);
+  uint32_t *tile_view = (uint32_t *)take((NT + 1) * 4), *ranges = (uint32_t *)take((NT + 1) * 4);
+  float *color = (float *)take(3 * NX * 4), *alph = (float *)take(NX * 4), *out = (float *)take(3 * NX * 4);
+  uint32_t *last = (uint32_t *)take(NX * 4);
+  if (off > NOALLOC_SCRATCH_BYTES)
+    return fail(c, SV_E_STATE, "scratch layout needs %zu bytes of %zu", (size_t)off, (size_t)NOALLOC_SCRATCH_BYTES);
+  // the one view, as prepare_batch sets up a batch of one
+  DevView d{};
+  std::memcpy(d.view, cam->view, sizeof(d.view));
+  std::memcpy(d.proj, cam->proj, sizeof(d.proj));
+  d.focal[0] = cam->focal[0];
+  d.focal[1] = cam->focal[1];
+  d.viewport[0] = cam->viewport[0];
+  d.viewport[1] = cam->viewport[1];
+  d.W = W;
+  d.H = H;
+  d.tiles_x = tx;
+  d.tiles_y = ty;
+  d.n_tiles = (uint32_t)T;
+  d.tile_base = 0;
+  d.pix_off = 0;
+  d.super_x = (tx + BIN_S - 1) / BIN_S;
+  d.super_base = 0;
+  const DevConfig dc = dev_config(c->cfg);
+  const uint32_t tile_bits = bitlen(T > 0 ? T - 1 : 0);
+  const double far = (double)c->cfg.far_plane;
+  const uint64_t dq_max = (far >= 0.0) ? (uint64_t)std::floor(std::min(far, 65535.0)) : 65535u;
+  const uint32_t max_key_bits = tile_bits + bitlen(dq_max);  // <= 13 + 16 bits: a 32-bit key
+  hipStream_t st = c->st;
+  HIP_OK(c, hipMemcpyAsync(d_view, &d, sizeof(DevView), hipMemcpyHostToDevice, st));
+  launch_fill_u32(st, (uint32_t *)m, 0u, 4 * sizeof(StepMeta) / 4);
+  launch_fill_u32(st, tile_view, 0u, T);
+  launch_project(st, c->P, c->cap, N, d_view, 1, dc, rec, aux, counts, &m->max_dq, nullptr, rects, tins, offs,
+                 nullptr, btot, nullptr);
+  launch_scan(st, btot, bbase, project_blocks(N), scan_tmp, &m->n_isect);
+  launch_keyinfo(st, m, tile_bits, max_key_bits, NP);
+  launch_emit<uint32_t>(st, rec, rects, aux, counts, offs, bbase, N, 1, d_view, m, keys, vals);
+  launch_sort<uint32_t>(st, keys, vals, keys2, vals2, NP, (int)((max_key_bits + 7) / 8), sort_tmp, m);
+  launch_ranges<uint32_t>(st, keys, keys2, NP, (uint32_t)T, m, ranges);
+  RasterArgs a{};
+  a.views = d_view;
+  a.tile_view = tile_view;
+  a.ranges = ranges;
+  a.vals0 = vals;
+  a.vals1 = vals2;
+  a.meta = m;
+  a.rec = rec;
+  a.N = N;
+  a.n_tiles_total = (uint32_t)T;
+  a.cfg = dc;
+  a.color = color;
+  a.alpha = alph;
+  a.last = last;
+  a.bbase = bbase;
+  a.nb = project_blocks(N);
+  a.aux = aux;
+  launch_forward(st, a, c->cfg.batch, (uint32_t)T);
+  launch_interleave3(st, color, P, out);
+  if (sv_status s = check_launch(c, "scratch render")) return s;
+  StepMeta hm{};
+  HIP_OK(c, hipMemcpyAsync(&hm, m, sizeof(StepMeta), hipMemcpyDeviceToHost, st));
+  HIP_OK(c, hipStreamSynchronize(st));
+  if (hm.overflow == 1)
+    return fail(c, SV_E_CAPACITY, "%llu intersections exceed the scratch render's maximum of %u",
+                (unsigned long long)hm.n_isect, NOALLOC_MAX_PAIRS);
+  if (hm.overflow)
+    return fail(c, SV_E_STATE, "scratch render: sort key of %u bits", hm.key_bits);
+  HIP_OK(c, hipMemcpy(rgb, out, 3 * P * sizeof(float), hipMemcpyDeviceToHost));
+  if (alpha) HIP_OK(c, hipMemcpy(alpha, alph, P * sizeof(float), hipMemcpyDeviceToHost));
+  return SV_OK;
 }
 
 sv_status sv_read_image(sv_ctx *c, int v, float *rgb) {

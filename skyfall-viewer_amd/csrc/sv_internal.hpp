@@ -276,4 +276,18 @@ struct KnnScratch {
 // Writes the three log-scale planes of P and w.density.
 hipError_t launch_knn_init(hipStream_t st, float *P, size_t cap, uint32_t N, float extent, const KnnScratch &w);
 
+// sv_noalloc.hip --------------------------------------------------------
+// Fixed device scratch of sv_render_camera_noalloc (one copy per device, held
+// until the process ends) and the largest render it serves.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+constexpr uint32_t NOALLOC_MAX_SPLATS = 1u << 16;
+constexpr uint32_t NOALLOC_MAX_PAIRS = 1u << 20;
+constexpr uint32_t NOALLOC_MAX_PIXELS = 1u << 20;
+constexpr uint32_t NOALLOC_MAX_TILES = 8192;
+// Device address of the scratch on `device`, which must be current;
+// resolved once per device (hipGetSymbolAddress) and kept.
+hipError_t noalloc_scratch(int device, void **out);
+// Colour planes (c * P + i) -> interleaved rgb (3 i + c).
+void launch_interleave3(hipStream_t st, const float *planes, size_t P, float *out);
+
 }  // namespace sv

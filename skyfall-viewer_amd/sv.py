@@ -90,6 +90,7 @@ _SIGS = {
     "sv_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(StepStats)]),
     "sv_render_batched": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int]),
     "sv_render_camera": (C.c_int, [C.c_void_p, C.POINTER(Camera), _P, _P]),
+    "sv_render_camera_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Camera), _P, _P]),
     "sv_read_image": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_read_alpha": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_read_last_ids": (C.c_int, [C.c_void_p, C.c_int, _U32]),
@@ -432,6 +433,14 @@ class Context:
         rgb = np.empty((h, w, 3), np.float32)
         alpha = np.empty((h, w), np.float32)
         self._ck(lib().sv_render_camera(self.h, C.byref(cam), _fp(rgb), _fp(alpha)), "render_camera")
+        return rgb, alpha
+
+    def render_camera_noalloc(self, cam: Camera):
+        """render_camera through the fixed device scratch (no allocation)."""
+        w, h = int(round(cam.viewport[0])), int(round(cam.viewport[1]))
+        rgb = np.empty((h, w, 3), np.float32)
+        alpha = np.empty((h, w), np.float32)
+        self._ck(lib().sv_render_camera_noalloc(self.h, C.byref(cam), _fp(rgb), _fp(alpha)), "render_camera_noalloc")
         return rgb, alpha
 
     def render_batched(self, views):
