@@ -285,6 +285,77 @@ def run_growth4m(args):
     print(json.dumps(out), flush=True)
 
 
+def run_adjoint2m(args):
+    """The memory / recompute comparison at 2,000,000 splats and one 1920x1080
+    view (camera at (0,0,3), 60 degree FOV): the same training step with the
+    adjoint backward, which rebuilds transmittance by reverse traversal from
+    the stored final alpha and last index (8 B per pixel kept from the
+    forward; 36 B adjoint records per reached pair written during the step),
+    against the reference's method, which stores 54 fixed-point gradient
+    slots per intersection (216 B, backward-tiles-local.wgsl:44-53, allocated
+    at gaussian-renderer.ts:1281-1349) and reduces them. For each: steps/s
+    and the device memory the context holds (peak, sv_memory_stats). A stored
+    per-pixel contributor list (every visited (pixel, splat) pair's index and
+    alpha, 8 B) is not built; its size is counted from the forward's last
+    indices and tile ranges."""
+    N, W, H = 2_000_000, 1920, 1080
+    g = sv.scene_gaussians(0, N, 1.0)
+    cams = [sv.scene_camera([0.0, 0.0, 3.0], [0.0, 0.0, 0.0], W, H, 60.0)]
+    targets = render_targets(g, cams, args)
+    res = {}
+    for name, mode, graph in (("adjoint_batched", sv.MODE_BATCHED, args.graph),
+                              ("adjoint_sequential", sv.MODE_SEQUENTIAL, 0),
+                              ("reference_stored_tile_grads", sv.MODE_PLAIN, 0)):
+        c = sv.Context(capacity=N)
+        configure(c, args)
+        c.set_config(use_graph=graph)
+        c.set_gaussians(g)
+        c.set_views(cams)
+        c.set_target(0, targets[0])
+        base = c.memory_stats()[0]
+        for _ in range(args.warmup):
+            c.step_views([0], mode)
+        c.sync()
+        c.reset_memory_peak()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = c.step_views([0], mode, stats=True)
+        c.sync()
+        dt = (time.perf_counter() - t0) / args.steps
+        now, peak = c.memory_stats()
+        res[name] = {"ms_per_step": 1e3 * dt, "steps_per_s": 1.0 / dt, "device_bytes_peak": peak,
+                     "device_bytes_before_first_step": base, "intersections": int(st.n_isect)}
+        if mode == sv.MODE_SEQUENTIAL:
+            last = c.last_ids(0).ravel().astype(np.int64)
+            offs = c.tile_offsets(0).astype(np.int64)
+            tx = (W + 15) // 16
+            yy, xx = np.divmod(np.arange(W * H), W)
+            start = offs[(yy // 16) * tx + xx // 16]
+            ok = last != 0xFFFFFFFF
+            visited = int(np.sum(last[ok] - start[ok] + 1))
+            res["stored_pixel_lists"] = {"built": False, "visited_pairs": visited,
+                                         "bytes": 8 * visited, "note": "index (4 B) + alpha (4 B) per visited "
+                                         "(pixel, splat) pair, counted from the forward's last indices"}
+        c.close()
+        log(f"adjoint2m {name}: {1e3 * dt:.2f} ms/step, peak {peak / 2**30:.2f} GiB")
+    I = res["adjoint_sequential"]["intersections"]
+    out = {
+        "metric": "train step time and device memory at 2M splats 1080p: reverse-traversal adjoint vs stored "
+                  "per-intersection gradients",
+        "value": res["adjoint_batched"]["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["adjoint_batched"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generator; target rendered from a jittered copy)",
+        "config": {"workload": f"{N} splats, 1 view {W}x{H} from (0,0,3)", "splats": N, "views": 1, "width": W,
+                   "height": H},
+        "methods": res,
+        "per_unit_bytes": {"reverse_traversal_saved_per_pixel": 8, "adjoint_record_per_reached_pair": 36,
+                           "stored_tile_grads_per_intersection": 216, "stored_pixel_list_per_visited_pair": 8},
+        "stored_tile_grads_bytes": 216 * I,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def make_workload(name: str, args):
     """(splats, cameras, sizes, width, height, description) of a workload."""
     N, V, W, H = args.n, args.views, args.width, args.height
@@ -356,12 +427,15 @@ def secondary(name, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=("ring16", "uneven64", "small256", "ring8_720p", "growth4m"), default="ring16",
+    ap.add_argument("--workload", choices=("ring16", "uneven64", "small256", "ring8_720p", "growth4m", "adjoint2m"),
+                    default="ring16",
                     help="ring16: 16 ring views at --width x --height (the headline); uneven64: 64 views "
                          "with seeded sizes 320-1920 x 240-1080 on a sphere of radius 4, means in [-2,2]^3; "
                          "small256: 16,384 splats, one 256x256 view from (0,0,3); ring8_720p: 500,000 splats, 8 ring "
                          "views at 1280x720; growth4m: the growth run (1M splats growing to at most 4M, 16 ring "
-                         "views at 1080p, prune / densify every 100 steps; give --steps, e.g. 2000)")
+                         "views at 1080p, prune / densify every 100 steps; give --steps, e.g. 2000); adjoint2m: "
+                         "2M splats, one 1080p view: step time and device memory of the reverse-traversal adjoint "
+                         "against the reference's stored per-intersection gradients")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1_000_000)
@@ -403,6 +477,9 @@ def main():
 
     if args.workload == "growth4m":
         run_growth4m(args)
+        return
+    if args.workload == "adjoint2m":
+        run_adjoint2m(args)
         return
     g, cams, sizes, W, H, workload = make_workload(args.workload, args)
     N, V = g["pos"].shape[0], len(cams)

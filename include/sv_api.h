@@ -390,6 +390,14 @@ sv_status sv_load_checkpoint(sv_ctx *ctx, const char *path);
 sv_status sv_autotune(sv_ctx *ctx, const int *views, int nviews, int steps,
                       sv_tune_entry *out, int *count);
 
+/* --- device memory ------------------------------------------------- */
+/* No counterpart in the reference (WebGPU buffer sizes are not reported).
+ * Device memory the context holds now (its parameter, moment, gradient,
+ * pixel and tile-build buffers) and the most it has held since sv_create
+ * or the last sv_reset_memory_peak. Either pointer may be NULL. */
+sv_status sv_memory_stats(const sv_ctx *ctx, uint64_t *device_bytes, uint64_t *peak_bytes);
+sv_status sv_reset_memory_peak(sv_ctx *ctx);
+
 /* --- timing --------------------------------------------------------- */
 /* Enable per-kernel hipEvent timing on the context stream. */
 sv_status sv_enable_timing(sv_ctx *ctx, int enable);

@@ -127,6 +127,13 @@ def expf(x: np.ndarray) -> np.ndarray:
     return y
 
 
+def exp_alpha(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_exp_alpha_v(C.c_size_t(x.size), _fp(x), _fp(y))
+    return y
+
+
 def f16_round(x: np.ndarray) -> np.ndarray:
     x = np.ascontiguousarray(x, np.float32)
     y = np.empty_like(x)

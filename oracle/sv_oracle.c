@@ -10,6 +10,8 @@
  *
  * How each function is pinned (tests/test_oracle_*.py):
  *   orc_expf            libm expf over 2^20 samples, <= 2 ulp
+ *   orc_exp_alpha       float64 exp, within 3 + 2|x| ulp (WGSL's exp accuracy)
+ *                       over [-80, 1/2]
  *   orc_f16_round       numpy float16 round-to-nearest-even
  *   orc_project         closed form for an isotropic on-axis splat (conic,
  *                       radius); pixel-mean closed form (fx x/z + cx,
@@ -85,6 +87,30 @@ static const float SH_C3[7] = {-0.5900435899266435f, 2.890611442640554f,
 /* ------------------------------------------------------------------ */
 static inline float bits_to_f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static inline uint32_t f_to_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* exp of the per-pixel splat weight e^-power (training-forward-tiles.wgsl:
+ * 190 and the backward passes), for x in [-80, 1/2]: 2^(x log2 e) with the
+ * product rounded, k = rint(y), f = y - k, 2^f by a degree-5 polynomial with
+ * constant term 1 fitted for relative error on [-1/2, 1/2], times 2^k built
+ * in the exponent bits. Within 3 + 2|x| ulp of e^x (WGSL's accuracy
+ * requirement for exp). Callers clamp the power to 80 first, as the device
+ * does (a power above 80 gives an alpha far below 1/255 either way). */
+float orc_exp_alpha(float x) {
+  const float y = x * 0x1.715476p+0f;
+  const float k = rintf(y);
+  const float f = y - k;
+  float p = 0x1.5bba14p-10f;
+  p = fmaf(p, f, 0x1.3cea88p-7f);
+  p = fmaf(p, f, 0x1.c6b752p-5f);
+  p = fmaf(p, f, 0x1.ebf9bcp-3f);
+  p = fmaf(p, f, 0x1.62e42ap-1f);
+  p = fmaf(p, f, 1.0f);
+  return p * bits_to_f((uint32_t)((int)k + 127) << 23);
+}
+
+void orc_exp_alpha_v(size_t n, const float *x, float *y) {
+  for (size_t i = 0; i < n; ++i) y[i] = orc_exp_alpha(x[i]);
+}
 
 float orc_expf(float x) {
   if (x != x) return x;
@@ -631,7 +657,7 @@ void orc_forward(const orc_params *p, const orc_proj *proj, const orc_camera *ca
         orc_view_dir(cam, gp->position_world, dir);
         const float opacity = sigmoid_ref(gp->opacity_logit);
         orc_sh_color(dir, &p->sh[48 * (size_t)g], v->sh_deg, vc);
-        const float gauss = orc_expf(-s);
+        const float gauss = orc_exp_alpha(-fminf(s, 80.0f));
         const float al = opacity * gauss;
         const float as = fminf(al, MAX_FRAGMENT_ALPHA);
         if (as < MIN_ALPHA) continue;
@@ -744,7 +770,7 @@ static int adjoint_pair(const orc_params *p, const orc_proj *proj,
   const float dx = px - gp->mean2d[0], dy = py - gp->mean2d[1];
   const float s = 0.5f * (aa * dx * dx + cc * dy * dy) + bb * dx * dy;
   if (s < 0.0f) return 0;
-  const float gauss = orc_expf(-s);
+  const float gauss = orc_exp_alpha(-fminf(s, 80.0f));
   float dir[3];
   orc_view_dir(cam, gp->position_world, dir);
   const float opacity = sigmoid_ref(gp->opacity_logit);

@@ -78,6 +78,10 @@ typedef struct orc_grads {
 #define ORC_WF_REFERENCE_ORDER 0x1u /* batches front-to-back, :262 */
 #define ORC_WF_SKIP_098        0x2u /* skip alpha > 0.98, :329-331 */
 
+/* exp of the per-pixel splat weight (2^(x log2 e), degree-5 polynomial),
+ * for x in [-80, 1/2]; within 3 + 2|x| ulp of e^x. */
+float orc_exp_alpha(float x);
+void orc_exp_alpha_v(size_t n, const float *x, float *y);
 float orc_expf(float x);
 float orc_f16_round(float x);
 

@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <unordered_map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -168,12 +170,19 @@ struct sv_ctx {
 
   // device address of the fixed scratch of sv_render_camera_noalloc
   unsigned char *noalloc = nullptr;
+  // persistent raster launches: per-XCD work counters and the resident
+  // workgroups of [forward, backward] x LDS batch [64, 128, 256]
+  uint32_t *d_work = nullptr;
+  uint32_t resident[2][3] = {{0, 0, 0}, {0, 0, 0}};
 
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   std::vector<uint64_t> gsig;
   bool capturing = false;
+
+  // device memory held by the context (dalloc / dfree) and its peak
+  uint64_t dev_bytes = 0, dev_peak = 0;
 
   // timing
   bool timing = false;
@@ -212,6 +221,11 @@ sv_status check_launch(sv_ctx *c, const char *what) {
   return SV_OK;
 }
 
+// Every device allocation of a context, so that sv_memory_stats can report
+// what the context holds (allocation -> owning context, bytes).
+std::mutex g_alloc_mu;
+std::unordered_map<const void *, std::pair<sv_ctx *, size_t>> g_allocs;
+
 template <typename T>
 sv_status dalloc(sv_ctx *c, T **p, size_t n) {
   *p = nullptr;
@@ -222,12 +236,28 @@ sv_status dalloc(sv_ctx *c, T **p, size_t n) {
     (void)hipGetLastError();
     return fail(c, SV_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
   }
+  if (c) {
+    std::lock_guard<std::mutex> lock(g_alloc_mu);
+    g_allocs[(const void *)*p] = {c, n * sizeof(T)};
+    c->dev_bytes += n * sizeof(T);
+    c->dev_peak = std::max(c->dev_peak, c->dev_bytes);
+  }
   return SV_OK;
 }
 
 template <typename T>
 void dfree(T *&p) {
-  if (p) (void)hipFree((void *)p);
+  if (p) {
+    {
+      std::lock_guard<std::mutex> lock(g_alloc_mu);
+      auto it = g_allocs.find((const void *)p);
+      if (it != g_allocs.end()) {
+        it->second.first->dev_bytes -= it->second.second;
+        g_allocs.erase(it);
+      }
+    }
+    (void)hipFree((void *)p);
+  }
   p = nullptr;
 }
 
@@ -724,18 +754,23 @@ RasterArgs raster_args(sv_ctx *c) {
   return a;
 }
 
-uint32_t raster_grid(sv_ctx *c, RasterArgs &a) {
+// Grid of a raster launch: one workgroup per tile, or with `persistent` as
+// many workgroups as the device holds at once (occupancy x CUs), pulling
+// tiles from the per-XCD work queues.
+uint32_t raster_grid(sv_ctx *c, RasterArgs &a, bool backward) {
   if (c->cfg.persistent) {
-    a.work_counter = &c->d_meta->work;
-    launch_fill_u32(c->st, a.work_counter, 0u, 1);
-    return std::min<uint32_t>(c->tiles_total, 256u * 8u);
+    const int bi = c->cfg.batch <= 64 ? 0 : (c->cfg.batch <= 128 ? 1 : 2);
+    const uint32_t resident = c->resident[backward ? 1 : 0][bi];
+    a.work_counter = c->d_work;
+    launch_fill_u32(c->st, a.work_counter, 0u, (size_t)NUM_WORK_QUEUES * WORK_QUEUE_STRIDE);
+    return std::min<uint32_t>(c->tiles_total, resident ? resident : 2048u);
   }
   return c->tiles_total;
 }
 
 sv_status enqueue_forward(sv_ctx *c) {
   RasterArgs a = raster_args(c);
-  const uint32_t grid = raster_grid(c, a);
+  const uint32_t grid = raster_grid(c, a, false);
   {
     Scoped t(c, PH_FORWARD);
     launch_forward(c->st, a, c->cfg.batch, grid);
@@ -754,7 +789,7 @@ sv_status forward_batch(sv_ctx *c) {
 sv_status enqueue_backward(sv_ctx *c, bool use_residual) {
   RasterArgs a = raster_args(c);
   if (use_residual) a.residual = c->residual;
-  const uint32_t grid = raster_grid(c, a);
+  const uint32_t grid = raster_grid(c, a, true);
   {
     Scoped t(c, PH_BACKWARD);
     launch_backward(c->st, a, c->cfg.batch, grid);
@@ -1027,7 +1062,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
                         c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
-                        c->chunk_super, c->ccount};
+                        c->chunk_super, c->ccount, c->d_work};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
   g.push_back(c->N);
   g.push_back(c->cap);
@@ -1280,6 +1315,12 @@ sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
   sv_status s = alloc_gaussian_arrays(c, capacity);
   if (!s) s = dalloc(c, &c->d_small, 8);
   if (!s) s = dalloc(c, &c->d_meta, 4);
+  if (!s) s = dalloc(c, &c->d_work, (size_t)NUM_WORK_QUEUES * WORK_QUEUE_STRIDE);
+  if (!s) {
+    const uint32_t bs[3] = {64, 128, 256};
+    for (int k = 0; k < 2; ++k)
+      for (int b = 0; b < 3; ++b) c->resident[k][b] = raster_resident_blocks(k == 1, bs[b]);
+  }
   if (!s) s = dalloc(c, &c->d_adam, 1);
   if (!s && hipHostMalloc((void **)&c->h_small, 8 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
     s = fail(c, SV_E_NOMEM, "hipHostMalloc failed");
@@ -1340,6 +1381,7 @@ sv_status sv_destroy(sv_ctx *c) {
   drop_graph(c);
   dfree(c->d_small);
   dfree(c->d_meta);
+  dfree(c->d_work);
   dfree(c->d_adam);
   if (c->h_meta) (void)hipHostFree(c->h_meta);
   if (c->h_adam) (void)hipHostFree(c->h_adam);
@@ -2511,6 +2553,21 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
   const int n = std::min(*count, (int)cand.size());
   for (int i = 0; i < n; ++i) out[i] = cand[i];
   *count = (int)cand.size();
+  return SV_OK;
+}
+
+sv_status sv_memory_stats(const sv_ctx *c, uint64_t *device_bytes, uint64_t *peak_bytes) {
+  if (!c) return SV_E_ARG;
+  std::lock_guard<std::mutex> lock(g_alloc_mu);
+  if (device_bytes) *device_bytes = c->dev_bytes;
+  if (peak_bytes) *peak_bytes = c->dev_peak;
+  return SV_OK;
+}
+
+sv_status sv_reset_memory_peak(sv_ctx *c) {
+  if (!c) return SV_E_ARG;
+  std::lock_guard<std::mutex> lock(g_alloc_mu);
+  c->dev_peak = c->dev_bytes;
   return SV_OK;
 }
 

@@ -159,8 +159,14 @@ struct RasterArgs {
   const float2 *aux;
   uint64_t *tile_cut;
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
-  uint32_t *work_counter; // persistent queue counter (null: one block per tile)
+  uint32_t *work_counter; // persistent launch: NUM_WORK_QUEUES counters,
+                          // WORK_QUEUE_STRIDE words apart (null: one block per tile)
 };
+constexpr uint32_t NUM_WORK_QUEUES = 8;     // one per XCD
+constexpr uint32_t WORK_QUEUE_STRIDE = 32;  // words between counters (a 128-byte line each)
+// Workgroups of the forward / backward raster kernel (LDS batch `batch`)
+// resident on the whole device at once: the persistent grid. 0 if unknown.
+uint32_t raster_resident_blocks(bool backward, uint32_t batch);
 void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 // backward_tiles_local (backward-tiles-local.wgsl:185-382) for one slot:

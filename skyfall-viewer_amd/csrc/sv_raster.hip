@@ -25,16 +25,6 @@ __device__ __forceinline__ bool resolve_meta(RasterArgs &a) {
   return true;
 }
 
-__device__ __forceinline__ bool next_tile(const RasterArgs &a, uint32_t &tg, uint32_t *s_tg) {
-  if (a.work_counter == nullptr) return false;
-  __syncthreads();
-  if (threadIdx.x == 0) *s_tg = atomicAdd(a.work_counter, 1u);
-  __syncthreads();
-  // uniform over the workgroup: keep it (and what derives from it) scalar
-  tg = __builtin_amdgcn_readfirstlane(*s_tg);
-  return tg < a.n_tiles_total;
-}
-
 // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD
 // b mod 8). With xcd_remap the blocks of one XCD take one contiguous range
 // of work items (a bijection on [0, n)), so neighbouring tiles, which read
@@ -46,11 +36,42 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
   return x * q + min(x, r) + b / NUM_XCD;
 }
 
-__device__ __forceinline__ uint32_t first_tile(const RasterArgs &a, uint32_t *s_tg) {
+// Persistent launch: one work queue per XCD, each a contiguous part of the
+// work items (the same parts as xcd_remap), its counter on a line of its
+// own (WORK_QUEUE_STRIDE words apart: no two XCDs' atomics share a cache
+// line). A workgroup pulls from its own XCD's queue and, once that is
+// drained, from the others in turn; when every queue is drained it gets n
+// (no item).
+static_assert(NUM_WORK_QUEUES == NUM_XCD, "one work queue per XCD");
+
+// Called by the whole wave (the raster workgroups are one wave): lane 0 does
+// the atomics, the item is read from it, every value stays scalar.
+__device__ __forceinline__ uint32_t pull_item(const RasterArgs &a) {
+  const uint32_t n = a.n_tiles_total, q = n / NUM_XCD, r = n % NUM_XCD;
+  const uint32_t home = blockIdx.x % NUM_XCD;
+#pragma unroll 1
+  for (uint32_t d = 0; d < NUM_XCD; ++d) {
+    const uint32_t x = (home + d) % NUM_XCD;
+    const uint32_t len = q + (x < r ? 1u : 0u);
+    uint32_t t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(a.work_counter + x * WORK_QUEUE_STRIDE, 1u);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t < len) return x * q + min(x, r) + t;
+  }
+  return n;
+}
+
+__device__ __forceinline__ uint32_t first_tile(const RasterArgs &a) {
   if (a.work_counter == nullptr) return a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
-  if (threadIdx.x == 0) *s_tg = atomicAdd(a.work_counter, 1u);
-  __syncthreads();
-  return __builtin_amdgcn_readfirstlane(*s_tg);
+  return pull_item(a);
+}
+
+// The next work item of a persistent launch (false: none left, or one
+// workgroup per tile).
+__device__ __forceinline__ bool next_tile(const RasterArgs &a, uint32_t &tg) {
+  if (a.work_counter == nullptr) return false;
+  tg = pull_item(a);
+  return tg < a.n_tiles_total;
 }
 
 // DPP move of a float (0 in lanes disabled by the row mask).
@@ -333,7 +354,7 @@ __device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0,
   const float dx = fx - x0.x;
   const float dy = fy - x0.y;
   const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-  const float gauss = sv_expf_neg(-fminf(sg, 80.0f));
+  const float gauss = sv_exp_alpha(-fminf(sg, 80.0f));
   *as = fminf(x1.y * gauss, SV_MAX_ALPHA);
   return !(sg < 0.0f) && !(*as < SV_MIN_ALPHA);
 }
@@ -369,8 +390,7 @@ __global__ __launch_bounds__(WT, 7) void k_forward(RasterArgs ain) {
   __shared__ float4 s_q1[B];
   __shared__ float4 s_q2[B];
   __shared__ uint32_t s_k[B];
-  __shared__ uint32_t s_tg;
-  uint32_t tg = first_tile(a, &s_tg);
+  uint32_t tg = first_tile(a);
   while (tg < a.n_tiles_total) {
     const TileGeom t = tile_geom(a, tg);
     const DevView &V = *t.V;
@@ -467,7 +487,7 @@ __global__ __launch_bounds__(WT, 7) void k_forward(RasterArgs ain) {
         }
       }
     }
-    if (!next_tile(a, tg, &s_tg)) break;
+    if (!next_tile(a, tg)) break;
   }
 }
 
@@ -498,9 +518,8 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
   __shared__ float4 s_q2[B];
   __shared__ uint32_t s_k[B];
   __shared__ uint32_t s_p[B];
-  __shared__ uint32_t s_tg;
   const int lane = threadIdx.x;
-  uint32_t tg = first_tile(a, &s_tg);
+  uint32_t tg = first_tile(a);
   while (tg < a.n_tiles_total) {
     const TileGeom t = tile_geom(a, tg);
     const DevView &V = *t.V;
@@ -508,7 +527,11 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
     const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
     const size_t vbase = (size_t)t.s * a.N;
     if (start < end) {
-      float r0[PPL], r1[PPL], r2[PPL], T[PPL], acc0[PPL], acc1[PPL], acc2[PPL], fxh[2], fyh[2];
+      // sacc: the residual's dot product with the colour accumulated behind
+      // the current entry (<r, acc>, background first); the update acc' =
+      // alpha c + (1 - alpha) acc is linear, so <r, acc'> = alpha <r, c> +
+      // (1 - alpha) <r, acc> keeps one sum per pixel instead of three
+      float r0[PPL], r1[PPL], r2[PPL], T[PPL], sacc[PPL], fxh[2], fyh[2];
       uint32_t lastp[PPL];
       bool valid[PPL];
       float l1 = 0.0f;
@@ -527,9 +550,6 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
         T[j] = 1.0f;
         lastp[j] = 0;
         valid[j] = false;
-        acc0[j] = a.cfg.bg[0];
-        acc1[j] = a.cfg.bg[1];
-        acc2[j] = a.cfg.bg[2];
         if (px < V.W && py < V.H) {
           const size_t lp = (size_t)py * V.W + px;
           lastp[j] = a.last[V.pix_off + lp];
@@ -551,6 +571,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
           valid[j] = (lastp[j] != NO_LAST) && lastp[j] >= start;
           if (valid[j]) m = max(m, lastp[j] + 1u);
         }
+        sacc[j] = fmaf(r2[j], a.cfg.bg[2], fmaf(r1[j], a.cfg.bg[1], r0[j] * a.cfg.bg[0]));
       }
       const float l1t = wave_total63(l1);
       if (lane == 63 && a.l1_partial) a.l1_partial[tg] = l1t;
@@ -596,7 +617,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               const float dx = fxh[j & 1] - x0.x;
               const float dy = fyh[j >> 1] - x0.y;
               const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
-              const float gauss = sv_expf_neg(-fminf(sg, 80.0f));  // as frag_alpha
+              const float gauss = sv_exp_alpha(-fminf(sg, 80.0f));  // as frag_alpha
               const float op = x1.y;
               const float as = fminf(op * gauss, SV_MAX_ALPHA);
               const bool take = (valid[j] & (i <= lastp[j])) & !(sg < 0.0f) & !(as < SV_MIN_ALPHA);
@@ -610,10 +631,8 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               v0 = fmaf(r0[j], contrib, v0);
               v1 = fmaf(r1[j], contrib, v1);
               v2 = fmaf(r2[j], contrib, v2);
-              const float cr = x1.z, cg = x1.w, cb = x2.x;
-              const float fg = fmaf(r2[j], cb, fmaf(r1[j], cg, r0[j] * cr));
-              const float bgt = fmaf(r2[j], acc2[j], fmaf(r1[j], acc1[j], r0[j] * acc0[j]));
-              const float dL_dalpha = (fg - bgt) * Tb;
+              const float fg = fmaf(r2[j], x2.x, fmaf(r1[j], x1.w, r0[j] * x1.z));
+              const float dL_dalpha = (fg - sacc[j]) * Tb;
               v3 = fmaf(dL_dalpha * as, 1.0f - op, v3);
               const float nds = dL_dalpha * (op * gauss);  // -dL/dpower
               const float hds = 0.5f * nds;
@@ -622,10 +641,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               v6 = fmaf(-hds * dy, dy, v6);
               v7 = fmaf(nds, fmaf(x0.z, dx, x0.w * dy), v7);
               v8 = fmaf(nds, fmaf(x0.w, dx, x1.x * dy), v8);
-              const float om = 1.0f - as;
-              acc0[j] = fmaf(as, cr, om * acc0[j]);
-              acc1[j] = fmaf(as, cg, om * acc1[j]);
-              acc2[j] = fmaf(as, cb, om * acc2[j]);
+              sacc[j] = fmaf(as, fg, (1.0f - as) * sacc[j]);
               T[j] = Tb;
             }
           }
@@ -648,7 +664,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
     } else if (lane == 0 && a.l1_partial) {
       a.l1_partial[tg] = 0.0f;
     }
-    if (!next_tile(a, tg, &s_tg)) break;
+    if (!next_tile(a, tg)) break;
   }
 }
 
@@ -750,7 +766,7 @@ __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs ain, const flo
         const float dx = fx - x0.x, dy = fy - x0.y;
         const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
         if (sg < 0.0f) continue;
-        const float gauss = sv_expf(-sg);
+        const float gauss = sv_exp_alpha(-fminf(sg, 80.0f));
         const float op = x1.y;
         const float as = fminf(op * gauss, SV_MAX_ALPHA);
         if (as < SV_MIN_ALPHA) continue;
@@ -789,6 +805,22 @@ __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs ain, const flo
     for (uint32_t z = threadIdx.x; z < bsz * WF_S; z += RT) fixed[(size_t)bs * WF_S + z] = s_acc[z];
     __syncthreads();
   }
+}
+
+uint32_t raster_resident_blocks(bool backward, uint32_t batch) {
+  int per_cu = 0, cus = 0, dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const void *k = backward ? (batch <= 64 ? (const void *)k_backward<64>
+                                          : batch <= 128 ? (const void *)k_backward<128> : (const void *)k_backward<256>)
+                           : (batch <= 64 ? (const void *)k_forward<64>
+                                          : batch <= 128 ? (const void *)k_forward<128> : (const void *)k_forward<256>);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, WT, 0);
+  if (e != hipSuccess || per_cu <= 0 || cus <= 0) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return (uint32_t)per_cu * (uint32_t)cus;
 }
 
 void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid) {

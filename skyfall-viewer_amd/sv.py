@@ -114,6 +114,8 @@ _SIGS = {
                               C.POINTER(C.c_int)]),
     "sv_save_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
     "sv_load_checkpoint": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "sv_memory_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "sv_reset_memory_peak": (C.c_int, [C.c_void_p]),
     "sv_enable_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "sv_timings": (C.c_int, [C.c_void_p, C.POINTER(KernelTime), C.POINTER(C.c_int)]),
     "sv_reset_timings": (C.c_int, [C.c_void_p]),
@@ -581,6 +583,16 @@ class Context:
         out = np.zeros(((w + 15) // 16) * ((h + 15) // 16), np.float32)
         self._ck(lib().sv_read_tile_l1(self.h, v, _fp(out)), "read_tile_l1")
         return out
+
+    # device memory ------------------------------------------------------------
+    def memory_stats(self):
+        """(bytes of device memory held now, peak since creation or the last reset)."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._ck(lib().sv_memory_stats(self.h, C.byref(a), C.byref(b)), "memory_stats")
+        return a.value, b.value
+
+    def reset_memory_peak(self):
+        self._ck(lib().sv_reset_memory_peak(self.h), "reset_memory_peak")
 
     # timing -------------------------------------------------------------------
     def enable_timing(self, on: bool = True):

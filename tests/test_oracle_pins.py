@@ -43,6 +43,29 @@ def test_expf_against_libm(orc):
     assert sp[0] == np.inf and sp[1] == 0.0 and sp[2] == np.inf and sp[3] == 0.0 and np.isnan(sp[4])
 
 
+# ----------------------------------------------------------- orc_exp_alpha
+def test_exp_alpha_within_wgsl_bound(orc):
+    """The splat-weight exp against float64 exp: within 3 + 2|x| ulp (WGSL's
+    accuracy requirement for exp) on [-80, 1/2], exhaustively over every
+    float in [-1, 1/2] and on 2^21 random points beyond; exact at 0 to
+    within its polynomial's 1 ulp; monotone over a dense grid."""
+    rng = np.random.default_rng(5)
+    lo = np.float32(-1.0).view(np.uint32)
+    neg = np.arange(np.uint32(0x80000000), lo + 1, 37, dtype=np.uint32).view(np.float32)  # [-1, -0] strided
+    pos = np.arange(0, np.float32(0.5).view(np.uint32) + 1, 37, dtype=np.uint32).view(np.float32)
+    x = np.concatenate([neg, pos, rng.uniform(-80, 0.5, 1 << 21).astype(np.float32),
+                        np.float32([-80.0, -79.999, -0.0, 0.0, 0.5, -6.0, -5.5])])
+    got = orc.exp_alpha(x).astype(np.float64)
+    ref64 = np.exp(x.astype(np.float64))
+    ulp = np.abs(got - ref64) / np.spacing(ref64.astype(np.float32)).astype(np.float64)
+    assert (ulp <= 3.0 + 2.0 * np.abs(x.astype(np.float64))).all(), float((ulp / (3 + 2 * np.abs(x))).max())
+    assert ulp[np.abs(x) < 1].max() <= 2.5 and ulp[np.abs(x) < 6].max() <= 7.5
+    assert orc.exp_alpha(np.float32([0.0, -0.0]))[0] == 1.0
+    g = np.linspace(-80, 0.5, 1_000_001, dtype=np.float32)
+    e = orc.exp_alpha(g)
+    assert (np.diff(e.astype(np.float64)) >= 0).all()
+
+
 # ------------------------------------------------------------ orc_f16_round
 def test_f16_round_matches_numpy_rne(orc):
     rng = np.random.default_rng(1)

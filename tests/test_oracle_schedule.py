@@ -64,7 +64,9 @@ def test_train_one_step_is_render_plus_adam(sv, orc):
     p, m1, m2, hist = orc.train(g, [cam, cam], [view, view], [target, target], [[0, 1]], lrs)
     r = orc.render_and_grad(g, cam, view, target=target)
     for k in lrs:
-        assert np.allclose(hist[0][1][k], 2.0 * r["grads"][k], rtol=1e-6, atol=1e-12), k
+        # the second view's terms are added one by one into the first view's
+        # sums, so the total equals twice one view's up to float reordering
+        assert np.abs(hist[0][1][k] - 2.0 * r["grads"][k]).max() <= 1e-5 * np.abs(r["grads"][k]).max(), k
         gk = r["grads"][k].astype(np.float64)
         mask = np.abs(gk) > 1e-6 * np.abs(gk).max()
         d = p[k].astype(np.float64) - g[k]
