@@ -127,8 +127,8 @@ def run_mode(mode, g, cams, targets, args, graph=0):
     if mode == sv.MODE_BATCHED and args.autotune:
         tune = ctx.autotune(views, 3)
         best = min(tune, key=lambda e: e["ms_per_step"])
-        log("autotune: " + ", ".join(f"b{e['batch']}/p{e['persistent']}/x{e['xcd_remap']} {e['ms_per_step']:.2f}"
-                                      for e in tune) + f" -> {best}")
+        log("autotune: " + ", ".join(f"t{e['tile']}/b{e['batch']}/p{e['persistent']}/x{e['xcd_remap']} "
+                                      f"{e['ms_per_step']:.2f}" for e in tune) + f" -> {best}")
     for i in range(args.warmup):
         ctx.step_views(views, mode)
     ctx.sync()
@@ -158,7 +158,7 @@ def run_mode(mode, g, cams, targets, args, graph=0):
         eager = (time.perf_counter() - e0) / args.steps
     st = ctx.step_views(views, mode, stats=True)
     cfg = ctx.config()
-    chosen = {"batch": cfg.batch, "persistent": cfg.persistent, "xcd_remap": cfg.xcd_remap}
+    chosen = {"batch": cfg.batch, "persistent": cfg.persistent, "xcd_remap": cfg.xcd_remap, "raster_px": cfg.raster_px}
     ctx.close()
     return (t1 - t0) / args.steps, timings, phase_steps, st, eager, tune, chosen
 
@@ -563,7 +563,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene generator; targets rendered from a jittered copy)",
         "config": {"workload": workload, "splats": N, "views": V, "width": W, "height": H, "pixels": P,
-                   "tile": 16, "lds_batch": chosen_b["batch"],
+                   "tile": 16, "raster_tile": chosen_b["raster_px"], "lds_batch": chosen_b["batch"],
                    "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"],
                    "binned_build": args.binned},
         "autotune": tune_b,

@@ -60,7 +60,10 @@ typedef struct sv_camera {
 typedef struct sv_config {
   uint32_t tile_size;  /* binning tile in pixels: 16 */
   uint32_t batch;      /* splats staged per LDS batch: 64, 128 or 256 */
-  uint32_t raster_px;  /* pixels per raster workgroup side: 8 or 16 */
+  uint32_t raster_px;  /* raster tile of the forward in pixels: 16 (one wave
+                          per binning tile), 8 (one wave per 8x8 quadrant) or
+                          32 (four waves per 2x2 tiles); the lists stay the
+                          16-px tiles', so the images are the same */
   uint32_t sh_deg;     /* RenderSettings.sh_deg, 0..3 */
   float bg[3];
   float near_plane, far_plane, radius_clip, eps2d;
@@ -122,7 +125,7 @@ typedef struct sv_tune_entry {
   uint32_t batch;       /* LDS batch of the rasterizers */
   uint32_t persistent;  /* persistent raster launch */
   uint32_t xcd_remap;   /* XCD-contiguous tile ranges */
-  uint32_t tile_size;   /* binning tile (16, see sv_autotune) */
+  uint32_t tile_size;   /* raster tile of the forward: 8, 16 or 32 */
   double ms_per_step;   /* mean device time of one batched step */
 } sv_tune_entry;
 
@@ -379,12 +382,14 @@ sv_status sv_load_checkpoint(sv_ctx *ctx, const char *path);
 /* No counterpart in the reference (its tile and batch sizes are
  * constants: 16x16 tiles, 64-entry batches, backward-tiles-local.wgsl:49-53,
  * 185). Times `steps` batched steps of `views` (after one untimed step) for
- * every candidate LDS batch 64/128/256 x persistent launch 0/1 x XCD tile
- * mapping 0/1, restoring parameters, moments, statistics and the step
- * counter before each candidate and at the end, and leaves the fastest
- * candidate in the configuration. The binning tile stays 16: it decides
- * which fringe pixels (power between the 3-sigma radius and ln(255 a)) a
- * splat is composited into, so another tile size changes the image.
+ * every candidate raster tile 8/16/32 x LDS batch 64/128/256 x XCD tile
+ * mapping 0/1 (and persistent launch 0/1 with 16-px raster tiles),
+ * restoring parameters, moments, statistics and the step counter before
+ * each candidate and at the end, and leaves the fastest candidate in the
+ * configuration. The binning tile stays 16 (gaussian-renderer.ts:366): it
+ * decides which fringe pixels (power between the 3-sigma radius and
+ * ln(255 a)) a splat is composited into; the raster tile only regroups the
+ * same per-pixel work, so every candidate renders the same image.
  * out: up to *count entries (NULL to query); *count receives the number of
  * candidates. */
 sv_status sv_autotune(sv_ctx *ctx, const int *views, int nviews, int steps,

@@ -100,6 +100,10 @@ struct sv_ctx {
   size_t d_views_cap = 0;
   uint32_t tiles_total = 0;
   uint32_t *tile_view = nullptr, *ranges = nullptr;
+  // 2x2-tile blocks of the batch (32-px raster tiles): count, block -> slot
+  uint32_t blocks_total = 0;
+  uint32_t *block_view = nullptr;
+  size_t blocks_cap = 0;
   float *l1_partial = nullptr;
   float *tile_l1 = nullptr;  // densify scoring, per global tile
   uint64_t *tile_cut = nullptr;  // per global tile: first unreached entry's (depth_q, id)
@@ -375,7 +379,7 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
   if (!same) {
     c->slot_views.assign(vs, vs + nv);
     c->h_slots.resize(nv);
-    uint32_t tb = 0, sb = 0;
+    uint32_t tb = 0, sb = 0, bb = 0;
     c->max_tiles_x = c->max_tiles_y = 0;
     for (int s = 0; s < nv; ++s) {
       const HostView &hv = c->views[vs[s]];
@@ -395,6 +399,9 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       d.pix_off = hv.pix_off;
       d.super_x = (hv.tiles_x + BIN_S - 1) / BIN_S;
       d.super_base = sb;
+      d.blk_x = (hv.tiles_x + 1) / 2;
+      d.blk_base = bb;
+      bb += d.blk_x * ((hv.tiles_y + 1) / 2);
       tb += hv.n_tiles;
       sb += d.super_x * ((hv.tiles_y + BIN_S - 1) / BIN_S);
       c->max_tiles_x = std::max(c->max_tiles_x, hv.tiles_x);
@@ -402,6 +409,13 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     }
     c->tiles_total = tb;
     c->super_total = sb;
+    c->blocks_total = bb;
+    if (c->blocks_cap < (size_t)bb + 1) {
+      dfree(c->block_view);
+      c->blocks_cap = 0;
+      if (sv_status s = dalloc(c, &c->block_view, (size_t)bb + 1)) return s;
+      c->blocks_cap = (size_t)bb + 1;
+    }
     if (sv_status s = grow(c, &c->d_views, c->d_views_cap, (size_t)nv)) return s;
     HIP_OK(c, hipMemcpyAsync(c->d_views, c->h_slots.data(), nv * sizeof(DevView), hipMemcpyHostToDevice, c->st));
     if (c->tiles_cap < (size_t)tb + 1) {
@@ -445,6 +459,13 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
     for (int s = 0; s < nv; ++s)
       for (uint32_t t = 0; t < c->h_slots[s].n_tiles; ++t) tv[c->h_slots[s].tile_base + t] = (uint32_t)s;
     if (tb) HIP_OK(c, hipMemcpyAsync(c->tile_view, tv.data(), tb * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
+    std::vector<uint32_t> bv(bb);
+    for (int s = 0; s < nv; ++s) {
+      const DevView &d = c->h_slots[s];
+      const uint32_t nbk = d.blk_x * ((d.tiles_y + 1) / 2);
+      for (uint32_t t = 0; t < nbk; ++t) bv[d.blk_base + t] = (uint32_t)s;
+    }
+    if (bb) HIP_OK(c, hipMemcpyAsync(c->block_view, bv.data(), bb * sizeof(uint32_t), hipMemcpyHostToDevice, c->st));
     // the host vectors must outlive the async copies
     HIP_OK(c, hipStreamSynchronize(c->st));
   }
@@ -751,6 +772,8 @@ RasterArgs raster_args(sv_ctx *c) {
   a.tile_cut = c->tile_cut;
   a.l1_partial = c->l1_partial;
   a.work_counter = nullptr;
+  a.block_view = c->block_view;
+  a.n_blocks_total = c->blocks_total;
   return a;
 }
 
@@ -770,10 +793,11 @@ uint32_t raster_grid(sv_ctx *c, RasterArgs &a, bool backward) {
 
 sv_status enqueue_forward(sv_ctx *c) {
   RasterArgs a = raster_args(c);
-  const uint32_t grid = raster_grid(c, a, false);
+  // 8- and 32-px raster tiles launch a workgroup per quadrant / block
+  const uint32_t grid = c->cfg.raster_px == 16 ? raster_grid(c, a, false) : c->tiles_total;
   {
     Scoped t(c, PH_FORWARD);
-    launch_forward(c->st, a, c->cfg.batch, grid);
+    launch_forward(c->st, a, c->cfg.batch, grid, c->cfg.raster_px);
   }
   return check_launch(c, "forward");
 }
@@ -1062,7 +1086,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
                         c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
-                        c->chunk_super, c->ccount, c->d_work};
+                        c->chunk_super, c->ccount, c->d_work, c->block_view};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
   g.push_back(c->N);
   g.push_back(c->cap);
@@ -1071,6 +1095,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
   g.push_back(c->chunk_cap);
   g.push_back(c->binned ? 1u : 0u);
   g.push_back(c->tiles_total);
+  g.push_back(c->blocks_total);
   g.push_back(c->super_total);
   g.push_back(c->super_bits);
   g.push_back((uint64_t)c->max_super_passes);
@@ -1368,6 +1393,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->last);
   dfree(c->d_views);
   dfree(c->tile_view);
+  dfree(c->block_view);
   dfree(c->ranges);
   dfree(c->l1_partial);
   dfree(c->tile_l1);
@@ -1427,7 +1453,8 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->tile_size != 16) return fail(c, SV_E_ARG, "tile_size must be 16 (got %u)", k->tile_size);
   if (k->batch != 64 && k->batch != 128 && k->batch != 256)
     return fail(c, SV_E_ARG, "batch must be 64, 128 or 256 (got %u)", k->batch);
-  if (k->raster_px != 16) return fail(c, SV_E_ARG, "raster_px must be 16 (got %u)", k->raster_px);
+  if (k->raster_px != 8 && k->raster_px != 16 && k->raster_px != 32)
+    return fail(c, SV_E_ARG, "raster_px must be 8, 16 or 32 (got %u)", k->raster_px);
   if (k->sh_deg > 3) return fail(c, SV_E_ARG, "sh_deg must be <= 3");
   if (k->plain_wf_flags > 3) return fail(c, SV_E_ARG, "plain_wf_flags must be a combination of SV_WF_*");
   if (k->xcd_remap > 1) return fail(c, SV_E_ARG, "xcd_remap must be 0 or 1");
@@ -1852,7 +1879,7 @@ sv_status sv_render_camera_noalloc(sv_ctx *c, const sv_camera *cam, float *rgb, 
   a.bbase = bbase;
   a.nb = project_blocks(N);
   a.aux = aux;
-  launch_forward(st, a, c->cfg.batch, (uint32_t)T);
+  launch_forward(st, a, c->cfg.batch, (uint32_t)T, 16u);
   launch_interleave3(st, color, P, out);
   if (sv_status s = check_launch(c, "scratch render")) return s;
   StepMeta hm{};
@@ -2479,9 +2506,13 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
   }
   const uint32_t batches[3] = {64, 128, 256};
   std::vector<sv_tune_entry> cand;
-  for (uint32_t b : batches)
-    for (uint32_t p = 0; p < 2; ++p)
-      for (uint32_t x = 0; x < 2; ++x) cand.push_back(sv_tune_entry{b, p, x, 16, 0.0});
+  // raster tile 16 x batch x persistent x XCD mapping; the 8- and 32-px
+  // raster tiles (forward only) launch one workgroup per quadrant / block
+  const uint32_t rasters[3] = {8, 16, 32};
+  for (uint32_t r : rasters)
+    for (uint32_t b : batches)
+      for (uint32_t p = 0; p < (r == 16 ? 2u : 1u); ++p)
+        for (uint32_t x = 0; x < 2; ++x) cand.push_back(sv_tune_entry{b, p, x, r, 0.0});
   if (!out) {
     *count = (int)cand.size();
     return SV_OK;
@@ -2519,6 +2550,7 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
     c->cfg.batch = cand[i].batch;
     c->cfg.persistent = cand[i].persistent;
     c->cfg.xcd_remap = cand[i].xcd_remap;
+    c->cfg.raster_px = cand[i].tile_size;
     c->step_t = t0;
     if ((st = step_batched(c, vs, nv))) break;  // untimed: compiles, grows buffers, captures
     if ((st = copy_all(false))) break;
@@ -2550,6 +2582,7 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
   c->cfg.batch = cand[best].batch;
   c->cfg.persistent = cand[best].persistent;
   c->cfg.xcd_remap = cand[best].xcd_remap;
+  c->cfg.raster_px = cand[best].tile_size;
   const int n = std::min(*count, (int)cand.size());
   for (int i = 0; i < n; ++i) out[i] = cand[i];
   *count = (int)cand.size();

@@ -29,7 +29,9 @@ constexpr uint32_t NO_LAST = 0xffffffffu;
 // Wavefront fixed-point backward flags (same values as SV_WF_* in sv_api.h).
 constexpr uint32_t WF_REFERENCE_ORDER = 0x1u;
 constexpr uint32_t WF_SKIP_098 = 0x2u;
-constexpr int REC9 = 9;  // per-(splat, tile) adjoint record: rgb3, logit, conic3, mean2
+constexpr int REC9 = 9;  // per-(splat, tile) adjoint record: sum res alpha T (rgb), dL/dlogit, and the
+                         // moments of -dL/dpower (m_xx, m_xy, m_yy, m_x, m_y); the record gather turns the
+                         // moments into dL/dconic (3) and dL/dmean (2)
 
 // Per-view constants resident in device memory.
 struct DevView {
@@ -43,6 +45,8 @@ struct DevView {
   uint32_t super_base; // first global supertile id of this view (binned build)
   uint32_t super_x;    // supertiles per row
   uint64_t pix_off;    // first pixel of this view in the pixel arrays
+  uint32_t blk_base;   // first global 2x2-tile block of this view (32-px raster tiles)
+  uint32_t blk_x;      // blocks per row
 };
 
 // Binned tile build: supertiles of BIN_S x BIN_S tiles; a supertile pair

@@ -161,13 +161,19 @@ struct RasterArgs {
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent launch: NUM_WORK_QUEUES counters,
                           // WORK_QUEUE_STRIDE words apart (null: one block per tile)
+  // 32-px raster tiles: blocks of 2 x 2 tiles, global block -> slot
+  const uint32_t *block_view;
+  uint32_t n_blocks_total;
 };
 constexpr uint32_t NUM_WORK_QUEUES = 8;     // one per XCD
 constexpr uint32_t WORK_QUEUE_STRIDE = 32;  // words between counters (a 128-byte line each)
 // Workgroups of the forward / backward raster kernel (LDS batch `batch`)
 // resident on the whole device at once: the persistent grid. 0 if unknown.
 uint32_t raster_resident_blocks(bool backward, uint32_t batch);
-void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
+// raster_px: 16 = one wave per 16x16 tile (grid_blocks: tiles or the
+// persistent grid); 8 = one wave per 8x8 quadrant; 32 = four waves per 2x2
+// block of tiles. The lists are the 16-px tiles' in every case (same images).
+void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks, uint32_t raster_px);
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
 // backward_tiles_local (backward-tiles-local.wgsl:185-382) for one slot:
 // i32 sums per (intersection, slot of 54) in sorted order; inv[emit

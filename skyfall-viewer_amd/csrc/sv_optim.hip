@@ -301,6 +301,18 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
         }
       }
     }
+    // the records hold moments of -dL/dpower over pixel offsets (m_xx,
+    // m_xy, m_yy, m_x, m_y); with this (view, Gaussian)'s conic (a, b, c):
+    // dL/dconic = -(m_xx / 2, m_xy, m_yy / 2), dL/dmean = (a m_x + b m_y,
+    // b m_x + c m_y)
+    const float4 q0 = a.rec[vg * 4];
+    const float cc = a.rec[vg * 4 + 1].x;
+    const float mx = r[7], my = r[8];
+    r[4] = -0.5f * r[4];
+    r[5] = -r[5];
+    r[6] = -0.5f * r[6];
+    r[7] = fmaf(q0.z, mx, q0.w * my);
+    r[8] = fmaf(q0.w, mx, cc * my);
   }
 #pragma unroll
   for (int q = 0; q < REC9; ++q) a.vrec[(size_t)q * M + vg] = r[q];

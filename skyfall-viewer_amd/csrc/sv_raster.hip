@@ -154,17 +154,6 @@ __device__ __forceinline__ float opaque_zero() {
   return z;
 }
 
-// a / b within about one ulp, for b in [1e-4, 1] (the backward's
-// T / max(1 - alpha, 1e-4)): the hardware reciprocal refined by one Newton
-// step, times a. The transmittance it rebuilds feeds only gradient sums
-// (which pairs a pixel takes is decided by alpha alone), so the division
-// need not be the correctly rounded one.
-__device__ __forceinline__ float div_approx(float a, float b) {
-  float r = __builtin_amdgcn_rcpf(b);
-  r = fmaf(fmaf(-b, r, 1.0f), r, r);
-  return a * r;
-}
-
 __device__ __forceinline__ float sign_ref(float x) {
   return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
 }
@@ -294,10 +283,10 @@ __device__ __forceinline__ uint32_t quad_mask(const float4 &q0, float c, const f
 
 template <int B>
 __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
-                                               uint32_t cnt, const QuadBounds &qb,
+                                               uint32_t cnt, const QuadBounds &qb, uint32_t wq,
                                                float4 *s_q0, float4 *s_q1, float4 *s_q2, uint32_t *s_k,
                                                uint32_t *s_p, uint32_t slot, uint32_t tx, uint32_t ty) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t kept = 0;
   for (uint32_t r = 0; r < cnt; r += WT) {
@@ -312,7 +301,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       q1 = q[1];
       q2 = q[2];
       qm = quad_mask(q0, q1.x, q2, qb);
-      touch = qm != 0;
+      touch = (qm & wq) != 0;
       if (s_p) {
         q3 = q[3];
         const uint32_t p = emit_pos(q3, a.bbase[(size_t)slot * a.nb + (id >> 8)], tx, ty);
@@ -373,6 +362,21 @@ __device__ __forceinline__ void composite(float as, float r, float g, float b, u
   last = fin ? idx : last;
 }
 
+// The LDS batch of one wave and its synchronisation: a wave's LDS accesses
+// are processed in issue order, so one wave needs only a fence that keeps
+// the compiler from moving them across (and waits for them); workgroups of
+// several independent waves use it instead of a workgroup barrier.
+template <int B>
+struct FwdLds {
+  float4 q0[B], q1[B], q2[B];
+  uint32_t k[B];
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // ---------------------------------------------------------------------
 // training_tiled_forward (training-forward-tiles.wgsl:107-215). Per pixel
 // (centre +0.5) over the tile list front to back: power = 0.5(a dx^2 +
@@ -381,116 +385,156 @@ __device__ __forceinline__ void composite(float as, float r, float g, float b, u
 // stop when 1 - A < 1e-4 after the add. last = last visited index (the
 // saturating entry, else the end of the list). Output C + (1 - A) bg.
 // Empty tile: bg, A = 0, last = NO_LAST.
+//
+// One wave renders the quadrants `wq` (bit h + 2 v) of the 16x16 tile tg
+// (the binning tile: its list is what every pixel composites, in order).
+// The raster tile is a launch choice only: 16 px = one wave, all four
+// quadrants; 8 px = one wave per quadrant; 32 px = a workgroup of four
+// waves, one 16x16 tile each. Per pixel the operations and their order are
+// the same in all three, so the images are identical bit for bit.
 // ---------------------------------------------------------------------
+template <int B>
+__device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, uint32_t wq, FwdLds<B> &L) {
+  const TileGeom t = tile_geom(a, tg);
+  const DevView &V = *t.V;
+  const size_t P = (size_t)V.W * V.H;
+  float *col = a.color + 3 * V.pix_off;
+  const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
+  if (start >= end) {
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      const uint32_t px = pix_x(t, j), py = pix_y(t, j);
+      if (((wq >> j) & 1u) && px < V.W && py < V.H) {
+        const size_t lp = (size_t)py * V.W + px;
+        col[lp] = a.cfg.bg[0];
+        col[P + lp] = a.cfg.bg[1];
+        col[2 * P + lp] = a.cfg.bg[2];
+        a.alpha[V.pix_off + lp] = 0.0f;
+        a.last[V.pix_off + lp] = NO_LAST;
+      }
+    }
+    return;
+  }
+  const size_t vbase = (size_t)t.s * a.N;
+  const QuadBounds qb = quad_bounds(t, V);
+  float c0[PPL], c1[PPL], c2[PPL], A[PPL], fxh[2], fyh[2];
+  uint32_t last[PPL];
+  bool done[PPL];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    fxh[h] = (float)pix_x(t, h) + 0.5f;
+    fyh[h] = (float)pix_y(t, 2 * h) + 0.5f;
+  }
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    c0[j] = c1[j] = c2[j] = A[j] = 0.0f;
+    last[j] = end - 1u;
+    done[j] = !(((wq >> j) & 1u) && pix_x(t, j) < V.W && pix_y(t, j) < V.H);
+  }
+  // the quadrant bits of an entry's word that this wave renders
+  const uint32_t wmask = wq << 16;
+  for (uint32_t b0 = start; b0 < end; b0 += B) {
+    const uint32_t cnt = min((uint32_t)B, end - b0);
+    const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, wq, L.q0, L.q1, L.q2, L.k, nullptr, 0, 0, 0);
+    wave_sync();
+    // Two entries per iteration: their alphas are independent (only the
+    // compositing that follows is ordered), so the two exp chains
+    // interleave. Per pixel the same operations run in the same order.
+    uint32_t e = 0;
+    for (; e + 1u < kept; e += 2u) {
+      // same words on every lane: the quadrant tests below are uniform
+      const uint32_t ka = __builtin_amdgcn_readfirstlane(L.k[e]) & (wmask | 0xFFFFu);
+      const uint32_t kb = __builtin_amdgcn_readfirstlane(L.k[e + 1u]) & (wmask | 0xFFFFu);
+      const float4 a0 = L.q0[e], a1 = L.q1[e], a2 = L.q2[e];
+      const float4 b0r = L.q0[e + 1u], b1 = L.q1[e + 1u], b2 = L.q2[e + 1u];
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        const bool ua = (ka >> (16 + j)) & 1u, ub = (kb >> (16 + j)) & 1u;
+        if (!(ua || ub)) continue;
+        if (done[j]) continue;
+        float asa, asb;
+        const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa);
+        const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, &asb);
+        if (va) composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+        if (vb && !done[j])
+          composite(asb, b1.z, b1.w, b2.x, b0 + (kb & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+      }
+      if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
+    }
+    if (e + 1u == kept) {
+      const uint32_t ka = __builtin_amdgcn_readfirstlane(L.k[e]) & (wmask | 0xFFFFu);
+      const float4 a0 = L.q0[e], a1 = L.q1[e], a2 = L.q2[e];
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        if (!((ka >> (16 + j)) & 1u)) continue;
+        if (done[j]) continue;
+        float asa;
+        if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa)) continue;
+        composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+      }
+    }
+    // a wave vote, then the sync that protects the LDS batch
+    const bool all_done = __all((done[0] && done[1]) && (done[2] && done[3]));
+    wave_sync();
+    if (all_done) break;
+  }
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const uint32_t px = pix_x(t, j), py = pix_y(t, j);
+    if (((wq >> j) & 1u) && px < V.W && py < V.H) {
+      const size_t lp = (size_t)py * V.W + px;
+      const float T = 1.0f - A[j];
+      col[lp] = c0[j] + T * a.cfg.bg[0];
+      col[P + lp] = c1[j] + T * a.cfg.bg[1];
+      col[2 * P + lp] = c2[j] + T * a.cfg.bg[2];
+      a.alpha[V.pix_off + lp] = A[j];
+      a.last[V.pix_off + lp] = last[j];
+    }
+  }
+}
+
+// 16-px raster tiles: one wave per tile (work item = global tile; the
+// persistent launch pulls them from the work queues).
 template <int B>
 __global__ __launch_bounds__(WT, 7) void k_forward(RasterArgs ain) {
   RasterArgs a = ain;
   if (!resolve_meta(a)) return;
-  __shared__ float4 s_q0[B];
-  __shared__ float4 s_q1[B];
-  __shared__ float4 s_q2[B];
-  __shared__ uint32_t s_k[B];
+  __shared__ FwdLds<B> L;
   uint32_t tg = first_tile(a);
   while (tg < a.n_tiles_total) {
-    const TileGeom t = tile_geom(a, tg);
-    const DevView &V = *t.V;
-    const size_t P = (size_t)V.W * V.H;
-    float *col = a.color + 3 * V.pix_off;
-    const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
-    if (start >= end) {
-#pragma unroll
-      for (int j = 0; j < PPL; ++j) {
-        const uint32_t px = pix_x(t, j), py = pix_y(t, j);
-        if (px < V.W && py < V.H) {
-          const size_t lp = (size_t)py * V.W + px;
-          col[lp] = a.cfg.bg[0];
-          col[P + lp] = a.cfg.bg[1];
-          col[2 * P + lp] = a.cfg.bg[2];
-          a.alpha[V.pix_off + lp] = 0.0f;
-          a.last[V.pix_off + lp] = NO_LAST;
-        }
-      }
-    } else {
-      const size_t vbase = (size_t)t.s * a.N;
-      const QuadBounds qb = quad_bounds(t, V);
-      float c0[PPL], c1[PPL], c2[PPL], A[PPL], fxh[2], fyh[2];
-      uint32_t last[PPL];
-      bool done[PPL];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        fxh[h] = (float)pix_x(t, h) + 0.5f;
-        fyh[h] = (float)pix_y(t, 2 * h) + 0.5f;
-      }
-#pragma unroll
-      for (int j = 0; j < PPL; ++j) {
-        c0[j] = c1[j] = c2[j] = A[j] = 0.0f;
-        last[j] = end - 1u;
-        done[j] = !(pix_x(t, j) < V.W && pix_y(t, j) < V.H);
-      }
-      for (uint32_t b0 = start; b0 < end; b0 += B) {
-        const uint32_t cnt = min((uint32_t)B, end - b0);
-        const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, s_q0, s_q1, s_q2, s_k, nullptr, 0, 0, 0);
-        __syncthreads();
-        // Two entries per iteration: their alphas are independent (only the
-        // compositing that follows is ordered), so the two exp chains
-        // interleave. Per pixel the same operations run in the same order.
-        uint32_t e = 0;
-        for (; e + 1u < kept; e += 2u) {
-          // same words on every lane: the quadrant tests below are uniform
-          const uint32_t ka = __builtin_amdgcn_readfirstlane(s_k[e]);
-          const uint32_t kb = __builtin_amdgcn_readfirstlane(s_k[e + 1u]);
-          const float4 a0 = s_q0[e], a1 = s_q1[e], a2 = s_q2[e];
-          const float4 b0r = s_q0[e + 1u], b1 = s_q1[e + 1u], b2 = s_q2[e + 1u];
-#pragma unroll
-          for (int j = 0; j < PPL; ++j) {
-            const bool ua = (ka >> (16 + j)) & 1u, ub = (kb >> (16 + j)) & 1u;
-            if (!(ua || ub)) continue;
-            if (done[j]) continue;
-            float asa, asb;
-            const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa);
-            const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, &asb);
-            if (va) composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
-            if (vb && !done[j])
-              composite(asb, b1.z, b1.w, b2.x, b0 + (kb & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
-          }
-          if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
-        }
-        if (e + 1u == kept) {
-          const uint32_t ka = __builtin_amdgcn_readfirstlane(s_k[e]);
-          const float4 a0 = s_q0[e], a1 = s_q1[e], a2 = s_q2[e];
-#pragma unroll
-          for (int j = 0; j < PPL; ++j) {
-            if (!((ka >> (16 + j)) & 1u)) continue;
-            if (done[j]) continue;
-            float asa;
-            if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa)) continue;
-            composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
-          }
-        }
-        // the workgroup is one wave: a wave vote, then the barrier that
-        // protects the LDS batch
-        const bool all_done = __all((done[0] && done[1]) && (done[2] && done[3]));
-        __syncthreads();
-        if (all_done) break;
-      }
-#pragma unroll
-      for (int j = 0; j < PPL; ++j) {
-        const uint32_t px = pix_x(t, j), py = pix_y(t, j);
-        if (px < V.W && py < V.H) {
-          const size_t lp = (size_t)py * V.W + px;
-          const float T = 1.0f - A[j];
-          col[lp] = c0[j] + T * a.cfg.bg[0];
-          col[P + lp] = c1[j] + T * a.cfg.bg[1];
-          col[2 * P + lp] = c2[j] + T * a.cfg.bg[2];
-          a.alpha[V.pix_off + lp] = A[j];
-          a.last[V.pix_off + lp] = last[j];
-        }
-      }
-    }
+    forward_tile<B>(a, tg, 0xFu, L);
     if (!next_tile(a, tg)) break;
   }
 }
 
+// 8-px raster tiles: one wave per quadrant (work item = 4 global tile + q).
+template <int B>
+__global__ __launch_bounds__(WT, 7) void k_forward8(RasterArgs ain) {
+  RasterArgs a = ain;
+  if (!resolve_meta(a)) return;
+  __shared__ FwdLds<B> L;
+  const uint32_t n = 4u * a.n_tiles_total;
+  const uint32_t item = a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (item >= n) return;
+  forward_tile<B>(a, item >> 2, 1u << (item & 3u), L);
+}
+
+// 32-px raster tiles: a workgroup of four waves, wave w the 16-px tile
+// (2 bx + (w & 1), 2 by + (w >> 1)) of block (bx, by) of its view.
+template <int B>
+__global__ __launch_bounds__(4 * WT, 7) void k_forward32(RasterArgs ain) {
+  RasterArgs a = ain;
+  if (!resolve_meta(a)) return;
+  __shared__ FwdLds<B> L[4];
+  const uint32_t blk = a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (blk >= a.n_blocks_total) return;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const DevView &V = a.views[a.block_view[blk]];
+  const uint32_t lb = blk - V.blk_base;
+  const uint32_t tx = 2u * (lb % V.blk_x) + (w & 1u), ty = 2u * (lb / V.blk_x) + (w >> 1);
+  if (tx >= V.tiles_x || ty >= V.tiles_y) return;
+  forward_tile<B>(a, V.tile_base + ty * V.tiles_x + tx, 0xFu, L[w]);
+}
 
 // ---------------------------------------------------------------------
 // Adjoint backward, order-correct. Math per (pixel, splat) is that of
@@ -499,7 +543,10 @@ __global__ __launch_bounds__(WT, 7) void k_forward(RasterArgs ain) {
 // alpha, 1e-4), no alpha > 0.98 skip. Instead of per-pixel float CAS
 // atomics into 54 per-Gaussian slots, the wave sums the 9 screen-space
 // quantities of one splat (sum of res * alpha * T_before per channel,
-// dL/dlogit, dL/dconic abc, dL/dmean xy) over the tile's 256 pixels (in
+// dL/dlogit, and the moments of q = -dL/dpower over the pixel offsets
+// (dx, dy): sums of q dx^2, q dx dy, q dy^2, q dx, q dy, from which the
+// record gather forms dL/dconic abc and dL/dmean xy) over the tile's 256
+// pixels (in
 // registers over each lane's 4 pixels, then a fixed butterfly
 // reduce-scatter over the wave for eight of them and a DPP tree for the
 // ninth) and writes
@@ -594,7 +641,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        const uint32_t kept = load_batch<B>(a, vbase, lo, cnt, qb, s_q0, s_q1, s_q2, s_k, s_p, t.s, t.tx, t.ty);
+        const uint32_t kept = load_batch<B>(a, vbase, lo, cnt, qb, 0xFu, s_q0, s_q1, s_q2, s_k, s_p, t.s, t.tx, t.ty);
         __syncthreads();
         for (int e = (int)kept - 1; e >= 0; --e) {
           // same word on every lane: the quadrant tests below are uniform
@@ -624,7 +671,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               if (!take) continue;
               hit = true;
               const float oma = fmaxf(1.0f - as, 0.0001f);
-              const float Tb = div_approx(T[j], oma);
+              const float Tb = T[j] * __builtin_amdgcn_rcpf(oma);
               // the gradient terms in fused multiply-adds (they feed only
               // sums; the alpha and T chain above stays the forward's)
               const float contrib = as * Tb;
@@ -634,13 +681,17 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               const float fg = fmaf(r2[j], x2.x, fmaf(r1[j], x1.w, r0[j] * x1.z));
               const float dL_dalpha = (fg - sacc[j]) * Tb;
               v3 = fmaf(dL_dalpha * as, 1.0f - op, v3);
+              // second and first moments of -dL/dpower over the pixel
+              // offsets; the conic gradient is -(1/2 m_xx, m_xy, 1/2 m_yy) and
+              // the mean gradient (a m_x + b m_y, b m_x + c m_y), applied per
+              // (view, Gaussian) when the records are gathered
               const float nds = dL_dalpha * (op * gauss);  // -dL/dpower
-              const float hds = 0.5f * nds;
-              v4 = fmaf(-hds * dx, dx, v4);
-              v5 = fmaf(-nds * dx, dy, v5);
-              v6 = fmaf(-hds * dy, dy, v6);
-              v7 = fmaf(nds, fmaf(x0.z, dx, x0.w * dy), v7);
-              v8 = fmaf(nds, fmaf(x0.w, dx, x1.x * dy), v8);
+              const float ux = nds * dx, uy = nds * dy;
+              v4 = fmaf(ux, dx, v4);
+              v5 = fmaf(ux, dy, v5);
+              v6 = fmaf(uy, dy, v6);
+              v7 = v7 + ux;
+              v8 = v8 + uy;
               sacc[j] = fmaf(as, fg, (1.0f - as) * sacc[j]);
               T[j] = Tb;
             }
@@ -823,11 +874,23 @@ uint32_t raster_resident_blocks(bool backward, uint32_t batch) {
   return (uint32_t)per_cu * (uint32_t)cus;
 }
 
-void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid) {
+void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid, uint32_t raster_px) {
   if (a.n_tiles_total == 0 || grid == 0) return;
-  if (batch <= 64) hipLaunchKernelGGL(k_forward<64>, dim3(grid), dim3(WT), 0, st, a);
-  else if (batch <= 128) hipLaunchKernelGGL(k_forward<128>, dim3(grid), dim3(WT), 0, st, a);
-  else hipLaunchKernelGGL(k_forward<256>, dim3(grid), dim3(WT), 0, st, a);
+  if (raster_px == 8) {
+    const uint32_t g8 = 4u * a.n_tiles_total;
+    if (batch <= 64) hipLaunchKernelGGL(k_forward8<64>, dim3(g8), dim3(WT), 0, st, a);
+    else if (batch <= 128) hipLaunchKernelGGL(k_forward8<128>, dim3(g8), dim3(WT), 0, st, a);
+    else hipLaunchKernelGGL(k_forward8<256>, dim3(g8), dim3(WT), 0, st, a);
+  } else if (raster_px == 32) {
+    if (a.n_blocks_total == 0) return;
+    if (batch <= 64) hipLaunchKernelGGL(k_forward32<64>, dim3(a.n_blocks_total), dim3(4 * WT), 0, st, a);
+    else if (batch <= 128) hipLaunchKernelGGL(k_forward32<128>, dim3(a.n_blocks_total), dim3(4 * WT), 0, st, a);
+    else hipLaunchKernelGGL(k_forward32<256>, dim3(a.n_blocks_total), dim3(4 * WT), 0, st, a);
+  } else {
+    if (batch <= 64) hipLaunchKernelGGL(k_forward<64>, dim3(grid), dim3(WT), 0, st, a);
+    else if (batch <= 128) hipLaunchKernelGGL(k_forward<128>, dim3(grid), dim3(WT), 0, st, a);
+    else hipLaunchKernelGGL(k_forward<256>, dim3(grid), dim3(WT), 0, st, a);
+  }
 }
 
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid) {

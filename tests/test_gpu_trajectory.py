@@ -22,10 +22,11 @@ where the gradients agree bit for bit. Hence three checks:
    gradient bit for bit (10 steps, both views per step, with and without
    the schedules);
 3. batched adjoint mode, free-running with the fused Adam of the product
-   path (graph launch): after 10 steps the parameters follow the oracle's
-   own 10-step run, for each tensor 97 % of the elements within 1e-4 of
-   max |ref| and none further apart than two runs can drift under Adam's
-   step bound (Cauchy-Schwarz on m and v: |m_hat| / sqrt(v_hat) <=
+   path (graph launch): after 10 steps every parameter is within 1e-4 of
+   max |ref| of the oracle's own 10-step run (on this scene no gradient sits
+   near enough to zero for the normalised step to flip; measured largest
+   deviation 2.4e-6), and none is further off than two runs can drift under
+   Adam's step bound (Cauchy-Schwarz on m and v: |m_hat| / sqrt(v_hat) <=
    (1 - b1) / sqrt(1 - b2) * sqrt(sum_k<t (b1^2 / b2)^k) * sqrt(1 - b2^t) /
    (1 - b1^t) at step t).
 """
@@ -140,5 +141,5 @@ def test_batched_free_running(sv, orc, scene):
         frac = float(np.mean(d > 1e-4 * scale))
         print(f"[traj] batched free-running {k}: beyond 1e-4 of max {100 * frac:.3f} %, "
               f"median {np.median(d) / scale:.2e}, max {d.max():.2e}", flush=True)
-        assert frac <= 0.03, k
+        assert frac == 0.0 and d.max() <= 1e-4 * scale, k
         assert d.max() <= bound, k
