@@ -150,7 +150,9 @@ sv_status sv_get_config(const sv_ctx *ctx, sv_config *cfg);
 /* --- parameters ------------------------------------------------------- */
 /* reloadPointCloud + training_init_params (gaussian-renderer.ts:2092-2126,
  * training-params.wgsl:35-79): upload n Gaussians; clears moments, grads,
- * statistics and the step counter. Any pointer may not be NULL. */
+ * statistics and the step counter. Any pointer may not be NULL. A NaN or
+ * infinite value returns SV_E_ARG and changes nothing (so do non-finite
+ * moments, targets and residuals below). */
 sv_status sv_set_gaussians(sv_ctx *ctx, size_t n, const float *pos,
                            const float *logscale, const float *quat,
                            const float *logit, const float *sh);

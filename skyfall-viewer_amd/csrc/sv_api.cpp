@@ -933,6 +933,26 @@ sv_status download_planar(sv_ctx *c, const float *src, size_t cap, float *dst, s
   return SV_OK;
 }
 
+// The first non-finite value of an input array (n * comps floats), or
+// SV_OK: NaN and infinities are refused at the boundary, before any kernel
+// sees them (the reference has no such check; a NaN would spread through
+// every later step).
+sv_status check_finite(sv_ctx *c, const char *what, const float *v, size_t count) {
+  if (!v) return SV_OK;
+  for (size_t i = 0; i < count; ++i)
+    if (!std::isfinite(v[i])) return fail(c, SV_E_ARG, "%s[%zu] is not finite", what, i);
+  return SV_OK;
+}
+
+sv_status check_params_finite(sv_ctx *c, size_t n, const float *pos, const float *ls, const float *q,
+                              const float *lg, const float *sh) {
+  if (sv_status s = check_finite(c, "pos", pos, 3 * n)) return s;
+  if (sv_status s = check_finite(c, "logscale", ls, 3 * n)) return s;
+  if (sv_status s = check_finite(c, "quat", q, 4 * n)) return s;
+  if (sv_status s = check_finite(c, "logit", lg, n)) return s;
+  return check_finite(c, "sh", sh, 48 * n);
+}
+
 sv_status set_params(sv_ctx *c, float *base, const float *pos, const float *ls, const float *q,
                      const float *lg, const float *sh) {
   if (sv_status s = upload_planar(c, base, c->cap, pos, c->N, 3, P_POS)) return s;
@@ -1476,6 +1496,7 @@ sv_status sv_set_gaussians(sv_ctx *c, size_t n, const float *pos, const float *l
   if (!pos || !ls || !q || !lg || !sh) return fail(c, SV_E_ARG, "null parameter pointer");
   if (n > c->cap) return fail(c, SV_E_CAPACITY, "%zu Gaussians exceed capacity %zu", n, c->cap);
   if (n > 0xffffffffull) return fail(c, SV_E_CAPACITY, "too many Gaussians");
+  if (sv_status s = check_params_finite(c, n, pos, ls, q, lg, sh)) return s;
   HIP_OK(c, hipStreamSynchronize(c->st));
   drop_graph(c);
   c->N = (uint32_t)n;
@@ -1505,6 +1526,7 @@ sv_status sv_set_moments(sv_ctx *c, int which, const float *pos, const float *ls
                          const float *lg, const float *sh) {
   if (sv_status s = check_ctx(c)) return s;
   if (which != 1 && which != 2) return fail(c, SV_E_ARG, "which must be 1 or 2");
+  if (sv_status s = check_params_finite(c, c->N, pos, ls, q, lg, sh)) return s;
   HIP_OK(c, hipStreamSynchronize(c->st));
   return set_params(c, which == 1 ? c->M1 : c->M2, pos, ls, q, lg, sh);
 }
@@ -1572,6 +1594,7 @@ sv_status sv_set_target(sv_ctx *c, int v, const float *rgb) {
   if (sv_status s = check_ctx(c)) return s;
   if (sv_status s = check_view(c, v)) return s;
   if (!rgb) return fail(c, SV_E_ARG, "null target");
+  if (sv_status s = check_finite(c, "target", rgb, 3 * (size_t)c->views[v].W * c->views[v].H)) return s;
   if (sv_status s = write_pixels3(c, c->target, v, rgb)) return s;
   c->views[v].has_target = true;
   return SV_OK;
@@ -1610,6 +1633,7 @@ sv_status sv_set_residual(sv_ctx *c, int v, const float *rgb) {
   if (sv_status s = check_ctx(c)) return s;
   if (sv_status s = check_view(c, v)) return s;
   if (!rgb) return fail(c, SV_E_ARG, "null residual");
+  if (sv_status s = check_finite(c, "residual", rgb, 3 * (size_t)c->views[v].W * c->views[v].H)) return s;
   if (sv_status s = write_pixels3(c, c->residual, v, rgb)) return s;
   c->residual_ready[v] = 1;
   return SV_OK;

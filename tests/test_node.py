@@ -133,3 +133,19 @@ def test_js_renderer_and_function_match_ctypes(sv, orc, tmp_path):
         off += m
         assert np.array_equal(got.view(np.uint32), a.ravel().view(np.uint32)), name
     assert off == blob.size
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_train_stops_mid_loop():
+    """train(1000) stops at the iteration after stopTraining(); a second
+    train() while the first runs warns and returns."""
+    script = os.path.join(ROOT, "tests", "node", "stop_train.js")
+    print("[node] stop mid-train", flush=True)
+    res = subprocess.run([NODE, script], capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    st = json.loads(res.stdout.strip().splitlines()[-1])
+    print(f"[node] {st}", flush=True)
+    assert st["during_training"] and not st["training"]
+    assert st["warned"] >= 1
+    assert 2 <= st["iterations"] < 1000 and st["target"] == 1000
