@@ -53,7 +53,17 @@ def configure(ctx: sv.Context, args, keep_grads=0):
     c.keep_grads = keep_grads
     c.xcd_remap = args.xcd_remap
     c.binned_build = args.binned
+    c.geom_flags = geom_flags(args)
     ctx.set_config(c)
+
+
+def geom_flags(args) -> int:
+    """The growth run trains with the mean's y in the forward's convention
+    unless asked otherwise; every other workload runs the geometry backward
+    as the reference's shader writes it."""
+    if args.geom_flags is not None:
+        return args.geom_flags
+    return sv.GEOM_MEAN_Y_FORWARD if args.workload == "growth4m" else 0
 
 
 def kernel_bytes(phase: str, N: int, V: int, I: int, P: int, key_bytes: int, passes: int) -> float:
@@ -223,8 +233,19 @@ def run_growth4m(args):
     max(1, floor(0.1 N)) splats, the best-scored candidates first (the
     reference's cap, gaussian-renderer.ts:3297-3306), and never more than
     the 4M budget; with 0 every candidate is taken up to the budget (in
-    index order). The timed region holds every step and every prune /
-    densify event."""
+    index order). With --growth-params reference (default) the run trains
+    with the reference trainer's defaults: learning rates means 1.6e-4
+    (decayed by the position schedule), SH 2.5e-3 every 16 steps, opacity
+    0.025, scale 0.005, rotation 1e-3 (gaussian-renderer.ts:86-96,
+    284-298, 2476-2484) and loss 0.8 L1 + 0.2 D-SSIM (:506-522); with
+    "unit" it keeps the parity configs' 1e-3 rates, sh interval 1 and
+    loss (0.8 L2, 0.2 L1), under which added splats are never trained
+    away and the image error grows. The timed region holds every step and
+    every prune / densify event. --geom-flags 1 (the default here) gives the
+    geometry backward's mean path the forward's y convention: with the
+    shader's +f_y (:264-266, --geom-flags 0) every position-y update moves
+    against the rendered image's dependence, so the positions drift and the
+    image error grows over the run."""
     N0, V, W, H, NMAX, EVERY = 1_000_000, 16, 1920, 1080, 4_000_000, 100
     g = sv.scene_gaussians(0, N0, 1.0)
     cams = sv.ring_cameras(V, 3.0, W, H)
@@ -233,6 +254,9 @@ def run_growth4m(args):
     c = sv.Context(capacity=NMAX)
     configure(c, args)
     c.set_config(use_graph=args.graph)
+    if args.growth_params == "reference":
+        c.set_config(lr_pos=0.00016, lr_sh=0.0025, lr_opacity=0.025, lr_scale=0.005, lr_rot=0.001,
+                     sh_update_interval=16, pos_lr_schedule=1, w_l2=0.0, w_l1=0.8, w_ssim=0.2)
     c.set_gaussians(tg)
     c.set_views(cams)
     targets = []
@@ -277,7 +301,13 @@ def run_growth4m(args):
         "config": {"workload": f"{N0} splats growing to <= {NMAX}, {V} views {W}x{H}, prune/densify every "
                                f"{EVERY} steps", "splats_start": N0, "splats_end": c.count, "views": V,
                    "width": W, "height": H, "growth_cap": "max(1, floor(0.1 N)) per event, best score first"
-                   if args.growth_cap else "none (4M budget only)"},
+                   if args.growth_cap else "none (4M budget only)",
+                   "geometry_backward": "mean y as the forward (SV_GEOM_MEAN_Y_FORWARD)"
+                   if geom_flags(args) & sv.GEOM_MEAN_Y_FORWARD else "the shader as written (+f_y mean path)",
+                   "training": "reference defaults: lr means 1.6e-4 scheduled, SH 2.5e-3 every 16 steps, opacity "
+                               "0.025, scale 0.005, rotation 1e-3; loss 0.8 L1 + 0.2 D-SSIM"
+                   if args.growth_params == "reference" else
+                   "unit: lr 1e-3 every group, SH every step; loss 0.8 L2 + 0.2 L1"},
         "intersections_per_s": isects / dt, "final_mean_l1": float(st.l1_sum) / (3.0 * P) if st else None,
         "events": events,
     }
@@ -454,6 +484,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--growth-cap", type=int, default=1,
                     help="growth4m: 1 = at most max(1, N // 10) new splats per event, best score first")
+    ap.add_argument("--growth-params", choices=("reference", "unit"), default="reference",
+                    help="growth4m: the reference trainer's default learning rates, schedules and loss, or the "
+                         "parity configs' 1e-3 rates")
+    ap.add_argument("--geom-flags", type=int, default=None,
+                    help="SV_GEOM_* flags of the geometry backward (default: 1 for growth4m, 0 otherwise)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the small256 and ring8_720p measurements reported beside the headline")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -87,6 +87,9 @@ typedef struct sv_config {
                                8x8-tile supertiles (sort of one pair per
                                splat and supertile) instead of sorting every
                                (tile, splat) pair; same lists and ranges */
+  uint32_t geom_flags;      /* SV_GEOM_* flags of the geometry backward; 0 =
+                               the reference's training_backward_geom as
+                               written */
 } sv_config;
 
 /* Step modes for sv_step. */
@@ -102,6 +105,16 @@ typedef struct sv_config {
 /* Wavefront fixed-point backward flags (backward-tiles-local.wgsl). */
 #define SV_WF_REFERENCE_ORDER 0x1u /* 64-entry batches front-to-back (:262) */
 #define SV_WF_SKIP_098 0x2u        /* skip alpha > 0.98 (:329-331) */
+
+/* Geometry backward flags (training-backward-geom.wgsl). */
+#define SV_GEOM_MEAN_Y_FORWARD 0x1u /* the mean path's y terms take the
+                                       forward's pixel convention, y = H - c_y
+                                       - f_y y / z (training-tiles.wgsl:149-151
+                                       with the projection of
+                                       training-camera.ts:39-51), instead of
+                                       the +f_y of :264-266, whose position-y
+                                       gradient has the opposite sign to the
+                                       forward's dependence */
 
 typedef struct sv_step_stats {
   uint64_t n_isect;     /* intersections sorted this step (all views) */

@@ -25,7 +25,7 @@ class OrcView(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tile_size", C.c_uint32),
                 ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32), ("sh_deg", C.c_uint32),
                 ("near_plane", C.c_float), ("far_plane", C.c_float), ("radius_clip", C.c_float),
-                ("eps2d", C.c_float), ("bg", C.c_float * 3)]
+                ("eps2d", C.c_float), ("bg", C.c_float * 3), ("geom_flags", C.c_uint32)]
 
 
 class OrcParams(C.Structure):
@@ -83,8 +83,9 @@ def lib():
 
 
 def view_params(w, h, sh_deg=3, tile=16, bg=(0.0, 0.0, 0.0), near=0.2, far=1e4, radius_clip=512.0,
-                eps2d=0.3) -> OrcView:
+                eps2d=0.3, geom_flags=0) -> OrcView:
     v = OrcView()
+    v.geom_flags = geom_flags
     v.width, v.height, v.tile_size = w, h, tile
     v.tiles_x, v.tiles_y = (w + tile - 1) // tile, (h + tile - 1) // tile
     v.sh_deg = sh_deg

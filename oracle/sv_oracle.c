@@ -959,8 +959,12 @@ void orc_backward_reduce(size_t n, const int32_t *fixed, const uint32_t *vals,
 /* a15: training_backward_geom, training-backward-geom.wgsl:36-299      */
 /* ------------------------------------------------------------------ */
 void orc_backward_geom(size_t n, const orc_params *p, const orc_camera *cam,
-                       orc_grads *g) {
+                       uint32_t geom_flags, orc_grads *g) {
   const float EPS_2D = 0.3f, LIM = 1.3f;
+  /* geom_flags bit 0: the mean path's y terms with the forward's pixel
+   * convention y = H - c_y - f_y y / z (training-tiles.wgsl:149-151), i.e.
+   * the :264-266 terms times -1 (exact) */
+  const float ysign = (geom_flags & 1u) ? -1.0f : 1.0f;
 #pragma omp parallel for schedule(static)
   for (size_t idx = 0; idx < n; ++idx) {
     const float *gg = &g->geom[5 * idx];
@@ -1083,8 +1087,8 @@ void orc_backward_geom(size_t n, const orc_params *p, const orc_camera *cam,
                           (((2.0f * fy) * tc[1]) * tz3_inv) * dJ12;
     /* STEP 7 (:264-274) */
     const float dtx_mean = (gmx * fx) * tz_inv;
-    const float dty_mean = (gmy * fy) * tz_inv;
-    const float dtz_mean = ((((-gmx) * fx) * t[0]) * tz2_inv) - (((gmy * fy) * t[1]) * tz2_inv);
+    const float dty_mean = ysign * ((gmy * fy) * tz_inv);
+    const float dtz_mean = ((((-gmx) * fx) * t[0]) * tz2_inv) - ysign * (((gmy * fy) * t[1]) * tz2_inv);
     const float dLdt[3] = {dtx_cov + dtx_mean, dty_cov + dty_mean, dtz_cov + dtz_mean};
     m3 Wt;
     m3_transpose(W, Wt);
@@ -1192,7 +1196,7 @@ int64_t orc_render_and_grad(size_t n, const orc_params *p, const orc_camera *cam
     orc_backward_legacy(n, p, o->proj, cam, v, o->offsets, o->vals, (size_t)m, o->residual,
                         o->alpha, o->last_ids, g);
   }
-  orc_backward_geom(n, p, cam, g);
+  orc_backward_geom(n, p, cam, v->geom_flags, g);
   return (int64_t)m;
 }
 

@@ -46,6 +46,9 @@ typedef struct orc_view {
   uint32_t sh_deg;
   float near_plane, far_plane, radius_clip, eps2d;
   float bg[3];
+  uint32_t geom_flags; /* 1: the geometry backward's mean path takes the
+                          forward's y convention (the library's
+                          SV_GEOM_MEAN_Y_FORWARD); 0: the shader as written */
 } orc_view;
 
 /* GaussProjection, common.wgsl:57-65 (48 bytes). */
@@ -119,7 +122,7 @@ void orc_backward_wavefront(const orc_params *p, const orc_proj *proj,
 void orc_backward_reduce(size_t n, const int32_t *tile_grads_fixed,
                          const uint32_t *vals, size_t m, orc_grads *g);
 void orc_backward_geom(size_t n, const orc_params *p, const orc_camera *cam,
-                       orc_grads *g);
+                       uint32_t geom_flags, orc_grads *g);
 void orc_adam(size_t count, float *param, float *m1, float *m2,
               const float *grad, float lr, float beta1, float beta2,
               float eps, float bc1_rcp, float bc2_sqrt_rcp);

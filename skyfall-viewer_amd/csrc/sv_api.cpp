@@ -345,6 +345,7 @@ DevConfig dev_config(const sv_config &k) {
   d.render_f16 = k.render_f16;
   d.importance_decay = k.importance_decay;
   d.xcd_remap = k.xcd_remap;
+  d.geom_flags = k.geom_flags;
   return d;
 }
 
@@ -1323,6 +1324,7 @@ sv_status sv_default_config(sv_config *o) {
   o->plain_wf_flags = 0;
   o->xcd_remap = 1;
   o->binned_build = 1;
+  o->geom_flags = 0;
   return SV_OK;
 }
 
@@ -1479,6 +1481,9 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->plain_wf_flags > 3) return fail(c, SV_E_ARG, "plain_wf_flags must be a combination of SV_WF_*");
   if (k->xcd_remap > 1) return fail(c, SV_E_ARG, "xcd_remap must be 0 or 1");
   if (k->binned_build > 1) return fail(c, SV_E_ARG, "binned_build must be 0 or 1");
+  static_assert(SV_GEOM_MEAN_Y_FORWARD == GEOM_MEAN_Y_FORWARD, "geometry flag values");
+  if (k->geom_flags & ~SV_GEOM_MEAN_Y_FORWARD)
+    return fail(c, SV_E_ARG, "geom_flags must be a combination of SV_GEOM_*");
   c->cfg = *k;
   return SV_OK;
 }

@@ -88,7 +88,9 @@ struct DevConfig {
   uint32_t render_f16;
   float importance_decay;
   uint32_t xcd_remap;
+  uint32_t geom_flags;  // SV_GEOM_* of the geometry backward
 };
+constexpr uint32_t GEOM_MEAN_Y_FORWARD = 0x1u;  // SV_GEOM_MEAN_Y_FORWARD
 
 // ---------------------------------------------------------------------
 // Constants of src/shaders/common.wgsl:1-18

@@ -47,9 +47,11 @@ __device__ __forceinline__ GeomPre geom_pre(const float prm[11]) {
 // 1.3 tan_fov, EWA covariance with EPS_2D = 0.3, 3DGS conic VJP, M = S R,
 // cov3D = M^T M, the quaternion formula on the normalised q, the mean path
 // with +fy, dL/dlog(sigma) = dL/dsigma * sigma). Returns false where the
-// shader returns early.
+// shader returns early. ysign = -1 gives the mean path's y terms the
+// forward's convention (pixel y = H - c_y - f_y y / z); +1 is the shader's
+// (the sign flips are exact, so +1 is the shader bit for bit).
 __device__ __forceinline__ bool geom_vjp(const DevView &V, const GeomPre &G, float gA, float gB,
-                                         float gC, float gmx, float gmy, float dpos[3],
+                                         float gC, float gmx, float gmy, float ysign, float dpos[3],
                                          float dls[3], float dq[4]) {
   const float conic_mag = (fabsf(gA) + fabsf(gB)) + fabsf(gC);
   const float mean_mag = fabsf(gmx) + fabsf(gmy);
@@ -178,8 +180,8 @@ __device__ __forceinline__ bool geom_vjp(const DevView &V, const GeomPre &G, flo
                         (((2.0f * fy) * tcy) * tz3_inv) * dJ12;
   // STEP 7
   const float dtx_mean = (gmx * fx) * tz_inv;
-  const float dty_mean = (gmy * fy) * tz_inv;
-  const float dtz_mean = ((((-gmx) * fx) * tx) * tz2_inv) - (((gmy * fy) * ty) * tz2_inv);
+  const float dty_mean = ysign * ((gmy * fy) * tz_inv);
+  const float dtz_mean = ((((-gmx) * fx) * tx) * tz2_inv) - ysign * (((gmy * fy) * ty) * tz2_inv);
   const float l0 = dtx_cov + dtx_mean, l1 = dty_cov + dty_mean, l2 = dtz_cov + dtz_mean;
   const M3 Wt = transpose(W);
 #pragma unroll
@@ -189,6 +191,10 @@ __device__ __forceinline__ bool geom_vjp(const DevView &V, const GeomPre &G, flo
   dls[1] = ds1 * G.sigma[1];
   dls[2] = ds2 * G.sigma[2];
   return true;
+}
+
+__device__ __forceinline__ float geom_ysign(const DevConfig &cfg) {
+  return (cfg.geom_flags & GEOM_MEAN_Y_FORWARD) ? -1.0f : 1.0f;
 }
 
 // accumulate_grads (densify-accum.wgsl:19-46).
@@ -377,7 +383,7 @@ __global__ __launch_bounds__(256) void k_vjp_geom(VjpArgs a) {
     const DevView &V = a.views[s];
     glogit = glogit + r[3];
     float dp[3], dl[3], dqq[4];
-    if (geom_vjp(V, pre, r[4], r[5], r[6], r[7], r[8], dp, dl, dqq)) {
+    if (geom_vjp(V, pre, r[4], r[5], r[6], r[7], r[8], geom_ysign(a.cfg), dp, dl, dqq)) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         gp[k] = gp[k] + dp[k];
@@ -643,7 +649,7 @@ __global__ __launch_bounds__(256) void k_geom(const float *__restrict__ P, size_
   }
   const GeomPre pre = geom_pre(prm);
   float dp[3], dl[3], dq[4];
-  if (!geom_vjp(*view, pre, r[0], r[1], r[2], r[3], r[4], dp, dl, dq)) return;
+  if (!geom_vjp(*view, pre, r[0], r[1], r[2], r[3], r[4], geom_ysign(cfg), dp, dl, dq)) return;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     G[(size_t)(P_POS + k) * cap + g] = G[(size_t)(P_POS + k) * cap + g] + dp[k];

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libsv.so")
 SV_OK, SV_E_ARG, SV_E_CAPACITY, SV_E_HIP, SV_E_STATE, SV_E_NOMEM = range(6)
 MODE_PLAIN, MODE_BATCHED, MODE_SEQUENTIAL = 0, 1, 2
 WF_REFERENCE_ORDER, WF_SKIP_098 = 0x1, 0x2
+GEOM_MEAN_Y_FORWARD = 0x1
 NO_LAST = 0xFFFFFFFF
 
 
@@ -38,6 +39,7 @@ class Config(C.Structure):
         ("render_f16", C.c_uint32), ("importance_decay", C.c_float),
         ("persistent", C.c_uint32), ("use_graph", C.c_uint32), ("keep_grads", C.c_uint32),
         ("plain_wf_flags", C.c_uint32), ("xcd_remap", C.c_uint32), ("binned_build", C.c_uint32),
+        ("geom_flags", C.c_uint32),
     ]
 
 

@@ -191,13 +191,14 @@ def _m3mul(A, B):
     return torch.einsum("nkj,nik->nij", A, B)
 
 
-def _geom_model_grads(g, cam, gup):
+def _geom_model_grads(g, cam, gup, ysign=1.0):
     """VJP of the model training-backward-geom.wgsl differentiates, by
     autograd: t moves with the position through transpose(W) (W = the view
     rotation read row-wise), cov2D = T^T cov3D^T T with T = W J and the
     frustum-clamped J, conic = (c, -b, a) / det, mean = (fx tx, fy ty) / tz,
     sigma = exp(log-scale), R from the normalised quaternion; the upstream
-    gradient of conic b enters twice (the shader's factor 2)."""
+    gradient of conic b enters twice (the shader's factor 2). ysign = -1
+    models the mean's y as the forward places it, H - c_y - fy ty / tz."""
     n = g["pos"].shape[0]
     view = np.array(cam.view, np.float64)
     Vm = view.reshape(4, 4).T  # math [row][col]
@@ -242,7 +243,7 @@ def _geom_model_grads(g, cam, gup):
     det = a * c - b * b
     conic = torch.stack([c / det, -b / det, a / det], 1)
     mx = fx * t[:, 0] / tz
-    my = fy * t[:, 1] / tz
+    my = ysign * fy * t[:, 1] / tz
     gu = torch.tensor(gup.astype(np.float64), dtype=F64)
     L = (conic[:, 0] * gu[:, 0] + 2.0 * conic[:, 1] * gu[:, 1] + conic[:, 2] * gu[:, 2]
          + mx * gu[:, 3] + my * gu[:, 4]).sum()
@@ -262,6 +263,55 @@ def test_geometry_backward_matches_autograd_of_shader_model(sv, orc):
         print(f"[pin] geometry backward {k} vs autograd of the shader model: {e:.2e}")
         assert e < 1e-4, k
     assert np.all(gg["pos"][~live] == 0)
+
+
+def test_geometry_mean_y_forward_matches_autograd(sv, orc):
+    """geom_flags bit 0 (the library's SV_GEOM_MEAN_Y_FORWARD): the mean's y
+    enters with the forward's sign; the covariance path is unchanged."""
+    g, cam, _, target = _scene(sv, orc, 4096, 64, 64, eye=(0.4, -0.3, 3.0))
+    view = orc.view_params(64, 64, sh_deg=3, geom_flags=1)
+    r = orc.render_and_grad(g, cam, view, target=target, w_l2=0.8, w_l1=0.2)
+    gg = r["grads"]
+    live = np.abs(gg["geom"]).sum(1) > 1e-10
+    ag = _geom_model_grads({k: v[live] for k, v in g.items()}, cam, gg["geom"][live], ysign=-1.0)
+    for k in ("pos", "logscale", "quat"):
+        e = close_rel(gg[k][live], ag[k], 1e-4)
+        print(f"[pin] geometry backward, mean y as the forward, {k} vs autograd: {e:.2e}")
+        assert e < 1e-4, k
+    r0 = orc.render_and_grad(g, cam, orc.view_params(64, 64, sh_deg=3), target=target, w_l2=0.8, w_l1=0.2)
+    for k in ("logscale", "quat", "logit", "sh", "geom"):
+        assert np.array_equal(r0["grads"][k], gg[k]), k
+
+
+def test_position_gradient_direction_against_fd(sv, orc):
+    """Central FD (h = 1e-4) of the FP64 forward for the positions of 40
+    Gaussians: with the mean's y in the forward's convention the gradient
+    points along the FD gradient (cosine >= 0.95); the shader as written
+    (+fy, A8) is reported, its y sign being opposite to the forward's."""
+    g, cam, _, target = _scene(sv, orc, 4096, 64, 64, eye=(0.4, -0.3, 3.0))
+    cos = {}
+    for flags in (0, 1):
+        view = orc.view_params(64, 64, sh_deg=3, geom_flags=flags)
+        r = orc.render_and_grad(g, cam, view, target=target, w_l2=0.8, w_l1=0.2)
+        live = np.flatnonzero(r["proj"]["tiles_count"] > 0)
+        pick = np.random.default_rng(3).choice(live, 40, replace=False)
+        g64 = {k: v.astype(np.float64) for k, v in g.items()}
+        a, fd = [], []
+        for i in pick:
+            for j in range(3):
+                gp = {k: v.copy() for k, v in g64.items()}
+                gm = {k: v.copy() for k, v in g64.items()}
+                gp["pos"][i, j] += 1e-4
+                gm["pos"][i, j] -= 1e-4
+                fd.append((orc.forward_loss_f64(gp, cam, view, target, 0.8, 0.2)
+                           - orc.forward_loss_f64(gm, cam, view, target, 0.8, 0.2)) / 2e-4)
+                a.append(r["grads"]["pos"][i, j])
+        a, fd = np.array(a), np.array(fd)
+        cos[flags] = float(a @ fd / (np.linalg.norm(a) * np.linalg.norm(fd)))
+    print(f"[pin] position gradient vs FD of the forward: cosine {cos[0]:+.3f} as the shader, "
+          f"{cos[1]:+.3f} with the forward's y")
+    assert cos[1] >= 0.95
+    assert cos[0] < cos[1]
 
 
 # ------------------------------------------------ finite differences (FP64)
