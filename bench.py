@@ -241,7 +241,13 @@ def run_growth4m(args):
     "unit" it keeps the parity configs' 1e-3 rates, sh interval 1 and
     loss (0.8 L2, 0.2 L1), under which added splats are never trained
     away and the image error grows. The timed region holds every step and
-    every prune / densify event. --geom-flags 1 (the default here) gives the
+    every prune / densify event. An event densifies, then prunes, as the
+    reference's runDensifyAndPruneCombined does; --densify-rule tile_l1
+    (default) decides with the tile-error rule of the 300-step
+    configuration (the scene grows to the 4M budget), gradient with the
+    trainer's rule (mean screen-space gradients since the last event,
+    multi-view importance, DEFAULT_DENSIFICATION_CONFIG thresholds; the
+    scene grows to about 1.4M). --geom-flags 1 (the default here) gives the
     geometry backward's mean path the forward's y convention: with the
     shader's +f_y (:264-266, --geom-flags 0) every position-y update moves
     against the rendered image's dependence, so the positions drift and the
@@ -267,7 +273,13 @@ def run_growth4m(args):
     for v in range(V):
         c.set_target(v, targets[v])
     views = list(range(V))
-    log(f"growth4m: {N0} splats, {V} views {W}x{H}, {args.warmup} warmup + {args.steps} steps")
+    # the trainer's densify rule: its default thresholds (scene extent 2 for
+    # the [-1, 1]^3 scene, pixel scale = width / 2), importance from 10 of
+    # the views (MULTI_VIEW_SAMPLE_COUNT; every other view here)
+    rule = sv.densify_rule(2.0, W)
+    sample = views[::2][:10]
+    log(f"growth4m: {N0} splats, {V} views {W}x{H}, {args.warmup} warmup + {args.steps} steps, "
+        f"densify rule {args.densify_rule}")
     for _ in range(args.warmup):
         c.step_views(views, sv.MODE_BATCHED)
     c.sync()
@@ -277,19 +289,26 @@ def run_growth4m(args):
         st = c.step_views(views, sv.MODE_BATCHED, stats=True)
         isects += int(st.n_isect)
         if it % EVERY == 0:
+            # runDensifyAndPruneCombined (gaussian-renderer.ts:3468-3488):
+            # densify, then prune
             l1 = float(st.l1_sum) / (3.0 * V * W * H)
-            removed = c.prune_opacity(0.005)
             budget = NMAX - c.count
             cap = min(max(1, c.count // 10), budget) if args.growth_cap else 0
-            act, nc, ns = c.densify_decide(views, 0.9, 8.0, max_new=max(cap, 0) if args.growth_cap else 0)
+            max_new = max(cap, 0) if args.growth_cap else 0
+            if args.densify_rule == "gradient":
+                imp, _, ns_views = c.multiview_scores(sample)
+                act, nc, ns = c.densify_decide_grad(rule, imp, ns_views, max_new=max_new)
+            else:
+                act, nc, ns = c.densify_decide(views, 0.9, 8.0, max_new=max_new)
             if args.growth_cap and cap <= 0:
                 act[:] = 0
             if nc + ns > budget:
                 idx = np.flatnonzero(act)
                 act[idx[max(budget, 0):]] = 0
-            added = c.densify(act, seed=it)
-            events.append({"step": it, "mean_l1": l1, "pruned": removed, "added": added, "count": c.count})
-            log(f"growth4m step {it}: mean L1 {l1:.5f}, pruned {removed}, added {added}, {c.count} splats")
+            added = c.densify(act, seed=it) if act.any() else 0
+            removed = c.prune_opacity(0.005)
+            events.append({"step": it, "mean_l1": l1, "added": added, "pruned": removed, "count": c.count})
+            log(f"growth4m step {it}: mean L1 {l1:.5f}, added {added}, pruned {removed}, {c.count} splats")
     c.sync()
     dt = time.perf_counter() - t0
     P = V * W * H
@@ -302,6 +321,8 @@ def run_growth4m(args):
                                f"{EVERY} steps", "splats_start": N0, "splats_end": c.count, "views": V,
                    "width": W, "height": H, "growth_cap": "max(1, floor(0.1 N)) per event, best score first"
                    if args.growth_cap else "none (4M budget only)",
+                   "densify_rule": "trainer's gradient rule (gaussian-renderer.ts:3249-3306), default thresholds"
+                   if args.densify_rule == "gradient" else "tile L1 above the 90th percentile, split above 8 px",
                    "geometry_backward": "mean y as the forward (SV_GEOM_MEAN_Y_FORWARD)"
                    if geom_flags(args) & sv.GEOM_MEAN_Y_FORWARD else "the shader as written (+f_y mean path)",
                    "training": "reference defaults: lr means 1.6e-4 scheduled, SH 2.5e-3 every 16 steps, opacity "
@@ -487,6 +508,8 @@ def main():
     ap.add_argument("--growth-params", choices=("reference", "unit"), default="reference",
                     help="growth4m: the reference trainer's default learning rates, schedules and loss, or the "
                          "parity configs' 1e-3 rates")
+    ap.add_argument("--densify-rule", choices=("gradient", "tile_l1"), default="tile_l1",
+                    help="growth4m: the trainer's gradient rule or the tile-error rule")
     ap.add_argument("--geom-flags", type=int, default=None,
                     help="SV_GEOM_* flags of the geometry backward (default: 1 for growth4m, 0 otherwise)")
     ap.add_argument("--no-configs", action="store_true",

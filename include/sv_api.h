@@ -338,6 +338,39 @@ sv_status sv_densify(sv_ctx *ctx, const uint8_t *actions, uint64_t seed,
 sv_status sv_densify_decide(sv_ctx *ctx, const int *views, int nviews, float percentile,
                             float split_px, size_t max_new, uint8_t *actions, size_t *n_clone,
                             size_t *n_split);
+/* Thresholds of the reference trainer's densify rule
+ * (DEFAULT_DENSIFICATION_CONFIG, densification.ts:26-33, and
+ * MULTI_VIEW_IMPORTANCE_THRESHOLD, gaussian-renderer.ts:71), in double as
+ * the JS computes them. */
+typedef struct sv_densify_rule {
+  double grad_threshold;       /* clone: mean |dL/dmean2d| >= this x pixel_scale (0.003) */
+  double grad_abs_threshold;   /* split: mean |dL/dconic| >= this x pixel_scale (0.015) */
+  double size_threshold;       /* world units: largest scale <= it takes the clone
+                                  test, > it the split test (scene extent x 0.001,
+                                  densification.ts:104-106) */
+  double min_visibility;       /* visible steps required, max(1, .) (25) */
+  double importance_threshold; /* importance / max(1, samples) must exceed it (5) */
+  double pixel_scale;          /* training image width / 2 (:3251) */
+} sv_densify_rule;
+/* The candidate selection of performDensification as the reference trainer
+ * runs it (gaussian-renderer.ts:3249-3326), decided on the device from the
+ * step statistics accumulated since the last prune / densify (grad_accum,
+ * densify-accum.wgsl: sum of |dL/dmean2d|, sum of |dL/dconic|, visible
+ * steps) and a multi-view importance count (sv_multiview_scores over
+ * `n_samples` views): a Gaussian with at least min_visibility visible
+ * steps and importance / max(1, n_samples) above importance_threshold is a
+ * clone candidate when its largest scale exp(log-scale) is at most
+ * size_threshold and its mean 2-D gradient reaches grad_threshold x
+ * pixel_scale, a split candidate when the scale is larger and its mean
+ * conic gradient reaches grad_abs_threshold x pixel_scale; its score is
+ * that mean over the threshold (the mean itself for a threshold <= 1e-8).
+ * max_new > 0 keeps the max_new candidates of highest score (ties in
+ * index order; the reference uses max(1, floor(0.1 N))); 0 keeps all.
+ * importance: N counts (host). actions: N bytes (0 none, 1 clone, 2
+ * split), ready for sv_densify. */
+sv_status sv_densify_decide_grad(sv_ctx *ctx, const sv_densify_rule *rule, const uint32_t *importance,
+                                 uint32_t n_samples, size_t max_new, uint8_t *actions, size_t *n_clone,
+                                 size_t *n_split);
 /* Per-tile L1 error of view `view` from the last sv_densify_decide
  * (tiles_x * tiles_y floats, row-major tiles). */
 sv_status sv_read_tile_l1(sv_ctx *ctx, int view, float *out);

@@ -402,6 +402,28 @@ def densify_decide(projs, views, tile_l1s, thr, split_px: float, with_score: boo
     return (act, score) if with_score else act
 
 
+def densify_decide_grad(accum4: np.ndarray, logscale: np.ndarray, importance: np.ndarray, n_samples: int,
+                        rule: dict, with_score: bool = False):
+    """The reference trainer's densify candidates (orc_densify_decide_grad);
+    rule keys: grad_threshold, grad_abs_threshold, size_threshold,
+    min_visibility, importance_threshold, pixel_scale."""
+    acc = np.ascontiguousarray(accum4, np.float32).reshape(-1, 4)
+    ls = np.ascontiguousarray(logscale, np.float32).reshape(-1, 3)
+    imp = np.ascontiguousarray(importance, np.uint32)
+    n = acc.shape[0]
+    r = (C.c_double * 6)(*[float(rule[k]) for k in DENSIFY_RULE_KEYS])
+    act = np.zeros(n, np.uint8)
+    score = np.zeros(n, np.float64)
+    lib().orc_densify_decide_grad(C.c_size_t(n), _fp(acc), _fp(ls), _u32p(imp), C.c_uint32(n_samples), r,
+                                  act.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  score.ctypes.data_as(C.POINTER(C.c_double)))
+    return (act, score) if with_score else act
+
+
+DENSIFY_RULE_KEYS = ("grad_threshold", "grad_abs_threshold", "size_threshold", "min_visibility",
+                     "importance_threshold", "pixel_scale")
+
+
 def densify_cap(actions: np.ndarray, score: np.ndarray, max_new: int) -> np.ndarray:
     """performDensification's growth cap (gaussian-renderer.ts:3297-3306):
     the candidates sorted by score, highest first, with a stable sort of the

@@ -1540,6 +1540,40 @@ void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
   }
 }
 
+void orc_densify_decide_grad(size_t n, const float *accum4, const float *logscale,
+                             const uint32_t *importance, uint32_t n_samples, const double *rule,
+                             uint8_t *actions, double *score) {
+  /* :3251-3255 */
+  const double grad_px = rule[0] * rule[5], abs_px = rule[1] * rule[5];
+  const double size_thr = rule[2];
+  const double min_vis = rule[3] > 1.0 ? rule[3] : 1.0;
+  const double samples = n_samples > 1u ? (double)n_samples : 1.0;
+  for (size_t i = 0; i < n; ++i) {
+    uint8_t act = 0;
+    double sc = 0.0;
+    const double denom = accum4[4 * i + 3];
+    if (!(denom < min_vis)) { /* :3263-3266 */
+      const double avg = (double)accum4[4 * i] / denom;
+      const double avg_abs = (double)accum4[4 * i + 2] / denom;
+      const double s0 = exp((double)logscale[3 * i]), s1 = exp((double)logscale[3 * i + 1]),
+                   s2 = exp((double)logscale[3 * i + 2]);
+      const double smax = fmax(fmax(s0, s1), s2); /* :3270-3275 */
+      const double imp = (double)importance[i] / samples;
+      if (!(imp <= rule[4])) { /* :3276-3280 */
+        if (smax <= size_thr && avg >= grad_px) {
+          act = 1;
+          sc = grad_px > 1e-8 ? avg / grad_px : avg;
+        } else if (smax > size_thr && avg_abs >= abs_px) {
+          act = 2;
+          sc = abs_px > 1e-8 ? avg_abs / abs_px : avg_abs;
+        }
+      }
+    }
+    actions[i] = act;
+    if (score) score[i] = sc;
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* knn_init: knn-init.wgsl:40-90                                        */
 /* ------------------------------------------------------------------ */

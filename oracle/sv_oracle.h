@@ -199,6 +199,17 @@ void orc_densify_decide(size_t n, size_t nviews, const orc_proj *const *proj,
                         const orc_view *views, const float *const *tile_l1,
                         const float *thr, float split_px, uint8_t *actions, float *score);
 
+/* The reference trainer's densify candidates (performDensification,
+ * gaussian-renderer.ts:3249-3290), in double as the JS evaluates them.
+ * rule[6] = grad_threshold, grad_abs_threshold, size_threshold,
+ * min_visibility, importance_threshold, pixel_scale; accum4 = per Gaussian
+ * (sum |dL/dmean2d|, unused, sum |dL/dconic|, visible steps); importance =
+ * multi-view counts over n_samples views. score (may be NULL): the
+ * normalised score of each candidate, 0 for the others. */
+void orc_densify_decide_grad(size_t n, const float *accum4, const float *logscale,
+                             const uint32_t *importance, uint32_t n_samples, const double *rule,
+                             uint8_t *actions, double *score);
+
 /* knn_init (knn-init.wgsl:40-90; constants gaussian-renderer.ts:77-79):
  * for each of the n points (positions xyz interleaved, rounded to f16 as
  * read_position does) the three smallest distances to the other points

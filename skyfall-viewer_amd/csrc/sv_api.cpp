@@ -2362,6 +2362,72 @@ sv_status sv_densify_decide(sv_ctx *c, const int *vs, int nv, float percentile, 
   return SV_OK;
 }
 
+sv_status sv_densify_decide_grad(sv_ctx *c, const sv_densify_rule *rule, const uint32_t *importance,
+                                 uint32_t n_samples, size_t max_new, uint8_t *actions, size_t *n_clone,
+                                 size_t *n_split) {
+  if (sv_status s = check_ctx(c)) return s;
+  if (!rule || !actions || (!importance && c->N)) return fail(c, SV_E_ARG, "rule, importance and actions are required");
+  const double vals[6] = {rule->grad_threshold, rule->grad_abs_threshold, rule->size_threshold,
+                          rule->min_visibility, rule->importance_threshold, rule->pixel_scale};
+  for (double v : vals)
+    if (!std::isfinite(v)) return fail(c, SV_E_ARG, "densify rule thresholds must be finite");
+  // performDensification (gaussian-renderer.ts:3249-3255): thresholds in
+  // pixels, visibility at least 1, importance over max(1, samples)
+  DensifyRuleDev r;
+  r.grad_px = rule->grad_threshold * rule->pixel_scale;
+  r.abs_px = rule->grad_abs_threshold * rule->pixel_scale;
+  r.size = rule->size_threshold;
+  r.min_vis = std::max(1.0, rule->min_visibility);
+  r.imp_threshold = rule->importance_threshold;
+  r.samples = (double)std::max<uint32_t>(1u, n_samples);
+  if (n_clone) *n_clone = 0;
+  if (n_split) *n_split = 0;
+  if (c->N == 0) return SV_OK;
+  uint32_t *d_imp = nullptr;
+  uint8_t *d_act = nullptr;
+  double *d_score = nullptr;
+  auto release = [&]() {
+    dfree(d_imp);
+    dfree(d_act);
+    dfree(d_score);
+  };
+  sv_status st = dalloc(c, &d_imp, (size_t)c->N);
+  if (!st) st = dalloc(c, &d_act, (size_t)c->N);
+  if (!st) st = dalloc(c, &d_score, (size_t)c->N);
+  if (!st && hipMemcpy(d_imp, importance, c->N * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    st = fail(c, SV_E_HIP, "importance upload failed");
+  if (!st) {
+    launch_densify_decide_grad(c->st, c->grad_accum, c->P, c->cap, c->N, d_imp, r, d_act, d_score);
+    st = check_launch(c, "densify_decide_grad");
+  }
+  if (!st && hipMemcpy(actions, d_act, c->N, hipMemcpyDeviceToHost) != hipSuccess)
+    st = fail(c, SV_E_HIP, "actions readback failed");
+  std::vector<uint32_t> cand;
+  for (uint32_t i = 0; !st && i < c->N; ++i)
+    if (actions[i]) cand.push_back(i);
+  if (!st && max_new && cand.size() > max_new) {
+    // the growth cap (:3297-3306): candidates by score, highest first, ties
+    // in index order (the JS sort is stable); the first max_new are kept
+    std::vector<double> score(c->N);
+    if (hipMemcpy(score.data(), d_score, c->N * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      st = fail(c, SV_E_HIP, "score readback failed");
+    if (!st) {
+      std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return score[a] > score[b]; });
+      for (size_t k = max_new; k < cand.size(); ++k) actions[cand[k]] = 0;
+    }
+  }
+  release();
+  if (st) return st;
+  size_t nc = 0, ns = 0;
+  for (uint32_t i = 0; i < c->N; ++i) {
+    nc += actions[i] == 1;
+    ns += actions[i] == 2;
+  }
+  if (n_clone) *n_clone = nc;
+  if (n_split) *n_split = ns;
+  return SV_OK;
+}
+
 sv_status sv_read_tile_l1(sv_ctx *c, int v, float *out) {
   if (sv_status s = check_ctx(c)) return s;
   if (sv_status s = check_view(c, v)) return s;

@@ -249,6 +249,15 @@ void launch_tile_l1(hipStream_t st, const DevView *views, const uint32_t *tile_v
 void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, const uint32_t *counts, uint32_t N,
                            const DevView *views, int nv, const float *tile_l1, const float *thr,
                            float split_px, uint8_t *actions, float *score);
+// The reference trainer's densify rule from the step statistics; the
+// thresholds already multiplied out in double (grad_px = grad_threshold x
+// pixel_scale, and so on) as the JS forms them.
+struct DensifyRuleDev {
+  double grad_px, abs_px, size, min_vis, imp_threshold, samples;
+};
+void launch_densify_decide_grad(hipStream_t st, const float4 *grad_accum, const float *P, size_t cap, uint32_t N,
+                                const uint32_t *importance, const DensifyRuleDev &r, uint8_t *actions,
+                                double *score);
 void launch_densify(hipStream_t st, const float *P, const float *M1, const float *M2, size_t cap,
                     uint32_t N, const uint8_t *actions, const uint32_t *keep_offs,
                     const uint32_t *child_offs, uint32_t n_keep, uint64_t seed, float *Pn,

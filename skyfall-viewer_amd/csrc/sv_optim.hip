@@ -825,6 +825,53 @@ __global__ __launch_bounds__(256) void k_densify_decide(const float4 *__restrict
   if (score) score[g] = cand ? best : 0.0f;
 }
 
+// performDensification's candidate loop (gaussian-renderer.ts:3262-3290),
+// in double as the JS evaluates it: visible steps (grad_accum.w) at least
+// min_vis; mean 2-D gradient (x / w) and mean conic gradient (z / w);
+// largest exp(log-scale); importance / samples above the threshold; clone
+// if the largest scale <= size and the mean gradient >= grad_px, split if
+// larger and the mean conic gradient >= abs_px; score = mean / threshold
+// (the mean itself for a threshold <= 1e-8).
+__global__ __launch_bounds__(256) void k_densify_decide_grad(const float4 *__restrict__ grad_accum,
+                                                             const float *__restrict__ P, size_t cap, uint32_t N,
+                                                             const uint32_t *__restrict__ importance,
+                                                             DensifyRuleDev r, uint8_t *__restrict__ actions,
+                                                             double *__restrict__ score) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  const float4 acc = grad_accum[g];
+  const double denom = (double)acc.w;
+  uint8_t act = 0;
+  double sc = 0.0;
+  if (!(denom < r.min_vis)) {
+    const double avg = (double)acc.x / denom, avg_abs = (double)acc.z / denom;
+    const double s0 = exp((double)P[(size_t)(P_SCALE + 0) * cap + g]);
+    const double s1 = exp((double)P[(size_t)(P_SCALE + 1) * cap + g]);
+    const double s2 = exp((double)P[(size_t)(P_SCALE + 2) * cap + g]);
+    const double smax = fmax(fmax(s0, s1), s2);
+    const double imp = (double)importance[g] / r.samples;
+    if (!(imp <= r.imp_threshold)) {
+      if (smax <= r.size && avg >= r.grad_px) {
+        act = 1;
+        sc = r.grad_px > 1e-8 ? avg / r.grad_px : avg;
+      } else if (smax > r.size && avg_abs >= r.abs_px) {
+        act = 2;
+        sc = r.abs_px > 1e-8 ? avg_abs / r.abs_px : avg_abs;
+      }
+    }
+  }
+  actions[g] = act;
+  score[g] = sc;
+}
+
+void launch_densify_decide_grad(hipStream_t st, const float4 *grad_accum, const float *P, size_t cap, uint32_t N,
+                                const uint32_t *importance, const DensifyRuleDev &r, uint8_t *actions,
+                                double *score) {
+  if (N == 0) return;
+  hipLaunchKernelGGL(k_densify_decide_grad, dim3((N + 255) / 256), dim3(256), 0, st, grad_accum, P, cap, N,
+                     importance, r, actions, score);
+}
+
 void launch_densify_decide(hipStream_t st, const float4 *rec, const float2 *aux, const uint32_t *counts, uint32_t N,
                            const DevView *views, int nv, const float *tile_l1, const float *thr,
                            float split_px, uint8_t *actions, float *score) {

@@ -36,6 +36,31 @@ export interface TrainingStatus {
 export interface PruneParams { minOpacity: number; maxScale?: number; maxScreenRadius?: number; useScreenSizePruning?: boolean; }
 export interface StepStats { nIsect: number; nViews: number; sortPasses: number; keyBits: number; l1Sum: number; }
 export interface Grads extends PointCloudArrays { geom: Float32Array; }
+export interface MultiViewScores { importance: Uint32Array; pruning: Float32Array; sampleCount: number; }
+// TrainingScheduleConfig (src/utils/training-controller.ts:1-21)
+export interface TrainingScheduleConfig {
+  densifyFromIter: number; densifyUntilIter: number; densificationInterval: number;
+  pruneFromIter: number; pruneUntilIter: number; pruneInterval: number;
+  opacityResetInterval: number; opacityResetValue: number;
+  minOpacity: number; maxScreenSize: number; maxScaleRatio: number; maxScaleAbsolute: number;
+}
+// DensificationConfig (src/utils/densification.ts:14-24)
+export interface DensificationConfig {
+  gradThreshold: number; gradAbsThreshold: number; denseSizeRatio: number; minVisibility: number;
+  importanceThreshold: number; postDensifyOpacityCap: number;
+}
+export declare const DEFAULT_TRAINING_SCHEDULE: TrainingScheduleConfig;
+export declare const DEFAULT_DENSIFICATION_CONFIG: DensificationConfig;
+export declare class TrainingController {
+  constructor(sceneExtent: number, config?: Partial<TrainingScheduleConfig>);
+  setIteration(iteration: number): void;
+  getIteration(): number;
+  shouldPrune(): boolean;
+  shouldDensify(): boolean;
+  shouldResetOpacity(): boolean;
+  getPruneParams(): Required<PruneParams>;
+  getOpacityResetValue(): number;
+}
 
 export declare class Context {
   constructor(capacity: number, device?: number);
@@ -67,12 +92,14 @@ export interface MI355XGaussianRenderer {
   resetOpacity(maxOpacity?: number): Promise<void>;
   initializeKnnAndSync(sceneExtent?: number): Promise<Float32Array>;
   renderCamera(camera: CameraData): { image: Float32Array; alpha: Float32Array; width: number; height: number };
-  getMultiViewScores(views?: number[]): Promise<{ importance: Uint32Array; pruning: Float32Array; sampleCount: number }>;
-  densify(opts?: { views?: number[]; percentile?: number; splitPx?: number; seed?: number; maxNew?: number }): Promise<{ cloned: number; split: number }>;
+  getMultiViewScores(views?: number[]): Promise<MultiViewScores>;
+  densify(opts?: { rule?: 'gradient' | 'tile_l1'; scores?: MultiViewScores; views?: number[]; percentile?: number;
+                   splitPx?: number; seed?: number; maxNew?: number }): Promise<{ cloned: number; split: number }>;
+  densifyAndPrune(): Promise<{ densified: boolean; pruned: boolean; pruneAttempted: boolean }>;
   enableProfiling(enabled: boolean): void;
   printProfilingReport(): void;
   profiledTrainStep(): Promise<StepStats>;
-  getTrainingController(): null;
+  getTrainingController(): TrainingController | null;
   dumpTrainingDiagnostics(): Promise<void>;
   readImage(view: number): Float32Array;
   getGaussians(): PointCloudArrays;
@@ -81,7 +108,9 @@ export interface MI355XGaussianRenderer {
 
 export declare function createGaussianRenderer(
   pointCloud: PointCloudArrays,
-  options?: { capacity?: number; device?: number; viewOrder?: 'cycle' | 'batched'; config?: Record<string, number>; sceneExtent?: number },
+  options?: { capacity?: number; device?: number; viewOrder?: 'cycle' | 'batched'; config?: Record<string, number>;
+             sceneExtent?: number; schedule?: Partial<TrainingScheduleConfig> | false;
+             densification?: Partial<DensificationConfig> },
 ): MI355XGaussianRenderer;
 
 export declare const native: Record<string, unknown>;

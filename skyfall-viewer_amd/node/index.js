@@ -9,7 +9,10 @@
 //    (setTrainingImages takes {data, width, height});
 //  - the camera per iteration is chosen by the caller's view order
 //    (option viewOrder: 'cycle' | 'batched') instead of Math.random;
-//  - prune uses the opacity rule (minOpacity) decided on the device;
+//  - prune uses the opacity rule (minOpacity) decided on the device, not
+//    the weighted random draw of applyMultiViewPrune;
+//  - the multi-view scores sample up to 10 views by a seeded shuffle of the
+//    iteration instead of Math.random;
 //  - errors of the device path throw; missing state warns and skips as the
 //    reference does (console.warn + return).
 
@@ -21,6 +24,79 @@ const DEFAULT_PARAMS = {
   sh_update_interval: 16,
 };
 const DEFAULT_LOSS = { w_l2: 0.0, w_l1: 0.8, w_ssim: 0.2 };
+// DEFAULT_TRAINING_SCHEDULE (training-controller.ts:24-44)
+const DEFAULT_TRAINING_SCHEDULE = {
+  densifyFromIter: 500, densifyUntilIter: 15000, densificationInterval: 100,
+  pruneFromIter: 500, pruneUntilIter: 25000, pruneInterval: 100,
+  opacityResetInterval: 3000, opacityResetValue: 0.01,
+  minOpacity: 0.005, maxScreenSize: 20.0, maxScaleRatio: 0.0, maxScaleAbsolute: 1.0,
+};
+// DEFAULT_DENSIFICATION_CONFIG (densification.ts:26-33); performDensification
+// gates on MULTI_VIEW_IMPORTANCE_THRESHOLD and samples MULTI_VIEW_SAMPLE_COUNT
+// views (gaussian-renderer.ts:71-72)
+const DEFAULT_DENSIFICATION_CONFIG = {
+  gradThreshold: 0.003, gradAbsThreshold: 0.015, denseSizeRatio: 0.001, minVisibility: 25,
+  importanceThreshold: 1.5, postDensifyOpacityCap: 0.8,
+};
+const MULTI_VIEW_IMPORTANCE_THRESHOLD = 5;
+const MULTI_VIEW_SAMPLE_COUNT = 10;
+
+// TrainingController (training-controller.ts:59-160): the iteration
+// predicates of the prune / densify / opacity-reset schedule.
+class TrainingController {
+  constructor(sceneExtent, config = {}) {
+    this.config = { ...DEFAULT_TRAINING_SCHEDULE, ...config };
+    this.sceneExtent = sceneExtent;
+    this.iteration = 0;
+  }
+  setIteration(i) { this.iteration = i; }
+  getIteration() { return this.iteration; }
+  shouldPrune() {
+    const i = this.iteration, c = this.config;
+    if (i < c.pruneFromIter || i > c.pruneUntilIter) return false;
+    return i % c.pruneInterval === 0;
+  }
+  shouldDensify() {
+    const i = this.iteration, c = this.config;
+    if (i < c.densifyFromIter || i > c.densifyUntilIter) return false;
+    return i % c.densificationInterval === 0;
+  }
+  shouldResetOpacity() {
+    const i = this.iteration, c = this.config;
+    if (i === 0) return false;
+    if (i < c.densifyFromIter || i > c.densifyUntilIter) return false;
+    return i % c.opacityResetInterval === 0;
+  }
+  getPruneParams() {
+    const c = this.config, useScreenSizePruning = this.iteration > c.opacityResetInterval;
+    let maxScale = 1e6;
+    if (c.maxScaleAbsolute > 0) maxScale = c.maxScaleAbsolute;
+    else if (c.maxScaleRatio > 0) maxScale = this.sceneExtent * c.maxScaleRatio;
+    return { minOpacity: c.minOpacity, maxScale, maxScreenRadius: useScreenSizePruning ? c.maxScreenSize : 10000.0,
+             useScreenSizePruning };
+  }
+  getOpacityResetValue() { return this.config.opacityResetValue; }
+}
+
+// Up to `count` distinct view indices of `n`, a seeded Fisher-Yates shuffle
+// (mulberry32 of the seed) standing in for sampleCameraIndices' Math.random.
+function sampleViews(n, count, seed) {
+  const idx = Array.from({ length: n }, (_, i) => i);
+  if (n <= count) return idx;
+  let s = seed >>> 0;
+  const rand = () => {
+    s = (s + 0x6D2B79F5) >>> 0;
+    let t = s;
+    t = Math.imul(t ^ (t >>> 15), t | 1);
+    t ^= t + Math.imul(t ^ (t >>> 7), t | 61);
+    return ((t ^ (t >>> 14)) >>> 0) / 4294967296;
+  };
+  for (let i = n - 1; i > 0; i--) {
+    const j = Math.floor(rand() * (i + 1));
+    [idx[i], idx[j]] = [idx[j], idx[i]];
+  }
+  return idx.slice(0, count);
+}
 
 function cameraToNative(cam) {
   const pick = (k) => Array.from(cam[k]);
@@ -88,6 +164,10 @@ function createGaussianRenderer(pointCloud, options = {}) {
   let cursor = 0;
   let profiling = false;
   const viewOrder = options.viewOrder || 'cycle';
+  const sceneExtent = options.sceneExtent || 1.0;
+  // the schedule runs inside train(); options.schedule = false turns it off
+  const controller = options.schedule === false ? null : new TrainingController(sceneExtent, options.schedule || {});
+  const densification = { ...DEFAULT_DENSIFICATION_CONFIG, ...(options.densification || {}) };
 
   function applyConfig(extra = {}) {
     native.setConfig(ctx.handle, {
@@ -146,7 +226,19 @@ function createGaussianRenderer(pointCloud, options = {}) {
       const t0 = Date.now();
       const start = iteration;
       try {
-        while (iteration < target && !stopRequested) await renderer.trainStep();
+        while (iteration < target && !stopRequested) {
+          await renderer.trainStep();
+          if (!controller) continue;
+          // gaussian-renderer.ts:3796-3828
+          controller.setIteration(iteration);
+          let densifyHandledPrune = false;
+          if (controller.shouldDensify()) {
+            await renderer.densifyAndPrune();
+            densifyHandledPrune = true;
+          }
+          if (!densifyHandledPrune && controller.shouldPrune()) await renderer.prune();
+          if (controller.shouldResetOpacity()) await renderer.resetOpacity(controller.getOpacityResetValue());
+        }
       } finally {
         training = false;
         const dt = (Date.now() - t0) / 1000;
@@ -159,8 +251,20 @@ function createGaussianRenderer(pointCloud, options = {}) {
       iterationsPerSecond: ips, gaussianCount: native.getCount(ctx.handle),
     }),
     async prune(customParams) {
-      const minOpacity = customParams && customParams.minOpacity !== undefined ? customParams.minOpacity : 0.005;
-      return native.pruneOpacity(ctx.handle, minOpacity);
+      const params = customParams || (controller ? controller.getPruneParams() : { minOpacity: 0.005 });
+      const minOpacity = params.minOpacity !== undefined ? params.minOpacity : 0.005;
+      native.pruneOpacity(ctx.handle, minOpacity);
+      return native.getCount(ctx.handle);
+    },
+    // runDensifyAndPruneCombined (gaussian-renderer.ts:3468-3488): one set
+    // of multi-view scores, densify, then prune
+    async densifyAndPrune() {
+      if (!native.getCount(ctx.handle)) return { densified: false, pruned: false, pruneAttempted: false };
+      const scores = await renderer.getMultiViewScores();
+      const d = await renderer.densify({ scores });
+      const before = native.getCount(ctx.handle);
+      await renderer.prune();
+      return { densified: d.cloned + d.split > 0, pruned: native.getCount(ctx.handle) !== before, pruneAttempted: true };
     },
     async resetOpacity(maxOpacity = 0.1) { native.resetOpacity(ctx.handle, maxOpacity); },
     // A live view of training from any camera (the viewer's render, drawn
@@ -171,16 +275,38 @@ function createGaussianRenderer(pointCloud, options = {}) {
     // parameters are the device masters, so there is nothing to sync.
     async initializeKnnAndSync(sceneExtent = options.sceneExtent || 1.0) { return native.knnInit(ctx.handle, sceneExtent); },
     // generateMultiViewScores (gaussian-renderer.ts:2754-2781) over the given
-    // views (the reference samples MULTI_VIEW_SAMPLE_COUNT = 10 at random)
-    async getMultiViewScores(views = cameras.map((_, i) => i).slice(0, 10)) {
+    // views (the reference samples MULTI_VIEW_SAMPLE_COUNT = 10 at random;
+    // here a seeded sample of the iteration)
+    async getMultiViewScores(views = sampleViews(cameras.length, MULTI_VIEW_SAMPLE_COUNT, iteration)) {
       return native.multiViewScores(ctx.handle, views);
     },
     // performDensification (gaussian-renderer.ts:3211-3466): at most
-    // max(1, floor(0.1 N)) new splats per event, highest score first (:3297-3306)
-    async densify({ views, percentile = 0.9, splitPx = 8, seed = iteration,
+    // max(1, floor(0.1 N)) new splats per event, highest score first
+    // (:3297-3306). rule 'gradient' (the trainer's): mean screen-space
+    // gradients since the last event and multi-view importance
+    // (:3249-3290); rule 'tile_l1': candidates on tiles whose L1 error is
+    // above the views' `percentile`, split above `splitPx` px.
+    async densify({ rule = 'gradient', scores, views, percentile = 0.9, splitPx = 8, seed = iteration,
                     maxNew = Math.max(1, Math.floor(0.1 * native.getCount(ctx.handle))) } = {}) {
-      const d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx, maxNew);
-      native.densify(ctx.handle, d.actions, seed);
+      let d;
+      if (rule === 'tile_l1') {
+        d = native.densifyDecide(ctx.handle, views || cameras.map((_, i) => i), percentile, splitPx, maxNew);
+      } else if (rule === 'gradient') {
+        const sc = scores || await renderer.getMultiViewScores(views);
+        if (!sc || !sc.sampleCount) {
+          console.warn('[densify] Unable to compute multi-view scores; skipping.');
+          return { cloned: 0, split: 0 };
+        }
+        const width = cameras.length ? Math.round(cameras[0].viewport[0]) : 1;
+        d = native.densifyDecideGrad(ctx.handle, {
+          gradThreshold: densification.gradThreshold, gradAbsThreshold: densification.gradAbsThreshold,
+          sizeThreshold: sceneExtent * densification.denseSizeRatio, minVisibility: densification.minVisibility,
+          importanceThreshold: MULTI_VIEW_IMPORTANCE_THRESHOLD, pixelScale: Math.max(width, 1) / 2.0,
+        }, sc.importance, sc.sampleCount, maxNew);
+      } else {
+        throw new Error(`unknown densify rule ${rule}`);
+      }
+      if (d.clones + d.splits > 0) native.densify(ctx.handle, d.actions, seed);
       return { cloned: d.clones, split: d.splits };
     },
     enableProfiling(enabled) { profiling = !!enabled; native.enableTiming(ctx.handle, profiling); },
@@ -196,7 +322,7 @@ function createGaussianRenderer(pointCloud, options = {}) {
       native.enableTiming(ctx.handle, profiling);
       return st;
     },
-    getTrainingController: () => null,
+    getTrainingController: () => controller,
     async dumpTrainingDiagnostics() {
       const s = renderer.getTrainingStatus();
       console.log(`[diag] iteration ${s.currentIteration}, ${s.gaussianCount} Gaussians, ${cameras.length} views`);
@@ -208,4 +334,7 @@ function createGaussianRenderer(pointCloud, options = {}) {
   return renderer;
 }
 
-module.exports = { native, Context, RasterizeFunction, createGaussianRenderer, cameraToNative };
+module.exports = {
+  native, Context, RasterizeFunction, createGaussianRenderer, cameraToNative, TrainingController,
+  DEFAULT_TRAINING_SCHEDULE, DEFAULT_DENSIFICATION_CONFIG,
+};

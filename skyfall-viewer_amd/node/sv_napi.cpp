@@ -823,6 +823,51 @@ napi_value DensifyDecide(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// densifyDecideGrad(ctx, rule {gradThreshold, gradAbsThreshold, sizeThreshold,
+// minVisibility, importanceThreshold, pixelScale}, importance Uint32Array,
+// sampleCount, maxNew) -> {actions, clones, splits}
+napi_value DensifyDecideGrad(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a, 5)) return nullptr;
+  sv_ctx *c = ctx_of(env, a.v[0]);
+  if (!c) return nullptr;
+  sv_densify_rule r{};
+  const struct {
+    const char *name;
+    double *dst;
+  } keys[] = {{"gradThreshold", &r.grad_threshold},         {"gradAbsThreshold", &r.grad_abs_threshold},
+              {"sizeThreshold", &r.size_threshold},         {"minVisibility", &r.min_visibility},
+              {"importanceThreshold", &r.importance_threshold}, {"pixelScale", &r.pixel_scale}};
+  for (const auto &k : keys) {
+    napi_value v = prop(env, a.v[1], k.name);
+    if (!v || !get_f64(env, v, k.dst)) {
+      napi_throw_type_error(env, nullptr, "densify rule needs gradThreshold, gradAbsThreshold, sizeThreshold, "
+                                          "minVisibility, importanceThreshold and pixelScale");
+      return nullptr;
+    }
+  }
+  const size_t n = sv_get_count(c);
+  const uint32_t *imp = typed<uint32_t>(env, a.v[2], napi_uint32_array, n);
+  if (!imp) return nullptr;
+  uint32_t samples = 0;
+  double max_new = 0;
+  if (!get_u32(env, a.v[3], &samples) || !get_f64(env, a.v[4], &max_new)) return nullptr;
+  uint8_t *act = nullptr;
+  napi_value arr = new_typed<uint8_t>(env, napi_uint8_array, n, &act);
+  if (!arr) return nullptr;
+  size_t nc = 0, ns = 0;
+  SV(c, sv_densify_decide_grad(c, &r, imp, samples, max_new > 0 ? (size_t)max_new : 0, act, &nc, &ns),
+     "sv_densify_decide_grad");
+  napi_value o, v;
+  CK(napi_create_object(env, &o));
+  CK(napi_set_named_property(env, o, "actions", arr));
+  CK(napi_create_double(env, (double)nc, &v));
+  CK(napi_set_named_property(env, o, "clones", v));
+  CK(napi_create_double(env, (double)ns, &v));
+  CK(napi_set_named_property(env, o, "splits", v));
+  return o;
+}
+
 // timing -------------------------------------------------------------------
 napi_value EnableTiming(napi_env env, napi_callback_info info) {
   Args a;
@@ -1003,7 +1048,7 @@ napi_value Init(napi_env env, napi_value exports) {
              {"readAlpha", ReadAlpha},     {"readLastIds", ReadLastIds}, {"readGrads", ReadGrads},
              {"readKeys", ReadKeys},       {"pruneOpacity", PruneOpacity}, {"pruneMask", PruneMask},
              {"densify", Densify},         {"resetOpacity", ResetOpacity}, {"knnInit", KnnInit},
-             {"multiViewScores", MultiViewScores}, {"renderCamera", RenderCamera},         {"densifyDecide", DensifyDecide}, {"enableTiming", EnableTiming},
+             {"multiViewScores", MultiViewScores}, {"renderCamera", RenderCamera},         {"densifyDecide", DensifyDecide}, {"densifyDecideGrad", DensifyDecideGrad}, {"enableTiming", EnableTiming},
              {"timings", Timings},         {"sync", Sync},               {"autotune", Autotune},
              {"saveCheckpoint", SaveCheckpoint}, {"loadCheckpoint", LoadCheckpoint},               {"sceneGaussians", SceneGaussians},
              {"sceneJitter", SceneJitter}, {"sceneCamera", SceneCamera}};

@@ -43,6 +43,23 @@ class Config(C.Structure):
     ]
 
 
+class DensifyRule(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("grad_threshold", "grad_abs_threshold", "size_threshold", "min_visibility",
+                                          "importance_threshold", "pixel_scale")]
+
+
+def densify_rule(scene_extent: float, train_width: int, **kw) -> DensifyRule:
+    """The reference trainer's densify thresholds (DEFAULT_DENSIFICATION_CONFIG,
+    densification.ts:26-33; size threshold = scene extent x 0.001, pixel
+    scale = training width / 2; importance threshold 5), overridable by
+    keyword."""
+    r = DensifyRule(grad_threshold=0.003, grad_abs_threshold=0.015, size_threshold=scene_extent * 0.001,
+                    min_visibility=25.0, importance_threshold=5.0, pixel_scale=max(train_width, 1) / 2.0)
+    for k, v in kw.items():
+        setattr(r, k, v)
+    return r
+
+
 class StepStats(C.Structure):
     _fields_ = [("n_isect", C.c_uint64), ("n_views", C.c_uint32), ("sort_passes", C.c_uint32),
                 ("key_bits", C.c_uint32), ("binned", C.c_uint32), ("l1_sum", C.c_double)]
@@ -111,6 +128,8 @@ _SIGS = {
     "sv_multiview_scores": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, _U32, _P, C.POINTER(C.c_int)]),
     "sv_densify_decide": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_float, C.c_float, C.c_size_t,
                                     _U8, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "sv_densify_decide_grad": (C.c_int, [C.c_void_p, C.POINTER(DensifyRule), _U32, C.c_uint32, C.c_size_t, _U8,
+                                         C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "sv_read_tile_l1": (C.c_int, [C.c_void_p, C.c_int, _P]),
     "sv_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int, C.c_int, C.POINTER(TuneEntry),
                               C.POINTER(C.c_int)]),
@@ -578,6 +597,20 @@ class Context:
         nc, ns = C.c_size_t(), C.c_size_t()
         self._ck(lib().sv_densify_decide(self.h, vs, len(views), percentile, split_px, max_new,
                                          act.ctypes.data_as(_U8), C.byref(nc), C.byref(ns)), "densify_decide")
+        return act, nc.value, ns.value
+
+    def densify_decide_grad(self, rule: "DensifyRule", importance: np.ndarray, n_samples: int, max_new: int = 0):
+        """The reference trainer's densify decision from the step statistics and
+        multi-view importance counts. Returns (actions uint8[N], n_clone,
+        n_split)."""
+        imp = np.ascontiguousarray(importance, np.uint32)
+        if imp.shape != (self.count,):
+            raise ValueError(f"importance must hold {self.count} counts")
+        act = np.zeros(self.count, np.uint8)
+        nc, ns = C.c_size_t(), C.c_size_t()
+        self._ck(lib().sv_densify_decide_grad(self.h, C.byref(rule), imp.ctypes.data_as(_U32), n_samples, max_new,
+                                              act.ctypes.data_as(_U8), C.byref(nc), C.byref(ns)),
+                 "densify_decide_grad")
         return act, nc.value, ns.value
 
     def tile_l1(self, v: int) -> np.ndarray:

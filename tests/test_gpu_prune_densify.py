@@ -128,3 +128,38 @@ def test_densify_decide_matches_oracle(sv, orc):
     assert np.array_equal(act_cap, want_cap)
     assert nc_cap + ns_cap == max_new
     c.close()
+
+
+def test_gradient_rule_matches_oracle(sv, orc):
+    """The reference trainer's densify decision (performDensification's
+    candidate loop, gaussian-renderer.ts:3249-3306) on the device equals the
+    oracle's on the same state: step statistics accumulated over 12 batched
+    steps of 4 views (read back), multi-view importance of the 4 views.
+    Thresholds are lowered so that both clone and split candidates occur;
+    uncapped and with the reference's cap max(1, floor(0.1 N))."""
+    g = sv.scene_gaussians(0, 30000, 1.0)
+    cams = sv.ring_cameras(4, 3.0, 128, 96)
+    tg = dict(g)
+    tg["pos"] = sv.scene_jitter(1, g["pos"], 0.02)
+    targets = [orc.render_and_grad(tg, c, orc.view_params(128, 96, sh_deg=3))["color"] for c in cams]
+    with _ctx(sv, g, 30000, cams, targets) as c:
+        for _ in range(12):
+            c.step_views([0, 1, 2, 3], sv.MODE_BATCHED)
+        _, acc = c.stats()
+        imp, _, ns = c.multiview_scores([0, 1, 2, 3])
+        ls = c.get_gaussians()["logscale"]
+        rule = sv.densify_rule(1.0, 128, grad_threshold=0.0005, grad_abs_threshold=0.0004, size_threshold=0.02,
+                               min_visibility=6.0, importance_threshold=1.0)
+        rd = {k: getattr(rule, k) for k in orc.DENSIFY_RULE_KEYS}
+        want, score = orc.densify_decide_grad(acc, ls, imp, ns, rd, with_score=True)
+        got, nc, nsp = c.densify_decide_grad(rule, imp, ns)
+        print(f"[densify-grad] {c.count} splats, {ns} views scored: {int((want == 1).sum())} clone, "
+              f"{int((want == 2).sum())} split candidates", flush=True)
+        assert (want == 1).sum() > 20 and (want == 2).sum() > 20
+        assert np.array_equal(got, want)
+        assert (nc, nsp) == (int((want == 1).sum()), int((want == 2).sum()))
+        cap = max(1, c.count // 10)
+        got_c, _, _ = c.densify_decide_grad(rule, imp, ns, max_new=cap)
+        want_c = orc.densify_cap(want, score, cap)
+        assert int((want_c > 0).sum()) == min(cap, int((want > 0).sum()))
+        assert np.array_equal(got_c, want_c)

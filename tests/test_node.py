@@ -26,7 +26,7 @@ EXPORTS = ["version", "create", "destroy", "getConfig", "setConfig", "setGaussia
            "getMoments", "getCount", "setViews", "setTarget", "setResidual", "buildTiles", "forward", "loss",
            "backward", "backwardFixed", "zeroGrad", "adam", "step", "stepAsync", "readImage", "readResidual",
            "readAlpha", "readLastIds", "readGrads", "readKeys", "pruneOpacity", "pruneMask", "densify",
-           "resetOpacity", "densifyDecide", "enableTiming", "timings", "sync", "autotune", "sceneGaussians",
+           "resetOpacity", "densifyDecide", "densifyDecideGrad", "enableTiming", "timings", "sync", "autotune", "sceneGaussians",
            "sceneJitter", "sceneCamera"]
 
 
@@ -149,3 +149,58 @@ def test_js_train_stops_mid_loop():
     assert st["during_training"] and not st["training"]
     assert st["warned"] >= 1
     assert 2 <= st["iterations"] < 1000 and st["target"] == 1000
+
+
+@pytest.mark.gpu
+@needs_node
+def test_js_schedule_densifies_and_prunes():
+    """train() runs the reference's schedule (gaussian-renderer.ts:3796-3828):
+    with it brought forward to every 10 iterations, the splat count changes
+    only at iterations 10, 20, 30 and 40 (densify with the trainer's rule,
+    then prune), some event grows the scene, and the parameters stay
+    finite."""
+    script = os.path.join(ROOT, "tests", "node", "schedule_train.js")
+    print("[node] scheduled train", flush=True)
+    res = subprocess.run([NODE, script], capture_output=True, text=True, timeout=120, cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    st = json.loads(res.stdout.strip().splitlines()[-1])
+    counts = st["counts"]
+    print(f"[node] counts at events: {[counts[i] for i in (9, 19, 29, 39)]}, max opacity {st['maxOpacity']:.4f}",
+          flush=True)
+    assert st["iteration"] == 40 and st["finite"]
+    for i in range(1, 40):
+        if (i + 1) % 10:
+            assert counts[i] == counts[i - 1], i
+    assert max(counts) > 8192
+
+
+@needs_node
+def test_training_controller_schedule():
+    """The JS TrainingController's predicates at the edges of the reference's
+    DEFAULT_TRAINING_SCHEDULE (training-controller.ts:24-44, 86-126):
+    densify and prune every 100 iterations from 500 (densify until 15000,
+    prune until 25000), opacity reset every 3000 inside the densify window,
+    screen-size pruning after 3000."""
+    js = f"""
+const sv = require({json.dumps(INDEX)});
+const c = new sv.TrainingController(2.0);
+const out = {{}};
+for (const i of [0, 100, 499, 500, 550, 600, 3000, 3100, 15000, 15100, 25000, 25100]) {{
+  c.setIteration(i);
+  out[i] = [c.shouldDensify(), c.shouldPrune(), c.shouldResetOpacity(), c.getPruneParams().maxScreenRadius];
+}}
+const r = new sv.TrainingController(2.0, {{ maxScaleAbsolute: 0, maxScaleRatio: 0.5 }});
+out.maxScale = r.getPruneParams().maxScale;
+out.reset = c.getOpacityResetValue();
+console.log(JSON.stringify(out));
+"""
+    r = _node(js)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    F, T = False, True
+    expect = {0: [F, F, F, 10000], 100: [F, F, F, 10000], 499: [F, F, F, 10000], 500: [T, T, F, 10000],
+              550: [F, F, F, 10000], 600: [T, T, F, 10000], 3000: [T, T, T, 10000], 3100: [T, T, F, 20],
+              15000: [T, T, T, 20], 15100: [F, T, F, 20], 25000: [F, T, F, 20], 25100: [F, F, F, 20]}
+    for i, e in expect.items():
+        assert out[str(i)] == e, i
+    assert out["maxScale"] == 1.0 and out["reset"] == 0.01

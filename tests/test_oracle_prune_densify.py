@@ -161,3 +161,55 @@ def test_densify_cap_order():
         for i in range(n):
             assert got[i] == (act[i] if i in keep else 0), (max_new, i)
     assert int(np.count_nonzero(orc.densify_cap(act, score, max(1, n // 10)))) == max(1, n // 10)
+
+
+def _rule(**kw):
+    r = {"grad_threshold": 0.125, "grad_abs_threshold": 0.25, "size_threshold": 1.0, "min_visibility": 25.0,
+         "importance_threshold": 5.0, "pixel_scale": 8.0}
+    r.update(kw)
+    return r
+
+
+def test_densify_decide_grad_boundaries(orc):
+    """performDensification's candidate tests (gaussian-renderer.ts:3262-3290)
+    at their boundaries. With gradThreshold 0.125, gradAbsThreshold 0.25 and
+    pixel scale 8 the thresholds are exactly 1 and 2 px; size threshold 1
+    (exp(0) = 1 exactly); 2 samples, so importance must exceed 10."""
+    rows = [  # (sum |g2d|, sum |gconic|, visible steps), log-scales, importance -> action, score
+        ((1e6, 1e6, 24.0), (0.0, 0.0, 0.0), 100, 0, 0.0),    # fewer than 25 visible steps
+        ((25.0, 0.0, 25.0), (0.0, -1.0, -1.0), 11, 1, 1.0),  # mean 2-D gradient = threshold: clone
+        ((25.0, 0.0, 25.0), (0.0, -1.0, -1.0), 10, 0, 0.0),  # importance 10 / 2 = 5, not above 5
+        ((24.99, 1e6, 25.0), (0.0, 0.0, 0.0), 50, 0, 0.0),   # just below; the conic test is for large splats
+        ((0.0, 100.0, 50.0), (0.01, -1.0, -1.0), 50, 2, 1.0),  # large, mean conic gradient = 2: split
+        ((1000.0, 49.0, 25.0), (-1.0, 0.01, -1.0), 50, 0, 0.0),  # large with a small conic gradient
+        ((120.0, 0.0, 40.0), (-2.0, -2.0, -2.0), 50, 1, 3.0),  # clone, score 3
+        ((0.0, 200.0, 25.0), (-1.0, -1.0, 0.5), 50, 2, 4.0),   # split, score 8 / 2
+    ]
+    acc = np.array([[r[0][0], 0.0, r[0][1], r[0][2]] for r in rows], np.float32)
+    ls = np.array([r[1] for r in rows], np.float32)
+    imp = np.array([r[2] for r in rows], np.uint32)
+    act, score = orc.densify_decide_grad(acc, ls, imp, 2, _rule(), with_score=True)
+    assert act.tolist() == [r[3] for r in rows]
+    assert score.tolist() == [r[4] for r in rows]
+    # visibility is at least 1 whatever the setting; a zero pixel threshold
+    # ranks by the mean itself
+    acc2 = np.array([[5.0, 0.0, 0.0, 0.5], [5.0, 0.0, 0.0, 1.0]], np.float32)
+    act2, score2 = orc.densify_decide_grad(acc2, np.zeros((2, 3), np.float32), np.full(2, 99, np.uint32), 1,
+                                           _rule(min_visibility=0.0, grad_threshold=0.0), with_score=True)
+    assert act2.tolist() == [0, 1] and score2.tolist() == [0.0, 5.0]
+    # one sample counts as max(1, samples): importance 6 > 5
+    act3 = orc.densify_decide_grad(acc[1:2], ls[1:2], np.array([6], np.uint32), 0, _rule())
+    assert act3.tolist() == [1]
+
+
+def test_densify_grad_cap_keeps_best_scores(orc):
+    """The growth cap on the gradient rule's scores: highest first, equal
+    scores in index order."""
+    rows = [(25.0, 25.0), (25.0, 25.0), (75.0, 25.0), (50.0, 25.0)]  # scores 1, 1, 3, 2
+    acc = np.array([[a, 0.0, 0.0, d] for a, d in rows], np.float32)
+    act, score = orc.densify_decide_grad(acc, np.full((4, 3), -2.0, np.float32), np.full(4, 50, np.uint32), 2,
+                                         _rule(), with_score=True)
+    assert act.tolist() == [1, 1, 1, 1] and score.tolist() == [1.0, 1.0, 3.0, 2.0]
+    assert orc.densify_cap(act, score, 2).tolist() == [0, 0, 1, 1]
+    assert orc.densify_cap(act, score, 3).tolist() == [1, 0, 1, 1]
+    assert orc.densify_cap(act, score, 0).tolist() == [1, 1, 1, 1]
