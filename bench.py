@@ -157,7 +157,7 @@ def run_mode(mode, g, cams, targets, args, graph=0):
     timings = ctx.timings()
     ctx.enable_timing(False)
     eager = None
-    if graph:
+    if graph and not args.only_batched:
         ctx.set_config(use_graph=0)
         ctx.step_views(views, mode)
         ctx.sync()
@@ -502,6 +502,9 @@ def main():
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: time the tuner's candidates first and run the batched step with the fastest")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run the batched step alone (no eager, sequential, plain, pair-sort, CPU or other "
+                         "workload legs): the target of the profiler runs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--growth-cap", type=int, default=1,
                     help="growth4m: 1 = at most max(1, N // 10) new splats per event, best score first")
@@ -518,6 +521,8 @@ def main():
                     help="only run a plain 1 GiB device copy (torch) three times, to scale the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.only_batched:
+        args.no_plain = args.no_cpu_baseline = args.no_configs = True
     if args.calibrate_copy:
         import torch
 
@@ -565,7 +570,7 @@ def main():
         log(f"plain: {1e3 * sec_p:.2f} ms/step")
     # the same batched step with every (tile, splat) pair sorted: sorted keys/s of the pair sort
     pair_sort = None
-    if args.binned:
+    if args.binned and not args.only_batched:
         ps_args = argparse.Namespace(**vars(args))
         ps_args.binned, ps_args.autotune, ps_args.steps, ps_args.warmup = 0, 0, 3, 1
         _, tim_s, ps_s, st_s, *_ = run_mode(sv.MODE_BATCHED, g, cams, targets, ps_args, graph=args.graph)
@@ -595,18 +600,24 @@ def main():
     if sq:
         valu = {"insts_per_launch": sq["waves"] * sq["valu_per_wave"],
                 "issue_frac": sq["waves"] * sq["valu_per_wave"] / (dom_ms * 1e-3 * VALU_ISSUE_PER_S),
+                "per_cu_cycle": sq.get("valu_per_cu_cycle"), "lane_util": sq.get("valu_lane_util"),
                 "active_frac": sq["valu_active_frac"], "waiting_frac": sq["waiting_frac"],
                 "model": "VALU wave-instructions / (launch time x 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles)"}
+    # the bound the counters show: the resource closer to its peak
+    hbm_frac = traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS if traffic else None
+    bound = "hbm"
+    if valu and (hbm_frac is None or valu["issue_frac"] > hbm_frac):
+        bound = "valu"
     limiter = None
     if traffic or sq:
         parts = []
-        if traffic:
-            parts.append(f"measured HBM {traffic / (dom_ms * 1e-3) / 1e9:.0f} GB/s = "
-                         f"{traffic / (dom_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS:.2f} of peak")
         if valu:
-            parts.append(f"VALU issue {valu['issue_frac']:.2f} of peak, waiting {valu['waiting_frac']:.2f} "
-                         f"of wave-cycles")
-        limiter = "latency (" + "; ".join(parts) + ")"
+            parts.append(f"VALU issue {valu['issue_frac']:.2f} of the unpacked-FP32 issue rate "
+                         f"({valu['per_cu_cycle']:.2f} instructions per CU-cycle, {valu['lane_util']:.2f} of lanes "
+                         f"active), waiting {valu['waiting_frac']:.2f} of wave-cycles")
+        if traffic:
+            parts.append(f"measured HBM {traffic / (dom_ms * 1e-3) / 1e9:.0f} GB/s = {hbm_frac:.2f} of peak")
+        limiter = "; ".join(parts)
     out = {
         "metric": METRIC,
         "value": 1.0 / sec_b,
@@ -641,8 +652,10 @@ def main():
         "step_b_alg_bytes": b_alg,
         "step_b_alg_GBps": b_alg / sec_b / 1e9,
         "step_b_alg_frac": b_alg / sec_b / 1e9 / HBM_PEAK_GBPS,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
+        "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "frac_basis": "algorithmic HBM bytes per launch / mean launch time, of 8 TB/s; the bound "
+                                   "is the resource the counters put nearest its peak",
                      "traffic_source": "profiles/hbm_traffic_1M_16x1080p.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                        "copy-calibrated)" if traffic else None,
                      "algorithmic_bytes_per_launch": dom_bytes, "mean_launch_ms": dom_ms,
