@@ -663,9 +663,12 @@ void orc_forward(const orc_params *p, const orc_proj *proj, const orc_camera *ca
         if (as < MIN_ALPHA) continue;
         const float oma = 1.0f - a;
         const float contrib = as * oma;
-        col[0] = col[0] + vc[0] * contrib;
-        col[1] = col[1] + vc[1] * contrib;
-        col[2] = col[2] + vc[2] * contrib;
+        /* the colour sums fused (WGSL leaves the contraction of
+         * col + c * contrib to the implementation; the library fuses it);
+         * the alpha chain, which decides the skips and the end, unfused */
+        col[0] = fmaf(vc[0], contrib, col[0]);
+        col[1] = fmaf(vc[1], contrib, col[1]);
+        col[2] = fmaf(vc[2], contrib, col[2]);
         a = a + contrib;
         if (1.0f - a < TRANSMITTANCE_THRESHOLD) break;
       }

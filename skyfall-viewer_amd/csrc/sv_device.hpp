@@ -114,6 +114,10 @@ constexpr uint32_t GEOM_MEAN_Y_FORWARD = 0x1u;  // SV_GEOM_MEAN_Y_FORWARD
 #define SV_MIN_ALPHA ((float)(1.0 / 255.0))
 #define SV_MAX_ALPHA 0.999f
 #define SV_T_EPS 1e-4f
+// 1 - A < 1e-4 exactly when A > SV_A_DONE, the largest float below the real
+// number 1 - 1e-4f (for A >= 1/2 the subtraction is exact, below it both
+// tests are false; checked over every float in [0, 2])
+#define SV_A_DONE 0x1.fff2e4p-1f
 
 // ---------------------------------------------------------------------
 // exp: Cody-Waite reduction by ln2 (split 0.693359375 - 2.12194440e-4) and
@@ -169,7 +173,16 @@ __device__ __forceinline__ float sv_exp_alpha(float x) {
   p = fmaf(p, f, 0x1.ebf9bcp-3f);
   p = fmaf(p, f, 0x1.62e42ap-1f);
   p = fmaf(p, f, 1.0f);
-  return p * __uint_as_float((uint32_t)((int)k + 127) << 23);
+  // p 2^k by ldexp: exact for every k of the argument range [-80, 1/2]
+  // (2^k normal), and it issues alongside the FP32 multiply-adds
+  return __builtin_ldexpf(p, (int)k);
+}
+
+// The splat's power 0.5 (a dx^2 + c dy^2) + b dx dy in the shader's order
+// (training-forward-tiles.wgsl:183-186). The halving is an ldexp by -1: the
+// same correctly rounded value as the multiply by 0.5, subnormals included.
+__device__ __forceinline__ float sv_power(float dx, float dy, float a, float b, float c) {
+  return __builtin_amdgcn_ldexpf(a * dx * dx + c * dy * dy, -1) + b * dx * dy;
 }
 
 // Logistic function as written at training-forward-tiles.wgsl:188.

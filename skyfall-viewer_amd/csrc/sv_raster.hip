@@ -342,7 +342,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
 __device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0, const float4 &x1, float *as) {
   const float dx = fx - x0.x;
   const float dy = fy - x0.y;
-  const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+  const float sg = sv_power(dx, dy, x0.z, x0.w, x1.x);
   const float gauss = sv_exp_alpha(-fminf(sg, 80.0f));
   *as = fminf(x1.y * gauss, SV_MAX_ALPHA);
   return !(sg < 0.0f) && !(*as < SV_MIN_ALPHA);
@@ -353,11 +353,13 @@ __device__ __forceinline__ bool frag_alpha(float fx, float fy, const float4 &x0,
 __device__ __forceinline__ void composite(float as, float r, float g, float b, uint32_t idx, float &c0, float &c1,
                                           float &c2, float &A, bool &done, uint32_t &last) {
   const float contrib = as * (1.0f - A);
-  c0 = c0 + r * contrib;
-  c1 = c1 + g * contrib;
-  c2 = c2 + b * contrib;
+  // the colour sums as fused multiply-adds (the alpha and transmittance
+  // chain, which decides every skip and the end, stays unfused)
+  c0 = fmaf(r, contrib, c0);
+  c1 = fmaf(g, contrib, c1);
+  c2 = fmaf(b, contrib, c2);
   A = A + contrib;
-  const bool fin = 1.0f - A < SV_T_EPS;  // selects, not a branch
+  const bool fin = A > SV_A_DONE;  // 1 - A < 1e-4; selects, not a branch
   done = done | fin;
   last = fin ? idx : last;
 }
@@ -663,7 +665,7 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               // lane (out-of-range lanes' values are discarded)
               const float dx = fxh[j & 1] - x0.x;
               const float dy = fyh[j >> 1] - x0.y;
-              const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+              const float sg = sv_power(dx, dy, x0.z, x0.w, x1.x);
               const float gauss = sv_exp_alpha(-fminf(sg, 80.0f));  // as frag_alpha
               const float op = x1.y;
               const float as = fminf(op * gauss, SV_MAX_ALPHA);
@@ -815,7 +817,7 @@ __global__ __launch_bounds__(RT) void k_backward_fixed(RasterArgs ain, const flo
         const float4 x0 = s_x0[k];
         const float4 x1 = s_x1[k];
         const float dx = fx - x0.x, dy = fy - x0.y;
-        const float sg = 0.5f * (x0.z * dx * dx + x1.x * dy * dy) + x0.w * dx * dy;
+        const float sg = sv_power(dx, dy, x0.z, x0.w, x1.x);
         if (sg < 0.0f) continue;
         const float gauss = sv_exp_alpha(-fminf(sg, 80.0f));
         const float op = x1.y;
