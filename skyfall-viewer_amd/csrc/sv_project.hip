@@ -113,6 +113,8 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
     const M3 C3D = mul(mul(transpose(mul(S, R)), S), R);
     const float logit = prm[P_LOGIT];
     const float opac = sv_sigmoid(logit);
+    // the power bound ln(255 opacity) + 0.02 depends on the Gaussian only
+    const float pmax = opac > 0.0f ? logf(255.0f * opac) + 0.02f : -1.0f;
     // SH coefficients once per Gaussian (re-reading them per view made the
     // kernel fetch 12x its parameters from memory)
     float shc[48];
@@ -175,9 +177,10 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           const float cc = c00 * det_inv;
           const float mx = (0.5f * (cx + 1.0f)) * (float)V.W;
           const float my = (0.5f * (cy + 1.0f)) * (float)V.H;
-          // tile rect, training-tiles.wgsl:161-171
-          const float ts = (float)cfg.tile_size;
-          const float tlx = mx / ts, tly = my / ts, trad = radius / ts;
+          // tile rect, training-tiles.wgsl:161-171; the tile size is 16, so
+          // the divisions are multiplications by 2^-4 (exact)
+          const float its = 1.0f / (float)cfg.tile_size;
+          const float tlx = mx * its, tly = my * its, trad = radius * its;
           uint32_t x0 = f2u_sat(floorf(tlx - trad)), y0 = f2u_sat(floorf(tly - trad));
           uint32_t x1 = f2u_sat(ceilf(tlx + trad)), y1 = f2u_sat(ceilf(tly + trad));
           x0 = min(x0, V.tiles_x);
@@ -204,7 +207,6 @@ __global__ __launch_bounds__(256) void k_project(const float *__restrict__ P, si
           }
           const uint32_t dq = f2u_sat(clampf(depth, 0.0f, 65535.0f)) & 0xffffu;
           if (count > 0u && live) local_max_dq = max(local_max_dq, dq);
-          const float pmax = opac > 0.0f ? logf(255.0f * opac) + 0.02f : -1.0f;
           const float qdet = aa * cc - bb * bb;
           float ex = -1.0f, ey = -1.0f;
           if (pmax > 0.0f && qdet > 0.0f) {

@@ -8,11 +8,12 @@ trustworthy. Each test names the oracle function it pins and is chosen so a
 plausible slip (a dropped term, a wrong sign, index or transposition) fails.
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
-from conftest import close_rel
+from conftest import ROOT, close_rel
 
 NO_LAST = 0xFFFFFFFF
 
@@ -368,3 +369,23 @@ def test_compaction_is_boolean_indexing(orc):
     src = rng.normal(size=(1000, 59)).astype(np.float32)
     keep = (rng.uniform(size=1000) < 0.7).astype(np.uint8)
     assert np.array_equal(orc.compact_rows(src, keep), src[keep.astype(bool)])
+
+
+def test_termination_threshold_is_the_reference_test():
+    """The forward ends a pixel when 1 - A < 1e-4 (training-forward-tiles.wgsl,
+    TRANSMITTANCE_THRESHOLD); the library tests A > SV_A_DONE instead (one
+    compare). Over every float A in [1/2, 2] the two tests agree, and below
+    1/2 both are false (1 - A rounds to at least 1/2 there; checked on a
+    sample). The constant is read from the device header."""
+    import re
+
+    hdr = open(os.path.join(ROOT, "skyfall-viewer_amd", "csrc", "sv_device.hpp")).read()
+    m = re.search(r"#define SV_A_DONE (0x[0-9a-fp.+-]+)f", hdr)
+    assert m, "SV_A_DONE not found"
+    thr = np.float32(float.fromhex(m.group(1)))
+    eps = np.float32(1e-4)
+    lo, hi = np.float32(0.5).view(np.uint32), np.float32(2.0).view(np.uint32)
+    a = np.arange(lo, hi + 1, dtype=np.uint32).view(np.float32)
+    assert np.array_equal((np.float32(1.0) - a) < eps, a > thr)
+    small = np.random.default_rng(9).uniform(0, 0.5, 1 << 20).astype(np.float32)
+    assert not ((np.float32(1.0) - small) < eps).any() and not (small > thr).any()
