@@ -407,6 +407,61 @@ def run_adjoint2m(args):
     print(json.dumps(out), flush=True)
 
 
+def visited_pairs(c, v: int, w: int, h: int) -> int:
+    """(pixel, splat) pairs the compositing of view v visits, from the last
+    forward's per-pixel last indices and the view's tile offsets: each pixel
+    walks its tile's list from the first entry to its last index."""
+    last = c.last_ids(v).ravel().astype(np.int64)
+    offs = c.tile_offsets(v).astype(np.int64)
+    tx = (w + 15) // 16
+    yy, xx = np.divmod(np.arange(w * h), w)
+    start = offs[(yy // 16) * tx + xx // 16]
+    ok = last != 0xFFFFFFFF
+    return int(np.sum(last[ok] - start[ok] + 1))
+
+
+def pairs_and_single_view(g, cams, targets, args, fwd_ms: float, bwd_ms: float):
+    """The secondary FLOP figure and the single-view plain point: pairs E =
+    visited (pixel, splat) pairs of a step (every view rendered on its own,
+    counted as visited_pairs), forward about 16 FLOP + 1 exp per pair and
+    backward about 45 + 1, over the batched launches' mean times, against
+    MI355X's 157.3 TFLOP/s FP32 vector peak (half of it unpacked); and the
+    reference method on view 0 alone."""
+    c = sv.Context(capacity=g["pos"].shape[0])
+    configure(c, args)
+    c.set_gaussians(g)
+    c.set_views(cams)
+    for v, t in enumerate(targets):
+        c.set_target(v, t)
+    E = 0
+    for v, cam in enumerate(cams):
+        c.forward(v)
+        w, h = int(round(cam.viewport[0])), int(round(cam.viewport[1]))
+        E += visited_pairs(c, v, w, h)
+    for _ in range(2):
+        c.step_views([0], sv.MODE_PLAIN)
+    c.sync()
+    t0 = time.perf_counter()
+    n = 5
+    for _ in range(n):
+        c.step_views([0], sv.MODE_PLAIN)
+    c.sync()
+    plain1 = (time.perf_counter() - t0) / n
+    c.close()
+    peak = 157.3e12
+    fw = 17.0 * E / (fwd_ms * 1e-3) if fwd_ms else None
+    bw = 46.0 * E / (bwd_ms * 1e-3) if bwd_ms else None
+    return {"visited_pairs_per_step": E,
+            "forward_flop_per_s": fw, "backward_flop_per_s": bw,
+            "forward_frac_of_fp32_peak": fw / peak if fw else None,
+            "backward_frac_of_fp32_peak": bw / peak if bw else None,
+            "model": "forward 16 FLOP + 1 exp, backward 45 FLOP + 1 exp per visited pair (the exp counted as 1); "
+                     "peak 157.3 TFLOP/s FP32 vector (packed; the kernels issue unpacked FP32)"}, {
+        "ms_per_step": 1e3 * plain1, "steps_per_s": 1.0 / plain1,
+        "workload": "view 0 alone, the reference's method (wavefront fixed-point backward, reduce, geometry "
+                    "backward, Adam)"}
+
+
 def make_workload(name: str, args):
     """(splats, cameras, sizes, width, height, description) of a workload."""
     N, V, W, H = args.n, args.views, args.width, args.height
@@ -662,6 +717,14 @@ def main():
                      "limiter": limiter, "valu": valu},
         "cpu_baseline": None,
     }
+    if args.workload == "ring16" and not args.only_batched:
+        pairs, single = pairs_and_single_view(g, cams, targets, args, phase.get("forward", (0,))[0],
+                                              phase.get("backward", (0,))[0])
+        out["pairs"] = pairs
+        out["plain_single_view"] = single
+        log(f"visited pairs {pairs['visited_pairs_per_step']}, forward {pairs['forward_flop_per_s'] / 1e12:.1f} "
+            f"TFLOP/s, backward {pairs['backward_flop_per_s'] / 1e12:.1f} TFLOP/s; plain single view "
+            f"{single['ms_per_step']:.2f} ms/step")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, one view)...")
         out["cpu_baseline"] = cpu_baseline(g, cams, targets, args)
