@@ -90,6 +90,11 @@ typedef struct sv_config {
   uint32_t geom_flags;      /* SV_GEOM_* flags of the geometry backward; 0 =
                                the reference's training_backward_geom as
                                written */
+  uint32_t split_backward;  /* 1: a batched step of two or more views runs
+                               its backward as two launches over halves of
+                               the views, the first half's record gather
+                               overlapping the second half's backward on a
+                               second stream (same results) */
 } sv_config;
 
 /* Step modes for sv_step. */

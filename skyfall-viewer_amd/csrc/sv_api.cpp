@@ -78,6 +78,10 @@ uint32_t bitlen(uint64_t v) {
 struct sv_ctx {
   int device = 0;
   hipStream_t st = nullptr;
+  // the split backward: the first views' record gather runs on st2 (forked
+  // and joined with the two events) while the other views' backward runs
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   size_t cap = 0;
   uint32_t N = 0;
   sv_config cfg{};
@@ -782,14 +786,15 @@ RasterArgs raster_args(sv_ctx *c) {
 // many workgroups as the device holds at once (occupancy x CUs), pulling
 // tiles from the per-XCD work queues.
 uint32_t raster_grid(sv_ctx *c, RasterArgs &a, bool backward) {
+  const uint32_t n = a.n_tiles_total - a.tile_begin;
   if (c->cfg.persistent) {
     const int bi = c->cfg.batch <= 64 ? 0 : (c->cfg.batch <= 128 ? 1 : 2);
     const uint32_t resident = c->resident[backward ? 1 : 0][bi];
     a.work_counter = c->d_work;
     launch_fill_u32(c->st, a.work_counter, 0u, (size_t)NUM_WORK_QUEUES * WORK_QUEUE_STRIDE);
-    return std::min<uint32_t>(c->tiles_total, resident ? resident : 2048u);
+    return std::min<uint32_t>(n, resident ? resident : 2048u);
   }
-  return c->tiles_total;
+  return n;
 }
 
 sv_status enqueue_forward(sv_ctx *c) {
@@ -868,6 +873,8 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.N = c->N;
   v.views = c->d_views;
   v.nv = (int)c->slot_views.size();
+  v.slot_begin = 0;
+  v.slot_count = v.nv;
   v.cfg = dev_config(c->cfg);
   v.counts = c->counts;
   v.offs = c->offs;
@@ -1068,13 +1075,56 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
 // the gradient planes + Adam when keep_grads). Adam parameters come from
 // `hp` or, when hp is null, from the mapped host block h_adam (re-read on
 // every replay).
+// The batched step's backward as two launches over two view ranges of
+// about half the tiles each (split at a view boundary); the first range's
+// record gather runs on the side stream while the second range's backward
+// runs (the gather of a range reads only the records and tile cuts that
+// range's backward wrote). Every slot's records are gathered when it returns.
+sv_status enqueue_split_backward(sv_ctx *c, bool use_res) {
+  const int nv = (int)c->slot_views.size();
+  int k = 1;
+  while (k < nv - 1 && c->h_slots[k].tile_base < c->tiles_total / 2) ++k;
+  const uint32_t tk = c->h_slots[k].tile_base;
+  {
+    Scoped t(c, PH_BACKWARD);
+    RasterArgs a = raster_args(c);
+    if (use_res) a.residual = c->residual;
+    a.n_tiles_total = tk;
+    launch_backward(c->st, a, c->cfg.batch, raster_grid(c, a, true));
+    HIP_OK(c, hipEventRecord(c->ev_fork, c->st));
+    HIP_OK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+    VjpArgs ga = vjp_args(c);
+    ga.slot_begin = 0;
+    ga.slot_count = k;
+    launch_gather_records(c->st2, ga);
+    RasterArgs b = raster_args(c);
+    if (use_res) b.residual = c->residual;
+    b.tile_begin = tk;
+    launch_backward(c->st, b, c->cfg.batch, raster_grid(c, b, true));
+    VjpArgs gb = vjp_args(c);
+    gb.slot_begin = k;
+    gb.slot_count = nv - k;
+    launch_gather_records(c->st, gb);
+    HIP_OK(c, hipEventRecord(c->ev_join, c->st2));
+    HIP_OK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
+  }
+  return check_launch(c, "split backward");
+}
+
 sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
   if (sv_status s = enqueue_build(c)) return s;
   if (sv_status s = enqueue_forward(c)) return s;
   const bool use_res = batch_needs_residual_buffer(c);
   if (use_res)
     if (sv_status s = loss_views(c)) return s;
-  if (sv_status s = enqueue_backward(c, use_res)) return s;
+  // (with the phase timers on, the backward stays one launch, so that its
+  // time is the kernel's)
+  const bool split = c->cfg.split_backward && c->slot_views.size() >= 2 && !c->cfg.keep_grads && !c->timing;
+  if (split) {
+    if (sv_status s = enqueue_split_backward(c, use_res)) return s;
+  } else if (sv_status s = enqueue_backward(c, use_res)) {
+    return s;
+  }
   launch_load_adam(c->st, hp ? *hp : AdamParams{}, hp ? nullptr : c->h_adam_dev, c->d_adam);
   VjpArgs va = vjp_args(c);
   if (c->cfg.keep_grads) {
@@ -1091,7 +1141,7 @@ sv_status enqueue_batched_step(sv_ctx *c, const AdamParams *hp) {
   va.geom = nullptr;
   {
     Scoped t(c, PH_VJP);
-    launch_gather_records(c->st, va);
+    if (!split) launch_gather_records(c->st, va);
     launch_vjp(c->st, va);
   }
   return check_launch(c, "vjp+adam");
@@ -1325,6 +1375,7 @@ sv_status sv_default_config(sv_config *o) {
   o->xcd_remap = 1;
   o->binned_build = 1;
   o->geom_flags = 0;
+  o->split_backward = 1;
   return SV_OK;
 }
 
@@ -1342,8 +1393,15 @@ sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
   sv_ctx *c = new sv_ctx();
   c->device = device;
   sv_default_config(&c->cfg);
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess) {
-    fail(nullptr, SV_E_HIP, "cannot create a stream on device %d", device);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->st) != hipSuccess ||
+      hipStreamCreate(&c->st2) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    fail(nullptr, SV_E_HIP, "cannot create streams and events on device %d", device);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->st2) (void)hipStreamDestroy(c->st2);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return SV_E_HIP;
   }
@@ -1353,6 +1411,9 @@ sv_status sv_create(int device, size_t capacity, sv_ctx **out) {
     const hipError_t e = noalloc_scratch(device, &p);
     if (e != hipSuccess) {
       fail(nullptr, SV_E_HIP, "scratch address on device %d: %s", device, hipGetErrorString(e));
+      (void)hipEventDestroy(c->ev_fork);
+      (void)hipEventDestroy(c->ev_join);
+      (void)hipStreamDestroy(c->st2);
       (void)hipStreamDestroy(c->st);
       delete c;
       return SV_E_HIP;
@@ -1465,6 +1526,10 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->fixed);
   dfree(c->fixed_inv);
   if (c->h_small) (void)hipHostFree(c->h_small);
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return SV_OK;
@@ -1482,6 +1547,7 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->xcd_remap > 1) return fail(c, SV_E_ARG, "xcd_remap must be 0 or 1");
   if (k->binned_build > 1) return fail(c, SV_E_ARG, "binned_build must be 0 or 1");
   static_assert(SV_GEOM_MEAN_Y_FORWARD == GEOM_MEAN_Y_FORWARD, "geometry flag values");
+  if (k->split_backward > 1) return fail(c, SV_E_ARG, "split_backward must be 0 or 1");
   if (k->geom_flags & ~SV_GEOM_MEAN_Y_FORWARD)
     return fail(c, SV_E_ARG, "geom_flags must be a combination of SV_GEOM_*");
   c->cfg = *k;

@@ -142,7 +142,8 @@ struct RasterArgs {
   uint64_t n_isect;
   const float4 *rec;          // 64-byte record per (slot, Gaussian)
   uint32_t N;                 // Gaussians (record stride per slot)
-  uint32_t n_tiles_total;
+  uint32_t n_tiles_total;     // end of this launch's tiles (global tile ids)
+  uint32_t tile_begin;        // first tile of this launch (0: all tiles)
   DevConfig cfg;
   // forward outputs / backward inputs (pixel arrays of the batch)
   float *color;     // 3 planes per view
@@ -188,6 +189,7 @@ struct VjpArgs {
   uint32_t N;
   const DevView *views;
   int nv;
+  int slot_begin, slot_count;  // the slots the record gather covers
   DevConfig cfg;
   const uint32_t *counts;  // per (slot, Gaussian)
   const uint32_t *offs;    // local prefix of the tile counts in the projection block

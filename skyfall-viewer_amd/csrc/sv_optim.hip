@@ -235,7 +235,8 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 // path; every Gaussian is updated, as the reference's Adam dispatches
 // cover all active elements).
 // ---------------------------------------------------------------------
-// Sum of the adjoint records of the reached pairs of one (slot, Gaussian),
+// Sum of the adjoint records of the reached pairs of one (slot, Gaussian)
+// (slots slot_begin .. slot_begin + slot_count - 1, one grid row each),
 // in emit order (ty-major over its tile rectangle): a pair at or past its
 // tile's cut (depth_q, id) was reached by no pixel and has no record, nor
 // has a pair outside the Gaussian's tight rectangle (no pixel there can
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   if (a.meta && a.meta->overflow) return;
   const size_t M = (size_t)a.nv * a.N;
   // grid: x over Gaussians, y over view slots
-  const uint32_t s = blockIdx.y, g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t s = (uint32_t)a.slot_begin + blockIdx.y, g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= a.N) return;
   const size_t vg = (size_t)s * a.N + g;
   float r[REC9];
@@ -327,7 +328,8 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
 void launch_gather_records(hipStream_t st, const VjpArgs &a) {
   const size_t M = (size_t)a.nv * a.N;
   if (M == 0) return;
-  hipLaunchKernelGGL(k_gather_records, dim3((a.N + 255) / 256, (uint32_t)a.nv), dim3(256), 0, st, a);
+  if (a.slot_count <= 0) return;
+  hipLaunchKernelGGL(k_gather_records, dim3((a.N + 255) / 256, (uint32_t)a.slot_count), dim3(256), 0, st, a);
 }
 
 // The per-Gaussian backward runs as two kernels with the same arithmetic

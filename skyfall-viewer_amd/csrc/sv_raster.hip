@@ -40,14 +40,15 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 // work items (the same parts as xcd_remap), its counter on a line of its
 // own (WORK_QUEUE_STRIDE words apart: no two XCDs' atomics share a cache
 // line). A workgroup pulls from its own XCD's queue and, once that is
-// drained, from the others in turn; when every queue is drained it gets n
-// (no item).
+// drained, from the others in turn; when every queue is drained it gets
+// n_tiles_total (no item). A launch covers tiles [tile_begin,
+// n_tiles_total).
 static_assert(NUM_WORK_QUEUES == NUM_XCD, "one work queue per XCD");
 
 // Called by the whole wave (the raster workgroups are one wave): lane 0 does
 // the atomics, the item is read from it, every value stays scalar.
 __device__ __forceinline__ uint32_t pull_item(const RasterArgs &a) {
-  const uint32_t n = a.n_tiles_total, q = n / NUM_XCD, r = n % NUM_XCD;
+  const uint32_t n = a.n_tiles_total - a.tile_begin, q = n / NUM_XCD, r = n % NUM_XCD;
   const uint32_t home = blockIdx.x % NUM_XCD;
 #pragma unroll 1
   for (uint32_t d = 0; d < NUM_XCD; ++d) {
@@ -56,13 +57,14 @@ __device__ __forceinline__ uint32_t pull_item(const RasterArgs &a) {
     uint32_t t = 0;
     if (threadIdx.x == 0) t = atomicAdd(a.work_counter + x * WORK_QUEUE_STRIDE, 1u);
     t = __builtin_amdgcn_readfirstlane(t);
-    if (t < len) return x * q + min(x, r) + t;
+    if (t < len) return a.tile_begin + x * q + min(x, r) + t;
   }
-  return n;
+  return a.n_tiles_total;
 }
 
 __device__ __forceinline__ uint32_t first_tile(const RasterArgs &a) {
-  if (a.work_counter == nullptr) return a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  if (a.work_counter == nullptr)
+    return a.tile_begin + (a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
   return pull_item(a);
 }
 
