@@ -12,8 +12,9 @@
  *  - Every function returns an sv_status (0 = SV_OK) unless stated.
  *  - Pointers are caller-owned HOST memory, copied in or out before the call
  *    returns. No device pointer crosses the ABI.
- *  - One sv_ctx owns one HIP stream on one device. A context is not
- *    thread-safe; calls block until the stream is idle.
+ *  - One sv_ctx owns one HIP stream on one device (and a side stream that
+ *    the batched step forks work onto and joins). A context is not
+ *    thread-safe; calls block until the streams are idle.
  *  - Gaussian parameters use the reference's f32 training layout
  *    (gaussian-renderer.ts:130-137): pos[3N], log-scale[3N], quaternion
  *    (x,y,z,w)[4N], opacity logit[N], SH[48N] (coefficient k, channel c at
