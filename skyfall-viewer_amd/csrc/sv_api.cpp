@@ -110,7 +110,6 @@ struct sv_ctx {
   size_t blocks_cap = 0;
   float *l1_partial = nullptr;
   float *tile_l1 = nullptr;  // densify scoring, per global tile
-  uint64_t *tile_cut = nullptr;  // per global tile: first unreached entry's (depth_q, id)
   size_t tiles_cap = 0;
   std::vector<int> scored_views;  // batch of the last sv_densify_decide
   std::vector<uint32_t> scored_base;
@@ -139,6 +138,9 @@ struct sv_ctx {
   uint32_t *sort_tmp = nullptr;
   size_t sort_tmp_cap = 0;
   float *isect_rec = nullptr;
+  uint8_t *rflag = nullptr;  // per intersection (emit order): the backward wrote its record
+  uint8_t *qmask = nullptr;  // per sorted entry: its quadrant mask, written by the forward
+  bool qmask_ready = false;  // the last forward wrote qmask
   int32_t *fixed = nullptr;
   uint32_t *fixed_inv = nullptr;  // emit position -> sorted position (plain mode)
   uint64_t fixed_cap = 0;
@@ -428,7 +430,6 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       dfree(c->ranges);
       dfree(c->l1_partial);
       dfree(c->tile_l1);
-      dfree(c->tile_cut);
       dfree(c->diff);
       c->tiles_cap = 0;
       c->scored_views.clear();
@@ -437,7 +438,6 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       if (sv_status s = dalloc(c, &c->ranges, want)) return s;
       if (sv_status s = dalloc(c, &c->l1_partial, want)) return s;
       if (sv_status s = dalloc(c, &c->tile_l1, want)) return s;
-      if (sv_status s = dalloc(c, &c->tile_cut, want)) return s;
       if (sv_status s = dalloc(c, &c->diff, want)) return s;
       c->tiles_cap = want;
     }
@@ -579,14 +579,19 @@ sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
   dfree(c->vals);
   dfree(c->vals2);
   dfree(c->isect_rec);
+  dfree(c->rflag);
+  dfree(c->qmask);
   dfree(c->sort_tmp);
   c->isect_cap = 0;
+  c->qmask_ready = false;
   c->sort_tmp_cap = 0;
   if (sv_status s = dalloc(c, (uint64_t **)&c->keys, want)) return s;
   if (sv_status s = dalloc(c, (uint64_t **)&c->keys2, want)) return s;
   if (sv_status s = dalloc(c, &c->vals, want)) return s;
   if (sv_status s = dalloc(c, &c->vals2, want)) return s;
   if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC9)) return s;
+  if (sv_status s = dalloc(c, &c->rflag, flag_array_bytes(want))) return s;
+  if (sv_status s = dalloc(c, &c->qmask, (size_t)want)) return s;
   const size_t stw = sort_tmp_words<uint64_t>(want);
   if (sv_status s = dalloc(c, &c->sort_tmp, stw)) return s;
   c->sort_tmp_cap = stw;
@@ -773,8 +778,9 @@ RasterArgs raster_args(sv_ctx *c) {
   a.bbase = c->bbase;
   a.nb = project_blocks(c->N);
   a.isect_rec = c->isect_rec;
+  a.rflag = c->rflag;
+  a.qmask = nullptr;
   a.aux = c->aux;
-  a.tile_cut = c->tile_cut;
   a.l1_partial = c->l1_partial;
   a.work_counter = nullptr;
   a.block_view = c->block_view;
@@ -799,6 +805,11 @@ uint32_t raster_grid(sv_ctx *c, RasterArgs &a, bool backward) {
 
 sv_status enqueue_forward(sv_ctx *c) {
   RasterArgs a = raster_args(c);
+  // quadrant masks for the backward; an 8-px raster wave renders one
+  // quadrant of a tile, stops when that quadrant is done and writes none
+  // (the backward then culls by itself)
+  a.qmask = c->cfg.raster_px != 8 ? c->qmask : nullptr;
+  c->qmask_ready = a.qmask != nullptr;
   // 8- and 32-px raster tiles launch a workgroup per quadrant / block
   const uint32_t grid = c->cfg.raster_px == 16 ? raster_grid(c, a, false) : c->tiles_total;
   {
@@ -816,12 +827,21 @@ sv_status forward_batch(sv_ctx *c) {
 }
 
 // use_residual: read the residual arena instead of the fused loss.
-sv_status enqueue_backward(sv_ctx *c, bool use_residual) {
+// The backward's raster arguments: the forward's quadrant masks when it
+// wrote them.
+RasterArgs backward_args(sv_ctx *c, bool use_residual) {
   RasterArgs a = raster_args(c);
   if (use_residual) a.residual = c->residual;
+  a.qmask = c->qmask_ready ? c->qmask : nullptr;
+  return a;
+}
+
+sv_status enqueue_backward(sv_ctx *c, bool use_residual) {
+  RasterArgs a = backward_args(c, use_residual);
   const uint32_t grid = raster_grid(c, a, true);
   {
     Scoped t(c, PH_BACKWARD);
+    launch_clear_flags(c->st, c->rflag, c->isect_cap, c->d_meta);
     launch_backward(c->st, a, c->cfg.batch, grid);
   }
   return check_launch(c, "backward");
@@ -885,7 +905,7 @@ VjpArgs vjp_args(sv_ctx *c) {
   v.tins = c->tins;
   v.aux = c->aux;
   v.isect_rec = c->isect_rec;
-  v.tile_cut = c->tile_cut;
+  v.rflag = c->rflag;
   v.vrec = c->vrec;
   v.importance = c->importance;
   v.G = c->G;
@@ -1078,8 +1098,9 @@ sv_status backward_fixed_batch(sv_ctx *c, uint32_t flags, bool stats) {
 // The batched step's backward as two launches over two view ranges of
 // about half the tiles each (split at a view boundary); the first range's
 // record gather runs on the side stream while the second range's backward
-// runs (the gather of a range reads only the records and tile cuts that
-// range's backward wrote). Every slot's records are gathered when it returns.
+// runs (the gather of a range reads only the records and flags that range's
+// backward wrote; the flags of every range are cleared before the first
+// launch). Every slot's records are gathered when it returns.
 sv_status enqueue_split_backward(sv_ctx *c, bool use_res) {
   const int nv = (int)c->slot_views.size();
   int k = 1;
@@ -1087,9 +1108,9 @@ sv_status enqueue_split_backward(sv_ctx *c, bool use_res) {
   const uint32_t tk = c->h_slots[k].tile_base;
   {
     Scoped t(c, PH_BACKWARD);
-    RasterArgs a = raster_args(c);
-    if (use_res) a.residual = c->residual;
+    RasterArgs a = backward_args(c, use_res);
     a.n_tiles_total = tk;
+    launch_clear_flags(c->st, c->rflag, c->isect_cap, c->d_meta);
     launch_backward(c->st, a, c->cfg.batch, raster_grid(c, a, true));
     HIP_OK(c, hipEventRecord(c->ev_fork, c->st));
     HIP_OK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
@@ -1097,8 +1118,7 @@ sv_status enqueue_split_backward(sv_ctx *c, bool use_res) {
     ga.slot_begin = 0;
     ga.slot_count = k;
     launch_gather_records(c->st2, ga);
-    RasterArgs b = raster_args(c);
-    if (use_res) b.residual = c->residual;
+    RasterArgs b = backward_args(c, use_res);
     b.tile_begin = tk;
     launch_backward(c->st, b, c->cfg.batch, raster_grid(c, b, true));
     VjpArgs gb = vjp_args(c);
@@ -1153,7 +1173,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
   std::vector<uint64_t> g;
   const void *ptrs[] = {c->P, c->M1, c->M2, c->G, c->geom, c->importance, c->grad_accum, c->color,
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
-                        c->l1_partial, c->tile_cut, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
+                        c->l1_partial, c->rflag, c->qmask, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
                         c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
@@ -1480,7 +1500,6 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->ranges);
   dfree(c->l1_partial);
   dfree(c->tile_l1);
-  dfree(c->tile_cut);
   dfree(c->rec);
   dfree(c->aux);
   dfree(c->counts);
@@ -1500,6 +1519,8 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->vals2);
   dfree(c->sort_tmp);
   dfree(c->isect_rec);
+  dfree(c->rflag);
+  dfree(c->qmask);
   dfree(c->scounts);
   dfree(c->soffs);
   dfree(c->rects);

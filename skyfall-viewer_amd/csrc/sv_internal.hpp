@@ -155,10 +155,15 @@ struct RasterArgs {
   const uint32_t *bbase;  // scanned block totals of the tile counts (emit positions)
   uint32_t nb;            // projection blocks per slot
   float *isect_rec;       // 9-float adjoint record per intersection (emit order)
-  // per (slot, Gaussian) depth and radius; per global tile, the sort key
-  // (depth_q << 32 | Gaussian) of the first entry no pixel reaches
-  const float2 *aux;
-  uint64_t *tile_cut;
+  // per intersection (emit order): 1 where the backward wrote a record,
+  // 0 (cleared before the backward) elsewhere
+  uint8_t *rflag;
+  // per sorted entry: the quadrants of its tile (bit h + 2 v) that the
+  // entry's power <= power_max region reaches (quad_mask). The forward
+  // writes it for every entry it loads (null: not written); the backward,
+  // when not null, reads it instead of culling again.
+  uint8_t *qmask;
+  const float2 *aux;      // per (slot, Gaussian) depth and radius
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent launch: NUM_WORK_QUEUES counters,
                           // WORK_QUEUE_STRIDE words apart (null: one block per tile)
@@ -200,7 +205,7 @@ struct VjpArgs {
   const uint32_t *tins;    // tight-rectangle insets per record (tight_insets)
   const float2 *aux;       // (depth, radius) per record (radius for the importance stats)
   const float *isect_rec;  // 9-float adjoint record per intersection
-  const uint64_t *tile_cut; // per global tile: records at or past this (depth_q, id) were not written
+  const uint8_t *rflag;    // per intersection: 1 where the backward wrote a record
   float *vrec;             // 9 planes of nv * N: per (slot, Gaussian) sum of its reached records
   float4 *importance;      // importance statistics (may be null)
   float *G;                // gradient planes (accumulate), or null when fused
@@ -267,6 +272,11 @@ void launch_densify(hipStream_t st, const float *P, const float *M1, const float
 void launch_u8_to_u32(hipStream_t st, const uint8_t *a, uint32_t *o, uint32_t N, int mode);
 void launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, size_t n);
 void launch_fill_f32(hipStream_t st, float *p, float v, size_t n);
+// Zero the first m->n_isect bytes of a per-intersection flag array of cap
+// bytes, rounded up to whole 16-byte words (the array holds
+// flag_array_bytes(cap)).
+inline size_t flag_array_bytes(uint64_t cap) { return (size_t)((cap + 15) / 16 * 16); }
+void launch_clear_flags(hipStream_t st, uint8_t *flags, uint64_t cap, const StepMeta *m);
 
 // sv_score.hip ----------------------------------------------------------
 // Multi-view scoring (generateMultiViewScores) of the rendered batch.

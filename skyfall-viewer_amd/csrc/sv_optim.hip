@@ -235,12 +235,11 @@ __device__ __forceinline__ int tensor_of_plane(int k) {
 // path; every Gaussian is updated, as the reference's Adam dispatches
 // cover all active elements).
 // ---------------------------------------------------------------------
-// Sum of the adjoint records of the reached pairs of one (slot, Gaussian)
+// Sum of the adjoint records of the taken pairs of one (slot, Gaussian)
 // (slots slot_begin .. slot_begin + slot_count - 1, one grid row each),
-// in emit order (ty-major over its tile rectangle): a pair at or past its
-// tile's cut (depth_q, id) was reached by no pixel and has no record, nor
-// has a pair outside the Gaussian's tight rectangle (no pixel there can
-// take it); the walk covers the tight rectangle only.
+// in emit order (ty-major over its tile rectangle): only a pair some pixel
+// took has a record, and its flag says so; no pair outside the Gaussian's
+// tight rectangle can be taken, so the walk covers that rectangle only.
 __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   if (a.meta && a.meta->overflow) return;
   const size_t M = (size_t)a.nv * a.N;
@@ -253,12 +252,9 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
   const uint32_t cnt = a.counts[vg];
   if (cnt) {
-    const DevView &V = a.views[s];
     const uint2 rc = a.rects[vg];
     const uint32_t ra = rc.x, rb = rc.y;
     const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
-    const uint32_t dq = f2u_sat(clampf(a.aux[vg].x, 0.0f, 65535.0f)) & 0xffffu;
-    const uint64_t me = ((uint64_t)dq << 32) | g;
     const TightRect tr = tight_rect(a.tins[vg], x0, y0, x1, y1);
     const uint32_t w = x1 - x0;
     const uint32_t tw = tr.tx1 > tr.tx0 ? (uint32_t)(tr.tx1 - tr.tx0) : 0u;
@@ -266,11 +262,12 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     // entry (dx, dy) of the tight rectangle is record (oy + dy) w + ox + dx
     // of the Gaussian's rectangle
     const uint32_t ox = n ? (uint32_t)tr.tx0 - x0 : 0u, oy = n ? (uint32_t)tr.ty0 - y0 : 0u;
-    const float *rp = a.isect_rec + ((size_t)block_pos(a.bbase, a.nb, s, g, a.offs[vg]) + oy * w + ox) * REC9;
-    const uint64_t *cut0 = a.tile_cut + V.tile_base + (size_t)(y0 + oy) * V.tiles_x + (x0 + ox);
+    const size_t e0 = (size_t)block_pos(a.bbase, a.nb, s, g, a.offs[vg]) + oy * w + ox;
+    const float *rp = a.isect_rec + e0 * REC9;
+    const uint8_t *fp = a.rflag + e0;
     bool first = true;
-    // four tight-rectangle entries per round: their cut and record loads are
-    // issued together, then folded in emit order
+    // four tight-rectangle entries per round: their flag and record loads
+    // are issued together, then folded in emit order
     constexpr uint32_t U = 4;
     // (cx, cy): column and row of entry k0, stepped without divisions
     uint32_t cx = 0, cy = 0;
@@ -286,7 +283,7 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
           cx = 0;
           ++cy;
         }
-        ok[u] = k0 + u < n && me < cut0[ey[u] * V.tiles_x + ex[u]];
+        ok[u] = k0 + u < n && fp[ey[u] * w + ex[u]] != 0u;
       }
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u) {

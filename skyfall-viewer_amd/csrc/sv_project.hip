@@ -554,6 +554,21 @@ void launch_fill_f32(hipStream_t st, float *p, float v, size_t n) {
   hipLaunchKernelGGL(k_fill_f32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
 }
 
+// A kernel, not a memset: the step is captured into a graph.
+__global__ __launch_bounds__(256) void k_clear_flags(uint4 *__restrict__ f, const StepMeta *m) {
+  if (m->overflow) return;
+  const uint64_t words = (m->n_isect + 15) / 16;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride)
+    f[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+void launch_clear_flags(hipStream_t st, uint8_t *flags, uint64_t cap, const StepMeta *m) {
+  const uint64_t words = flag_array_bytes(cap) / 16;
+  if (words == 0) return;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((words + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_clear_flags, dim3(blocks), dim3(256), 0, st, (uint4 *)flags, m);
+}
+
 template <typename K>
 void launch_ranges(hipStream_t st, const K *keys0, const K *keys1, uint64_t cap, uint32_t n_tiles,
                    const StepMeta *m, uint32_t *ranges) {

@@ -193,12 +193,6 @@ __device__ __forceinline__ uint32_t emit_pos(const float4 &q3, uint32_t base, ui
   return base + __float_as_uint(q3.x) + (ty - y0) * (x1 - x0) + (tx - x0);
 }
 
-// Load up to B entries of a tile list from `first` (count cnt) into LDS,
-// keeping only those whose power <= power_max region (axis-aligned bounds
-// ext_x, ext_y around the mean) overlaps the pixel centres of some quadrant
-// of the tile; kept entries stay in list order. s_k holds the entry's index
-// in the batch (bits 0..15) and the mask of quadrants it overlaps (bits
-// 16..19). Returns how many were kept.
 // Pixel-centre bounds of the quadrants of a tile: x of quadrant columns
 // 0/1, y of quadrant rows 0/1 (an empty range has lo > hi).
 struct QuadBounds {
@@ -283,11 +277,18 @@ __device__ __forceinline__ uint32_t quad_mask(const float4 &q0, float c, const f
   return qm;
 }
 
+// Load up to B entries of a tile list from `first` (count cnt) into LDS,
+// keeping only those whose power <= power_max region (axis-aligned bounds
+// ext_x, ext_y around the mean) overlaps the pixel centres of some quadrant
+// of `wq`; kept entries stay in list order. s_k holds the entry's index
+// in the batch (bits 0..15) and the mask of quadrants it overlaps (bits
+// 16..19), which also goes to qmask[first + k] for every loaded entry when
+// qmask is not null. Returns how many were kept.
 template <int B>
 __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
                                                uint32_t cnt, const QuadBounds &qb, uint32_t wq,
                                                float4 *s_q0, float4 *s_q1, float4 *s_q2, uint32_t *s_k,
-                                               uint32_t *s_p, uint32_t slot, uint32_t tx, uint32_t ty) {
+                                               uint8_t *qmask) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t kept = 0;
@@ -295,7 +296,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
     const uint32_t k = r + lane;
     bool touch = false;
     uint32_t qm = 0;
-    float4 q0, q1, q2, q3;
+    float4 q0, q1, q2;
     if (k < cnt) {
       const uint32_t id = min(a.vals[first + k], a.N - 1u);
       const float4 *q = a.rec + (vbase + id) * 4;
@@ -304,23 +305,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       q2 = q[2];
       qm = quad_mask(q0, q1.x, q2, qb);
       touch = (qm & wq) != 0;
-      if (s_p) {
-        q3 = q[3];
-        const uint32_t p = emit_pos(q3, a.bbase[(size_t)slot * a.nb + (id >> 8)], tx, ty);
-        s_p[k] = p;
-        // an entry that touches no pixel of the tile contributes nothing:
-        // a zero record inside the Gaussian's tight rectangle (the record
-        // gather reads only there), no record outside it
-        const uint32_t ru = __float_as_uint(q3.y), rw = __float_as_uint(q3.z);
-        const TightRect tr = tight_rect(__float_as_uint(q3.w), ru & 0xffffu, ru >> 16, rw & 0xffffu, rw >> 16);
-        const bool in_tight = (int32_t)tx >= tr.tx0 && (int32_t)tx < tr.tx1 && (int32_t)ty >= tr.ty0 &&
-                              (int32_t)ty < tr.ty1;
-        if (!touch && in_tight && (uint64_t)p < a.n_isect) {
-          float *o = a.isect_rec + (size_t)p * REC9;
-#pragma unroll
-          for (int q9 = 0; q9 < REC9; ++q9) o[q9] = 0.0f;
-        }
-      }
+      if (qmask) qmask[first + k] = (uint8_t)qm;
     }
     const uint64_t m = __ballot(touch);
     if (touch) {
@@ -329,6 +314,51 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       s_q1[pos] = q1;
       s_q2[pos] = q2;
       s_k[pos] = k | (qm << 16);
+    }
+    kept += (uint32_t)__popcll(m);
+  }
+  return kept;
+}
+
+// The backward's batch: up to B entries of a tile list from `first` (count
+// cnt), in list order: those quad_mask keeps, with its quadrants. With the
+// forward's quadrant masks (a.qmask) the masks are read and only the kept
+// entries' records loaded; without them every record is loaded and culled
+// here. s_k as in load_batch; s_p holds the kept entry's emit position.
+// Returns how many were kept.
+template <int B>
+__device__ __forceinline__ uint32_t load_batch_bwd(const RasterArgs &a, size_t vbase, uint32_t first,
+                                                   uint32_t cnt, const QuadBounds &qb, float4 *s_q0,
+                                                   float4 *s_q1, float4 *s_q2, uint32_t *s_k, uint32_t *s_p,
+                                                   uint32_t slot, uint32_t tx, uint32_t ty) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t kept = 0;
+  for (uint32_t r = 0; r < cnt; r += WT) {
+    const uint32_t k = r + lane;
+    uint32_t qm = 0, id = 0;
+    float4 q0, q1, q2, q3;
+    if (k < cnt) {
+      if (a.qmask) qm = a.qmask[first + k];
+      if (!a.qmask || qm) {
+        id = min(a.vals[first + k], a.N - 1u);
+        const float4 *q = a.rec + (vbase + id) * 4;
+        q0 = q[0];
+        q1 = q[1];
+        q2 = q[2];
+        q3 = q[3];
+        if (!a.qmask) qm = quad_mask(q0, q1.x, q2, qb);
+      }
+    }
+    const bool touch = qm != 0u;
+    const uint64_t m = __ballot(touch);
+    if (touch) {
+      const uint32_t pos = kept + (uint32_t)__popcll(m & lt_mask);
+      s_q0[pos] = q0;
+      s_q1[pos] = q1;
+      s_q2[pos] = q2;
+      s_k[pos] = k | (qm << 16);
+      s_p[pos] = emit_pos(q3, a.bbase[(size_t)slot * a.nb + (id >> 8)], tx, ty);
     }
     kept += (uint32_t)__popcll(m);
   }
@@ -437,9 +467,12 @@ __device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, u
   }
   // the quadrant bits of an entry's word that this wave renders
   const uint32_t wmask = wq << 16;
+  // the quadrant masks of the loaded entries, for the backward (only a wave
+  // that renders the whole tile loads every entry any pixel of it reaches)
+  uint8_t *const qmask = wq == 0xFu ? a.qmask : nullptr;
   for (uint32_t b0 = start; b0 < end; b0 += B) {
     const uint32_t cnt = min((uint32_t)B, end - b0);
-    const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, wq, L.q0, L.q1, L.q2, L.k, nullptr, 0, 0, 0);
+    const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, wq, L.q0, L.q1, L.q2, L.k, qmask);
     wave_sync();
     // Two entries per iteration: their alphas are independent (only the
     // compositing that follows is ordered), so the two exp chains
@@ -550,13 +583,13 @@ __global__ __launch_bounds__(4 * WT, 7) void k_forward32(RasterArgs ain) {
 // dL/dlogit, and the moments of q = -dL/dpower over the pixel offsets
 // (dx, dy): sums of q dx^2, q dx dy, q dy^2, q dx, q dy, from which the
 // record gather forms dL/dconic abc and dL/dmean xy) over the tile's 256
-// pixels (in
-// registers over each lane's 4 pixels, then a fixed butterfly
+// pixels (in registers over each lane's 4 pixels, then a fixed butterfly
 // reduce-scatter over the wave for eight of them and a DPP tree for the
-// ninth) and writes
-// one 36-byte record at the intersection's emit position. The SH chain is
-// applied per Gaussian afterwards (exact: the SH basis depends only on the
-// Gaussian and the view). Deterministic.
+// ninth) and, for an entry some pixel took, writes one 36-byte record at
+// the intersection's emit position and sets its flag. With the forward's
+// quadrant masks it does not cull the entries again. The SH chain
+// is applied per Gaussian afterwards (exact: the SH basis depends only on
+// the Gaussian and the view). Deterministic.
 // The residual is fused (w_l2 (p - t) + w_l1 sign(p - t)) or read from
 // a.residual.
 // ---------------------------------------------------------------------
@@ -626,26 +659,14 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
       }
       const float l1t = wave_total63(l1);
       if (lane == 63 && a.l1_partial) a.l1_partial[tg] = l1t;
+      // entries from tend on are reached by no pixel
       uint32_t tend = wave_max_u32(m);
       if (tend < start) tend = start;
-      // Entries from tend on are reached by no pixel: their records are not
-      // written, and the VJP skips every record at or past this tile's cut
-      // (the sort key (depth_q, id) of entry tend; the list is ordered by it)
-      if (lane == 0) {
-        uint64_t cut = ~0ull;
-        if (tend < end) {
-          // the depth quantum of the sort key, as emit_intersections forms it
-          const uint32_t g = a.vals[tend];
-          const uint32_t dq = f2u_sat(clampf(a.aux[vbase + min(g, a.N - 1u)].x, 0.0f, 65535.0f)) & 0xffffu;
-          cut = ((uint64_t)dq << 32) | g;
-        }
-        a.tile_cut[tg] = cut;
-      }
       const QuadBounds qb = quad_bounds(t, V);
       for (uint32_t hi = tend; hi > start;) {
         const uint32_t lo = (hi - start > (uint32_t)B) ? hi - B : start;
         const uint32_t cnt = hi - lo;
-        const uint32_t kept = load_batch<B>(a, vbase, lo, cnt, qb, 0xFu, s_q0, s_q1, s_q2, s_k, s_p, t.s, t.tx, t.ty);
+        const uint32_t kept = load_batch_bwd<B>(a, vbase, lo, cnt, qb, s_q0, s_q1, s_q2, s_k, s_p, t.s, t.tx, t.ty);
         __syncthreads();
         for (int e = (int)kept - 1; e >= 0; --e) {
           // same word on every lane: the quadrant tests below are uniform
@@ -700,18 +721,20 @@ __global__ __launch_bounds__(WT, 5) void k_backward(RasterArgs ain) {
               T[j] = Tb;
             }
           }
-          // the entry's record at its emit position: the sums over the
-          // tile's 256 pixels, or zeros. Components 0..7 end in lanes 8c,
-          // component 8 in lane 63; one store of nine lanes.
-          const uint32_t p = s_p[k];
-          float out = 0.0f;
+          // an entry some pixel took: its record at its emit position, the
+          // sums over the tile's 256 pixels (components 0..7 end in lanes
+          // 8c, component 8 in lane 63; one store of nine lanes), and its
+          // flag. No other entry writes either: the record gather reads a
+          // record only where the flag is set.
           if (__any(hit)) {
+            const uint32_t p = s_p[e];
             const float s07 = wave_scatter8(v0, v1, v2, v3, v4, v5, v6, v7);
             const float s8 = wave_total63(v8);
-            out = lane == 63 ? s8 : s07;
+            if (((lane & 7) == 0 || lane == 63) && (uint64_t)p < a.n_isect) {
+              a.isect_rec[(size_t)p * REC9 + (lane == 63 ? 8u : (uint32_t)lane >> 3)] = lane == 63 ? s8 : s07;
+              if (lane == 0) a.rflag[p] = 1u;
+            }
           }
-          if (((lane & 7) == 0 || lane == 63) && (uint64_t)p < a.n_isect)
-            a.isect_rec[(size_t)p * REC9 + (lane == 63 ? 8u : (uint32_t)lane >> 3)] = out;
         }
         __syncthreads();
         hi = lo;
