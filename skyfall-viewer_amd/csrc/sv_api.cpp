@@ -139,8 +139,8 @@ struct sv_ctx {
   size_t sort_tmp_cap = 0;
   float *isect_rec = nullptr;
   uint8_t *rflag = nullptr;  // per intersection (emit order): the backward wrote its record
-  uint8_t *qmask = nullptr;  // per sorted entry: its quadrant mask, written by the forward
-  bool qmask_ready = false;  // the last forward wrote qmask
+  uint8_t *tmask = nullptr;  // per sorted entry: its quadrant mask, written by the forward
+  bool tmask_ready = false;  // the last forward wrote tmask
   int32_t *fixed = nullptr;
   uint32_t *fixed_inv = nullptr;  // emit position -> sorted position (plain mode)
   uint64_t fixed_cap = 0;
@@ -580,10 +580,10 @@ sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
   dfree(c->vals2);
   dfree(c->isect_rec);
   dfree(c->rflag);
-  dfree(c->qmask);
+  dfree(c->tmask);
   dfree(c->sort_tmp);
   c->isect_cap = 0;
-  c->qmask_ready = false;
+  c->tmask_ready = false;
   c->sort_tmp_cap = 0;
   if (sv_status s = dalloc(c, (uint64_t **)&c->keys, want)) return s;
   if (sv_status s = dalloc(c, (uint64_t **)&c->keys2, want)) return s;
@@ -591,7 +591,7 @@ sv_status ensure_isect_capacity(sv_ctx *c, uint64_t n) {
   if (sv_status s = dalloc(c, &c->vals2, want)) return s;
   if (sv_status s = dalloc(c, &c->isect_rec, (size_t)want * REC9)) return s;
   if (sv_status s = dalloc(c, &c->rflag, flag_array_bytes(want))) return s;
-  if (sv_status s = dalloc(c, &c->qmask, (size_t)want)) return s;
+  if (sv_status s = dalloc(c, &c->tmask, (size_t)want)) return s;
   const size_t stw = sort_tmp_words<uint64_t>(want);
   if (sv_status s = dalloc(c, &c->sort_tmp, stw)) return s;
   c->sort_tmp_cap = stw;
@@ -779,7 +779,7 @@ RasterArgs raster_args(sv_ctx *c) {
   a.nb = project_blocks(c->N);
   a.isect_rec = c->isect_rec;
   a.rflag = c->rflag;
-  a.qmask = nullptr;
+  a.tmask = nullptr;
   a.aux = c->aux;
   a.l1_partial = c->l1_partial;
   a.work_counter = nullptr;
@@ -808,8 +808,8 @@ sv_status enqueue_forward(sv_ctx *c) {
   // quadrant masks for the backward; an 8-px raster wave renders one
   // quadrant of a tile, stops when that quadrant is done and writes none
   // (the backward then culls by itself)
-  a.qmask = c->cfg.raster_px != 8 ? c->qmask : nullptr;
-  c->qmask_ready = a.qmask != nullptr;
+  a.tmask = c->cfg.raster_px != 8 ? c->tmask : nullptr;
+  c->tmask_ready = a.tmask != nullptr;
   // 8- and 32-px raster tiles launch a workgroup per quadrant / block
   const uint32_t grid = c->cfg.raster_px == 16 ? raster_grid(c, a, false) : c->tiles_total;
   {
@@ -832,7 +832,7 @@ sv_status forward_batch(sv_ctx *c) {
 RasterArgs backward_args(sv_ctx *c, bool use_residual) {
   RasterArgs a = raster_args(c);
   if (use_residual) a.residual = c->residual;
-  a.qmask = c->qmask_ready ? c->qmask : nullptr;
+  a.tmask = c->tmask_ready ? c->tmask : nullptr;
   return a;
 }
 
@@ -1173,7 +1173,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
   std::vector<uint64_t> g;
   const void *ptrs[] = {c->P, c->M1, c->M2, c->G, c->geom, c->importance, c->grad_accum, c->color,
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
-                        c->l1_partial, c->rflag, c->qmask, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
+                        c->l1_partial, c->rflag, c->tmask, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
                         c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
@@ -1520,7 +1520,7 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->sort_tmp);
   dfree(c->isect_rec);
   dfree(c->rflag);
-  dfree(c->qmask);
+  dfree(c->tmask);
   dfree(c->scounts);
   dfree(c->soffs);
   dfree(c->rects);

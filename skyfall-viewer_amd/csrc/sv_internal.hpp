@@ -162,7 +162,7 @@ struct RasterArgs {
   // entry's power <= power_max region reaches (quad_mask). The forward
   // writes it for every entry it loads (null: not written); the backward,
   // when not null, reads it instead of culling again.
-  uint8_t *qmask;
+  uint8_t *tmask;
   const float2 *aux;      // per (slot, Gaussian) depth and radius
   float *l1_partial;      // per-tile sum |pred - target| (may be null)
   uint32_t *work_counter; // persistent launch: NUM_WORK_QUEUES counters,

@@ -282,13 +282,13 @@ __device__ __forceinline__ uint32_t quad_mask(const float4 &q0, float c, const f
 // ext_x, ext_y around the mean) overlaps the pixel centres of some quadrant
 // of `wq`; kept entries stay in list order. s_k holds the entry's index
 // in the batch (bits 0..15) and the mask of quadrants it overlaps (bits
-// 16..19), which also goes to qmask[first + k] for every loaded entry when
-// qmask is not null. Returns how many were kept.
+// 16..19). With zmask, zmask[first + k] = 0 for every entry not kept.
+// Returns how many were kept.
 template <int B>
 __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase, uint32_t first,
                                                uint32_t cnt, const QuadBounds &qb, uint32_t wq,
                                                float4 *s_q0, float4 *s_q1, float4 *s_q2, uint32_t *s_k,
-                                               uint8_t *qmask) {
+                                               uint8_t *zmask) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint32_t kept = 0;
@@ -305,7 +305,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
       q2 = q[2];
       qm = quad_mask(q0, q1.x, q2, qb);
       touch = (qm & wq) != 0;
-      if (qmask) qmask[first + k] = (uint8_t)qm;
+      if (zmask && !touch) zmask[first + k] = 0;
     }
     const uint64_t m = __ballot(touch);
     if (touch) {
@@ -322,7 +322,7 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
 
 // The backward's batch: up to B entries of a tile list from `first` (count
 // cnt), in list order: those quad_mask keeps, with its quadrants. With the
-// forward's quadrant masks (a.qmask) the masks are read and only the kept
+// forward's quadrant masks (a.tmask) the masks are read and only the kept
 // entries' records loaded; without them every record is loaded and culled
 // here. s_k as in load_batch; s_p holds the kept entry's emit position.
 // Returns how many were kept.
@@ -339,15 +339,15 @@ __device__ __forceinline__ uint32_t load_batch_bwd(const RasterArgs &a, size_t v
     uint32_t qm = 0, id = 0;
     float4 q0, q1, q2, q3;
     if (k < cnt) {
-      if (a.qmask) qm = a.qmask[first + k];
-      if (!a.qmask || qm) {
+      if (a.tmask) qm = a.tmask[first + k];
+      if (!a.tmask || qm) {
         id = min(a.vals[first + k], a.N - 1u);
         const float4 *q = a.rec + (vbase + id) * 4;
         q0 = q[0];
         q1 = q[1];
         q2 = q[2];
         q3 = q[3];
-        if (!a.qmask) qm = quad_mask(q0, q1.x, q2, qb);
+        if (!a.tmask) qm = quad_mask(q0, q1.x, q2, qb);
       }
     }
     const bool touch = qm != 0u;
@@ -404,6 +404,8 @@ template <int B>
 struct FwdLds {
   float4 q0[B], q1[B], q2[B];
   uint32_t k[B];
+  // byte j of word e: some pixel of quadrant j composited kept entry e
+  uint32_t t[B];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -467,12 +469,21 @@ __device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, u
   }
   // the quadrant bits of an entry's word that this wave renders
   const uint32_t wmask = wq << 16;
-  // the quadrant masks of the loaded entries, for the backward (only a wave
-  // that renders the whole tile loads every entry any pixel of it reaches)
-  uint8_t *const qmask = wq == 0xFu ? a.qmask : nullptr;
+  // Taken masks, for the backward: per entry, the quadrants where some pixel
+  // composited it. Every lane that composites kept entry e in quadrant j
+  // stores 1 into byte j of L.t[e] (all of them the same value; always, as
+  // a branch would cost more than the store), and after the batch every
+  // loaded entry's mask goes to its list position (0 for the entries culled
+  // here or left when every pixel was done). Only a wave that renders the
+  // whole tile writes them: it loads every entry any pixel of the tile
+  // reaches.
+  uint8_t *const tm = wq == 0xFu ? a.tmask : nullptr;
+  uint8_t *const lt = reinterpret_cast<uint8_t *>(L.t);
+  const uint32_t lane = threadIdx.x & 63u;
   for (uint32_t b0 = start; b0 < end; b0 += B) {
     const uint32_t cnt = min((uint32_t)B, end - b0);
-    const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, wq, L.q0, L.q1, L.q2, L.k, qmask);
+    const uint32_t kept = load_batch<B>(a, vbase, b0, cnt, qb, wq, L.q0, L.q1, L.q2, L.k, tm);
+    for (uint32_t x = lane; x < kept; x += WT) L.t[x] = 0u;
     wave_sync();
     // Two entries per iteration: their alphas are independent (only the
     // compositing that follows is ordered), so the two exp chains
@@ -492,9 +503,14 @@ __device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, u
         float asa, asb;
         const bool va = ua && frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa);
         const bool vb = ub && frag_alpha(fxh[j & 1], fyh[j >> 1], b0r, b1, &asb);
-        if (va) composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
-        if (vb && !done[j])
+        if (va) {
+          composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+          lt[4u * e + j] = 1u;
+        }
+        if (vb && !done[j]) {
           composite(asb, b1.z, b1.w, b2.x, b0 + (kb & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+          lt[4u * e + 4u + j] = 1u;
+        }
       }
       if (__all((done[0] && done[1]) && (done[2] && done[3]))) break;
     }
@@ -508,10 +524,18 @@ __device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, u
         float asa;
         if (!frag_alpha(fxh[j & 1], fyh[j >> 1], a0, a1, &asa)) continue;
         composite(asa, a1.z, a1.w, a2.x, b0 + (ka & 0xFFFFu), c0[j], c1[j], c2[j], A[j], done[j], last[j]);
+        lt[4u * e + j] = 1u;
       }
     }
     // a wave vote, then the sync that protects the LDS batch
     const bool all_done = __all((done[0] && done[1]) && (done[2] && done[3]));
+    if (tm) {
+      wave_sync();
+      for (uint32_t x = lane; x < kept; x += WT) {
+        const uint32_t w = L.t[x];
+        tm[b0 + (L.k[x] & 0xFFFFu)] = (uint8_t)((w & 1u) | ((w >> 7) & 2u) | ((w >> 14) & 4u) | ((w >> 21) & 8u));
+      }
+    }
     wave_sync();
     if (all_done) break;
   }

@@ -7,8 +7,8 @@ of 32 (partial blocks and quadrants at the edges), images, alpha and last
 indices are identical bit for bit, images match the oracle within 1e-5 and
 alpha exactly, and a batched training step (graph launch) leaves identical
 parameters: after an 8-px forward the backward culls every entry itself,
-after a 16- or 32-px forward it reads the quadrant masks the forward wrote,
-and both visit the same entries and quadrants. The
+after a 16- or 32-px forward it visits only the entries and quadrants the
+forward's taken masks name, and both write the same records. The
 autotuner times all three raster tiles and keeps the configuration it
 reports as fastest.
 """
