@@ -139,7 +139,7 @@ struct sv_ctx {
   size_t sort_tmp_cap = 0;
   float *isect_rec = nullptr;
   uint8_t *rflag = nullptr;  // per intersection (emit order): the backward wrote its record
-  uint8_t *tmask = nullptr;  // per sorted entry: its quadrant mask, written by the forward
+  uint8_t *tmask = nullptr;  // per sorted entry: the quadrants that took it in the forward
   bool tmask_ready = false;  // the last forward wrote tmask
   int32_t *fixed = nullptr;
   uint32_t *fixed_inv = nullptr;  // emit position -> sorted position (plain mode)
@@ -805,9 +805,9 @@ uint32_t raster_grid(sv_ctx *c, RasterArgs &a, bool backward) {
 
 sv_status enqueue_forward(sv_ctx *c) {
   RasterArgs a = raster_args(c);
-  // quadrant masks for the backward; an 8-px raster wave renders one
-  // quadrant of a tile, stops when that quadrant is done and writes none
-  // (the backward then culls by itself)
+  // taken masks for the backward; an 8-px raster wave renders one quadrant
+  // of a tile, stops when that quadrant is done and writes none (the
+  // backward then culls by itself)
   a.tmask = c->cfg.raster_px != 8 ? c->tmask : nullptr;
   c->tmask_ready = a.tmask != nullptr;
   // 8- and 32-px raster tiles launch a workgroup per quadrant / block
@@ -827,8 +827,8 @@ sv_status forward_batch(sv_ctx *c) {
 }
 
 // use_residual: read the residual arena instead of the fused loss.
-// The backward's raster arguments: the forward's quadrant masks when it
-// wrote them.
+// The backward's raster arguments: the forward's taken masks when it wrote
+// them.
 RasterArgs backward_args(sv_ctx *c, bool use_residual) {
   RasterArgs a = raster_args(c);
   if (use_residual) a.residual = c->residual;

@@ -158,10 +158,10 @@ struct RasterArgs {
   // per intersection (emit order): 1 where the backward wrote a record,
   // 0 (cleared before the backward) elsewhere
   uint8_t *rflag;
-  // per sorted entry: the quadrants of its tile (bit h + 2 v) that the
-  // entry's power <= power_max region reaches (quad_mask). The forward
-  // writes it for every entry it loads (null: not written); the backward,
-  // when not null, reads it instead of culling again.
+  // per sorted entry: the quadrants of its tile (bit h + 2 v) where some
+  // pixel composited the entry. The forward writes it for every entry it
+  // loads (null: not written); the backward, when not null, visits only
+  // the entries and quadrants it names instead of culling.
   uint8_t *tmask;
   const float2 *aux;      // per (slot, Gaussian) depth and radius
   float *l1_partial;      // per-tile sum |pred - target| (may be null)

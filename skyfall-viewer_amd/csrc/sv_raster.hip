@@ -321,11 +321,11 @@ __device__ __forceinline__ uint32_t load_batch(const RasterArgs &a, size_t vbase
 }
 
 // The backward's batch: up to B entries of a tile list from `first` (count
-// cnt), in list order: those quad_mask keeps, with its quadrants. With the
-// forward's quadrant masks (a.tmask) the masks are read and only the kept
-// entries' records loaded; without them every record is loaded and culled
-// here. s_k as in load_batch; s_p holds the kept entry's emit position.
-// Returns how many were kept.
+// cnt), in list order. With the forward's taken masks (a.tmask) the entries
+// some pixel took, with the quadrants where one did (only their records are
+// loaded); without them every record is loaded and the entries and
+// quadrants quad_mask keeps are kept. s_k as in load_batch; s_p holds the
+// kept entry's emit position. Returns how many were kept.
 template <int B>
 __device__ __forceinline__ uint32_t load_batch_bwd(const RasterArgs &a, size_t vbase, uint32_t first,
                                                    uint32_t cnt, const QuadBounds &qb, float4 *s_q0,
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(4 * WT, 7) void k_forward32(RasterArgs ain) {
 // reduce-scatter over the wave for eight of them and a DPP tree for the
 // ninth) and, for an entry some pixel took, writes one 36-byte record at
 // the intersection's emit position and sets its flag. With the forward's
-// quadrant masks it does not cull the entries again. The SH chain
+// taken masks it visits only the taken entries and quadrants. The SH chain
 // is applied per Gaussian afterwards (exact: the SH basis depends only on
 // the Gaussian and the view). Deterministic.
 // The residual is fused (w_l2 (p - t) + w_l1 sign(p - t)) or read from
