@@ -77,6 +77,7 @@ def lib():
         L.orc_count_isects.restype = C.c_uint64
         L.orc_render_and_grad.restype = C.c_int64
         L.orc_forward_loss_f64.restype = C.c_double
+        L.orc_forward_loss_f64_only.restype = C.c_double
         L.orc_compact_rows.restype = C.c_size_t
         _lib = L
     return _lib
@@ -268,6 +269,17 @@ def forward_loss_f64(g: dict, cam, view: OrcView, target, w_l2, w_l1, want_color
     val = lib().orc_forward_loss_f64(C.c_size_t(n), *[_dp(a) for a in arrs], C.byref(oc), C.byref(view), _fp(t),
                                      C.c_double(w_l2), C.c_double(w_l1), _dp(col) if col is not None else None)
     return (val, col) if want_color else val
+
+
+def forward_loss_f64_only(g: dict, cam, view: OrcView, target, w_l2, w_l1, only: int):
+    """The FP64 loss over the pixels of Gaussian `only`'s tile rectangle (for
+    central differences of that Gaussian's SH coefficients)."""
+    n = g["pos"].shape[0]
+    arrs = [np.ascontiguousarray(g[k], np.float64) for k in ("pos", "logscale", "quat", "logit", "sh")]
+    t = np.ascontiguousarray(target, np.float32)
+    oc = camera(cam)
+    return lib().orc_forward_loss_f64_only(C.c_size_t(n), *[_dp(a) for a in arrs], C.byref(oc), C.byref(view),
+                                           _fp(t), C.c_double(w_l2), C.c_double(w_l1), C.c_size_t(only))
 
 
 def compact_rows(src: np.ndarray, keep: np.ndarray) -> np.ndarray:

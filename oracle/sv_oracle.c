@@ -29,7 +29,10 @@
  *   orc_loss            numpy closed form
  *   orc_backward_legacy torch float64 autograd of the compositing model
  *                       (conic, mean2d, logit, colour), and central finite
- *                       differences of orc_forward_loss_f64 for SH/logit
+ *                       differences of orc_forward_loss_f64 for the logit
+ *                       and every SH coefficient of 256 Gaussians
+ *   orc_forward_loss_f64_only  its central differences for an SH change
+ *                       equal orc_forward_loss_f64's
  *   orc_backward_wavefront  equals orc_backward_legacy (to fixed-point
  *                       truncation) where every pixel list fits one batch
  *                       and alpha <= 0.98; integer sums brute force
@@ -1268,10 +1271,12 @@ static int cmp_isect_d(const void *a, const void *b) {
   return (x[1] > y[1]) - (x[1] < y[1]);
 }
 
-double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
-                            const double *quat, const double *logit, const double *sh,
-                            const orc_camera *cam, const orc_view *v, const float *target,
-                            double w_l2, double w_l1, double *color_out) {
+/* The FP64 forward and loss; with only < n the loss sums the pixels of
+ * Gaussian `only`'s tile rectangle alone (0 when it is culled). */
+static double forward_loss_f64(size_t n, const double *pos, const double *logscale,
+                               const double *quat, const double *logit, const double *sh,
+                               const orc_camera *cam, const orc_view *v, const float *target,
+                               double w_l2, double w_l1, double *color_out, size_t only) {
   dproj *pr = (dproj *)calloc(n ? n : 1, sizeof(dproj));
   const double W = v->width, H = v->height, ts = v->tile_size;
   uint64_t total = 0;
@@ -1358,9 +1363,22 @@ double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
   }
   double loss = 0;
   const double bg[3] = {v->bg[0], v->bg[1], v->bg[2]};
+  // pixel window: the whole image, or Gaussian `only`'s tile rectangle
+  uint32_t wx0 = 0, wy0 = 0, wx1 = v->width, wy1 = v->height;
+  if (only < n) {
+    const dproj *o = &pr[only];
+    const uint32_t t = v->tile_size;
+    wx0 = wy0 = wx1 = wy1 = 0;
+    if (o->alive) {
+      wx0 = o->rect[0] * t;
+      wy0 = o->rect[1] * t;
+      wx1 = o->rect[2] * t < v->width ? o->rect[2] * t : v->width;
+      wy1 = o->rect[3] * t < v->height ? o->rect[3] * t : v->height;
+    }
+  }
 #pragma omp parallel for reduction(+ : loss) schedule(dynamic, 4)
-  for (int64_t gy = 0; gy < (int64_t)v->height; ++gy)
-    for (uint32_t gx = 0; gx < v->width; ++gx) {
+  for (int64_t gy = wy0; gy < (int64_t)wy1; ++gy)
+    for (uint32_t gx = wx0; gx < wx1; ++gx) {
       const double px = gx + 0.5, py = gy + 0.5;
       const uint32_t tid = (uint32_t)(py / ts) * v->tiles_x + (uint32_t)(px / ts);
       double col[3] = {0, 0, 0}, a = 0;
@@ -1389,6 +1407,20 @@ double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
   free(isect);
   free(pr);
   return loss;
+}
+
+double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
+                            const double *quat, const double *logit, const double *sh,
+                            const orc_camera *cam, const orc_view *v, const float *target,
+                            double w_l2, double w_l1, double *color_out) {
+  return forward_loss_f64(n, pos, logscale, quat, logit, sh, cam, v, target, w_l2, w_l1, color_out, (size_t)-1);
+}
+
+double orc_forward_loss_f64_only(size_t n, const double *pos, const double *logscale,
+                                 const double *quat, const double *logit, const double *sh,
+                                 const orc_camera *cam, const orc_view *v, const float *target,
+                                 double w_l2, double w_l1, size_t only) {
+  return forward_loss_f64(n, pos, logscale, quat, logit, sh, cam, v, target, w_l2, w_l1, NULL, only);
 }
 
 size_t orc_compact_rows(size_t n, size_t stride, const float *src, const uint8_t *keep,

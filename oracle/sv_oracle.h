@@ -163,6 +163,16 @@ double orc_forward_loss_f64(size_t n, const double *pos, const double *logscale,
                             const orc_view *v, const float *target,
                             double w_l2, double w_l1, double *color_out);
 
+/* The same loss summed over the pixels of Gaussian `only`'s tile rectangle
+ * alone (0 when it is culled). A change of that Gaussian's SH coefficients
+ * leaves its projection, every tile list and every other pixel unchanged,
+ * so the central difference of this sum is that of the whole loss. */
+double orc_forward_loss_f64_only(size_t n, const double *pos, const double *logscale,
+                                 const double *quat, const double *logit,
+                                 const double *sh, const orc_camera *cam,
+                                 const orc_view *v, const float *target,
+                                 double w_l2, double w_l1, size_t only);
+
 /* Stable compaction (gaussian-renderer.ts:2917-3000): rows with keep[i] != 0
  * are copied in order. Returns the kept count. `stride` floats per row. */
 size_t orc_compact_rows(size_t n, size_t stride, const float *src,

@@ -10,12 +10,16 @@
   orc_backward_geom      float64 torch autograd of the covariance / mean model
                          the shader differentiates (training-backward-geom.wgsl)
   SH / logit gradients   central finite differences of the FP64 CPU forward
+                         (the logit and every one of the 48 SH coefficients
+                         of 256 Gaussians)
   geometry gradients     finite-difference gap printed, not enforced: the
                          reference differentiates a model that differs from its
                          forward (transposed view in the position chain, the
                          conic-b factor 2, no quaternion normalisation Jacobian)
 """
 import math
+import multiprocessing
+import os
 
 import numpy as np
 import pytest
@@ -359,6 +363,92 @@ def test_sh_and_logit_gradients_match_central_fd_of_fp64_forward(sv, orc):
     err = np.abs(ana - f2)[smooth].max() / scale
     print(f"[pin] FD SH/logit: {smooth.sum()}/{len(rows)} smooth points, max err {err:.2e} of max|g| {scale:.3e}")
     assert smooth.mean() >= 0.9
+    assert err < 1e-4
+
+
+# state the worker processes of the per-coefficient check read (set by
+# _fd_init in each worker; each worker perturbs its own copy in place)
+_FD = {}
+
+
+def _fd_init(g64, cam, view, target):
+    import oracle
+
+    _FD.update(orc=oracle, g64=g64, cam=cam, view=view, target=target)
+
+
+def _fd_sh_gaussian(i):
+    """Central differences of the FP64 forward's loss for each of the 48 SH
+    coefficients of Gaussian i, at h = 1e-3 and h / 2. The loss is summed over
+    the pixels of the Gaussian's tile rectangle only: the other pixels' terms
+    do not depend on its colour and cancel in the difference."""
+    orc, g64, cam, view, target = _FD["orc"], _FD["g64"], _FD["cam"], _FD["view"], _FD["target"]
+    sh = g64["sh"]
+    out = np.zeros((48, 2))
+    for k in range(48):
+        x0 = sh[i, k]
+        for col, h in enumerate((1e-3, 5e-4)):
+            sh[i, k] = x0 + h
+            fp = orc.forward_loss_f64_only(g64, cam, view, target, 0.8, 0.2, i)
+            sh[i, k] = x0 - h
+            fm = orc.forward_loss_f64_only(g64, cam, view, target, 0.8, 0.2, i)
+            sh[i, k] = x0
+            out[k, col] = (fp - fm) / (2 * h)
+    return out
+
+
+def test_sh_gradients_match_central_fd_per_coefficient(sv, orc):
+    """Every parameter's derivative, not a projection: for each of the 48 SH
+    coefficients of the 256 Gaussians of the test above (configuration 2's
+    scene, seed 3), the central difference (h = 1e-3) of the FP64 forward's
+    loss against the oracle's float32 gradient. As above, points where h and
+    h / 2 disagree (an L1 sign or colour clamp switching inside the step) are
+    left out, at most 10 %. The loss of each evaluation is summed over the
+    Gaussian's tile rectangle (checked against the whole image's difference
+    on three coefficients); the 49k evaluations run in worker processes. They are spawned, not forked:
+    this process has loaded the HIP runtime (the scene generator lives in
+    libsv), and a forked child of it stalls."""
+    g, cam, view, target = _scene(sv, orc, 4096, 64, 64)
+    r = orc.render_and_grad(g, cam, view, target=target, w_l2=0.8, w_l1=0.2)
+    ana = r["grads"]["sh"]
+    live = np.flatnonzero(r["proj"]["tiles_count"] > 0)
+    pick = np.random.default_rng(3).choice(live, 256, replace=False)
+    g64 = {k: v.astype(np.float64) for k, v in g.items()}
+    # the rectangle-restricted loss differences equal the whole image's
+    for i, k in ((int(pick[0]), 0), (int(pick[1]), 17), (int(pick[2]), 47)):
+        d = {}
+        for sign in (1, -1):
+            gd = {key: v.copy() for key, v in g64.items()}
+            gd["sh"][i, k] += sign * 1e-3
+            d[sign] = (orc.forward_loss_f64(gd, cam, view, target, 0.8, 0.2),
+                       orc.forward_loss_f64_only(gd, cam, view, target, 0.8, 0.2, i))
+        full, part = d[1][0] - d[-1][0], d[1][1] - d[-1][1]
+        assert abs(full - part) <= 1e-9 * max(abs(full), 1e-12) + 1e-13, (i, k, full, part)
+        assert d[1][1] < d[1][0]  # a strict part of the image
+    # one OpenMP thread per worker (the forward's sort is OpenMP-parallel;
+    # eight workers of eight threads each would oversubscribe the cores)
+    saved = os.environ.get("OMP_NUM_THREADS")
+    os.environ["OMP_NUM_THREADS"] = "1"
+    try:
+        with multiprocessing.get_context("spawn").Pool(max(1, min(8, os.cpu_count() or 1)), initializer=_fd_init,
+                                                       initargs=(g64, cam, view, target)) as pool:
+            fd = np.stack(pool.map(_fd_sh_gaussian, [int(i) for i in pick], chunksize=4))
+    finally:
+        if saved is None:
+            del os.environ["OMP_NUM_THREADS"]
+        else:
+            os.environ["OMP_NUM_THREADS"] = saved
+    a = ana[pick].astype(np.float64)  # (256, 48)
+    f1, f2 = fd[:, :, 0], fd[:, :, 1]
+    scale = np.abs(a).max()
+    smooth = np.abs(f1 - f2) <= 1e-5 * scale
+    err = np.abs(a - f2)[smooth].max() / scale
+    # every coefficient index is checked somewhere, and on most Gaussians
+    per_coeff = smooth.mean(axis=0)
+    print(f"[pin] FD per SH coefficient: {smooth.sum()}/{smooth.size} smooth points, max err {err:.2e} of "
+          f"max|g| {scale:.3e}; fewest smooth points of one coefficient {per_coeff.min():.2f}")
+    assert smooth.mean() >= 0.9
+    assert per_coeff.min() >= 0.5
     assert err < 1e-4
 
 
