@@ -96,6 +96,11 @@ typedef struct sv_config {
                                the views, the first half's record gather
                                overlapping the second half's backward on a
                                second stream (same results) */
+  uint32_t tile_order;      /* 1 (default): the 16-px forward and backward
+                               take each XCD's tiles longest list first (a
+                               per-step bucket sort of the tiles by list
+                               length), so the longest tiles do not start
+                               last; same results */
 } sv_config;
 
 /* Step modes for sv_step. */

@@ -104,6 +104,10 @@ struct sv_ctx {
   size_t d_views_cap = 0;
   uint32_t tiles_total = 0;
   uint32_t *tile_view = nullptr, *ranges = nullptr;
+  // longest-first work orders of the forward and of the backward launches
+  uint32_t *order_f = nullptr, *order_b = nullptr;
+  // per tile: the entries some pixel visited in the last forward
+  uint32_t *tile_cost = nullptr;
   // 2x2-tile blocks of the batch (32-px raster tiles): count, block -> slot
   uint32_t blocks_total = 0;
   uint32_t *block_view = nullptr;
@@ -141,6 +145,7 @@ struct sv_ctx {
   uint8_t *rflag = nullptr;  // per intersection (emit order): the backward wrote its record
   uint8_t *tmask = nullptr;  // per sorted entry: the quadrants that took it in the forward
   bool tmask_ready = false;  // the last forward wrote tmask
+  bool cost_ready = false;  // the last forward wrote tile_cost
   int32_t *fixed = nullptr;
   uint32_t *fixed_inv = nullptr;  // emit position -> sorted position (plain mode)
   uint64_t fixed_cap = 0;
@@ -431,6 +436,9 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       dfree(c->l1_partial);
       dfree(c->tile_l1);
       dfree(c->diff);
+      dfree(c->order_f);
+      dfree(c->order_b);
+      dfree(c->tile_cost);
       c->tiles_cap = 0;
       c->scored_views.clear();
       size_t want = (size_t)tb + 1;
@@ -439,6 +447,9 @@ sv_status prepare_batch(sv_ctx *c, const int *vs, int nv) {
       if (sv_status s = dalloc(c, &c->l1_partial, want)) return s;
       if (sv_status s = dalloc(c, &c->tile_l1, want)) return s;
       if (sv_status s = dalloc(c, &c->diff, want)) return s;
+      if (sv_status s = dalloc(c, &c->order_f, want)) return s;
+      if (sv_status s = dalloc(c, &c->order_b, want)) return s;
+      if (sv_status s = dalloc(c, &c->tile_cost, want)) return s;
       c->tiles_cap = want;
     }
     if (c->super_tab_cap < (size_t)sb + 1) {
@@ -803,6 +814,10 @@ uint32_t raster_grid(sv_ctx *c, RasterArgs &a, bool backward) {
   return n;
 }
 
+// Work items of a 16-px raster launch go to the XCDs in contiguous ranges
+// (xcd_remap, the persistent queues) or round-robin in item order.
+bool order_contiguous(const sv_ctx *c) { return c->cfg.persistent || c->cfg.xcd_remap; }
+
 sv_status enqueue_forward(sv_ctx *c) {
   RasterArgs a = raster_args(c);
   // taken masks for the backward; an 8-px raster wave renders one quadrant
@@ -812,8 +827,16 @@ sv_status enqueue_forward(sv_ctx *c) {
   c->tmask_ready = a.tmask != nullptr;
   // 8- and 32-px raster tiles launch a workgroup per quadrant / block
   const uint32_t grid = c->cfg.raster_px == 16 ? raster_grid(c, a, false) : c->tiles_total;
+  // the work order (16-px tiles) and, for the backward's, each tile's
+  // visited entries (written by waves that render whole tiles)
+  if (c->cfg.tile_order) a.tile_cost = c->tile_cost;
+  c->cost_ready = c->cfg.tile_order && c->cfg.raster_px != 8;
   {
     Scoped t(c, PH_FORWARD);
+    if (c->cfg.tile_order && c->cfg.raster_px == 16) {
+      launch_tile_order(c->st, c->ranges, nullptr, c->d_meta, 0u, c->tiles_total, order_contiguous(c), c->order_f);
+      a.tile_order = c->order_f;
+    }
     launch_forward(c->st, a, c->cfg.batch, grid, c->cfg.raster_px);
   }
   return check_launch(c, "forward");
@@ -842,6 +865,13 @@ sv_status enqueue_backward(sv_ctx *c, bool use_residual) {
   {
     Scoped t(c, PH_BACKWARD);
     launch_clear_flags(c->st, c->rflag, c->isect_cap, c->d_meta);
+    // longest first by the entries the forward visited (by list length
+    // when it wrote no costs)
+    if (c->cfg.tile_order) {
+      launch_tile_order(c->st, c->ranges, c->cost_ready ? c->tile_cost : nullptr, c->d_meta, 0u, c->tiles_total,
+                        order_contiguous(c), c->order_b);
+      a.tile_order = c->order_b;
+    }
     launch_backward(c->st, a, c->cfg.batch, grid);
   }
   return check_launch(c, "backward");
@@ -1111,6 +1141,15 @@ sv_status enqueue_split_backward(sv_ctx *c, bool use_res) {
     RasterArgs a = backward_args(c, use_res);
     a.n_tiles_total = tk;
     launch_clear_flags(c->st, c->rflag, c->isect_cap, c->d_meta);
+    // each range's work order over its own items (the XCD ranges of a
+    // launch follow its size)
+    if (c->cfg.tile_order) {
+      const uint32_t *cost = c->cost_ready ? c->tile_cost : nullptr;
+      launch_tile_order(c->st, c->ranges, cost, c->d_meta, 0u, tk, order_contiguous(c), c->order_b);
+      launch_tile_order(c->st, c->ranges, cost, c->d_meta, tk, c->tiles_total - tk, order_contiguous(c),
+                        c->order_b + tk);
+      a.tile_order = c->order_b;
+    }
     launch_backward(c->st, a, c->cfg.batch, raster_grid(c, a, true));
     HIP_OK(c, hipEventRecord(c->ev_fork, c->st));
     HIP_OK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
@@ -1120,6 +1159,7 @@ sv_status enqueue_split_backward(sv_ctx *c, bool use_res) {
     launch_gather_records(c->st2, ga);
     RasterArgs b = backward_args(c, use_res);
     b.tile_begin = tk;
+    if (c->cfg.tile_order) b.tile_order = c->order_b + tk;
     launch_backward(c->st, b, c->cfg.batch, raster_grid(c, b, true));
     VjpArgs gb = vjp_args(c);
     gb.slot_begin = k;
@@ -1175,7 +1215,7 @@ std::vector<uint64_t> graph_signature(const sv_ctx *c) {
                         c->alpha, c->residual, c->target, c->last, c->d_views, c->tile_view, c->ranges,
                         c->l1_partial, c->rflag, c->tmask, c->rec, c->aux, c->counts, c->offs, c->vrec, c->scan_tmp, c->keys,
                         c->keys2, c->vals, c->vals2, c->sort_tmp, c->isect_rec, c->d_meta, c->d_adam,
-                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff,
+                        c->h_adam_dev, c->scounts, c->soffs, c->rects, c->tins, c->btot, c->sbtot, c->bbase, c->sbbase, c->super_view, c->sranges, c->diff, c->order_f, c->order_b, c->tile_cost,
                         c->skeys, c->skeys2, c->svals, c->svals2, c->ssort_tmp, c->nch, c->cbase,
                         c->chunk_super, c->ccount, c->d_work, c->block_view};
   for (const void *p : ptrs) g.push_back((uint64_t)(uintptr_t)p);
@@ -1396,6 +1436,7 @@ sv_status sv_default_config(sv_config *o) {
   o->binned_build = 1;
   o->geom_flags = 0;
   o->split_backward = 1;
+  o->tile_order = 1;
   return SV_OK;
 }
 
@@ -1500,6 +1541,9 @@ sv_status sv_destroy(sv_ctx *c) {
   dfree(c->ranges);
   dfree(c->l1_partial);
   dfree(c->tile_l1);
+  dfree(c->order_f);
+  dfree(c->order_b);
+  dfree(c->tile_cost);
   dfree(c->rec);
   dfree(c->aux);
   dfree(c->counts);
@@ -1569,6 +1613,7 @@ sv_status sv_set_config(sv_ctx *c, const sv_config *k) {
   if (k->binned_build > 1) return fail(c, SV_E_ARG, "binned_build must be 0 or 1");
   static_assert(SV_GEOM_MEAN_Y_FORWARD == GEOM_MEAN_Y_FORWARD, "geometry flag values");
   if (k->split_backward > 1) return fail(c, SV_E_ARG, "split_backward must be 0 or 1");
+  if (k->tile_order > 1) return fail(c, SV_E_ARG, "tile_order must be 0 or 1");
   if (k->geom_flags & ~SV_GEOM_MEAN_Y_FORWARD)
     return fail(c, SV_E_ARG, "geom_flags must be a combination of SV_GEOM_*");
   c->cfg = *k;

@@ -170,6 +170,14 @@ struct RasterArgs {
   // 32-px raster tiles: blocks of 2 x 2 tiles, global block -> slot
   const uint32_t *block_view;
   uint32_t n_blocks_total;
+  // 16-px raster tiles: work item i of the launch renders tile tile_begin +
+  // tile_order[i] (a permutation of 0 .. n - 1, longest list first; null:
+  // tile tile_begin + i)
+  const uint32_t *tile_order;
+  // forward output (may be null): per tile, the entries of its list some
+  // pixel visits (the backward's work), written by a wave that renders the
+  // whole tile
+  uint32_t *tile_cost;
 };
 constexpr uint32_t NUM_WORK_QUEUES = 8;     // one per XCD
 constexpr uint32_t WORK_QUEUE_STRIDE = 32;  // words between counters (a 128-byte line each)
@@ -181,6 +189,14 @@ uint32_t raster_resident_blocks(bool backward, uint32_t batch);
 // block of tiles. The lists are the 16-px tiles' in every case (same images).
 void launch_forward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks, uint32_t raster_px);
 void launch_backward(hipStream_t st, const RasterArgs &a, uint32_t batch, uint32_t grid_blocks);
+// The longest-first work order of a 16-px raster launch over tiles
+// [tile_begin, tile_begin + n), within the items each XCD takes: contiguous
+// eighths of the launch (xcd_remap, the persistent queues) or, with
+// `contiguous` false, the items i = x mod 8 of XCD x. A tile's work is
+// cost[t] when cost is given (the forward's visited entries), else its list
+// length.
+void launch_tile_order(hipStream_t st, const uint32_t *ranges, const uint32_t *cost, const StepMeta *meta,
+                       uint32_t tile_begin, uint32_t n, bool contiguous, uint32_t *order);
 // backward_tiles_local (backward-tiles-local.wgsl:185-382) for one slot:
 // i32 sums per (intersection, slot of 54) in sorted order; inv[emit
 // position] = sorted position of every intersection.

@@ -45,6 +45,12 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 // n_tiles_total).
 static_assert(NUM_WORK_QUEUES == NUM_XCD, "one work queue per XCD");
 
+// The tile of work item i of a launch: its list order, or with a tile
+// order the i-th entry of it (a permutation of the launch's items).
+__device__ __forceinline__ uint32_t item_tile(const RasterArgs &a, uint32_t i) {
+  return a.tile_begin + (a.tile_order ? a.tile_order[i] : i);
+}
+
 // Called by the whole wave (the raster workgroups are one wave): lane 0 does
 // the atomics, the item is read from it, every value stays scalar.
 __device__ __forceinline__ uint32_t pull_item(const RasterArgs &a) {
@@ -57,15 +63,89 @@ __device__ __forceinline__ uint32_t pull_item(const RasterArgs &a) {
     uint32_t t = 0;
     if (threadIdx.x == 0) t = atomicAdd(a.work_counter + x * WORK_QUEUE_STRIDE, 1u);
     t = __builtin_amdgcn_readfirstlane(t);
-    if (t < len) return a.tile_begin + x * q + min(x, r) + t;
+    if (t < len) return item_tile(a, x * q + min(x, r) + t);
   }
   return a.n_tiles_total;
 }
 
 __device__ __forceinline__ uint32_t first_tile(const RasterArgs &a) {
   if (a.work_counter == nullptr)
-    return a.tile_begin + (a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
+    return item_tile(a, a.cfg.xcd_remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x);
   return pull_item(a);
+}
+
+// ---------------------------------------------------------------------
+// Longest-first work order. A tile's raster time grows with its list, and
+// the lists are very uneven (most tiles of a view hold no splat, a few
+// central ones thousands), so taking the work items in tile order leaves
+// the longest tiles to start late and run alone at the end of the launch.
+// The order lists the items each XCD takes, longest first, in the
+// places that XCD takes them: the eight contiguous item ranges when
+// workgroups of one XCD take one range (xcd_remap, the persistent queues),
+// else the eight classes of items i = x mod 8 (workgroups reach the XCDs
+// round-robin in item order). A tile's work is its list length for the
+// forward and, for the backward, the entries some pixel visited in the
+// forward. One workgroup per XCD buckets its items by work (four buckets
+// per octave) with LDS counters and writes them
+// bucket by bucket, longest first; the order inside a bucket is whatever the
+// atomics give, which changes no result (every tile is computed on its own).
+// ---------------------------------------------------------------------
+constexpr int ORDER_BUCKETS = 124;
+constexpr int ORDER_THREADS = 1024;
+
+__device__ __forceinline__ uint32_t order_bucket(uint32_t len) {
+  if (len < 4u) return len;
+  const uint32_t lg = 31u - (uint32_t)__clz(len);  // >= 2
+  return 4u + 4u * (lg - 2u) + ((len >> (lg - 2u)) & 3u);
+}
+
+// A tile's work: its forward's visited entries when given, else its list.
+__device__ __forceinline__ uint32_t tile_work(const uint32_t *ranges, const uint32_t *cost, uint32_t t) {
+  return cost ? cost[t] : ranges[t + 1] - ranges[t];
+}
+
+__global__ __launch_bounds__(ORDER_THREADS) void k_tile_order(const uint32_t *__restrict__ ranges,
+                                                             const uint32_t *__restrict__ cost,
+                                                             const StepMeta *__restrict__ meta, uint32_t tile_begin,
+                                                             uint32_t n, uint32_t contiguous,
+                                                             uint32_t *__restrict__ order) {
+  if (meta->overflow) return;
+  __shared__ uint32_t cnt[ORDER_BUCKETS];
+  // this XCD's items: the j-th is base + j * step, j < m; its k-th place in
+  // the order is base + k * step
+  const uint32_t x = blockIdx.x, q = n / NUM_XCD, r = n % NUM_XCD;
+  const uint32_t base = contiguous ? x * q + min(x, r) : x;
+  const uint32_t step = contiguous ? 1u : NUM_XCD;
+  const uint32_t m = contiguous ? q + (x < r ? 1u : 0u) : (x < n ? (n - x + NUM_XCD - 1u) / NUM_XCD : 0u);
+  for (uint32_t b = threadIdx.x; b < (uint32_t)ORDER_BUCKETS; b += ORDER_THREADS) cnt[b] = 0u;
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < m; j += ORDER_THREADS) {
+    const uint32_t t = tile_begin + base + j * step;
+    atomicAdd(&cnt[order_bucket(tile_work(ranges, cost, t))], 1u);
+  }
+  __syncthreads();
+  // first place of each bucket, the longest bucket first
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int b = ORDER_BUCKETS - 1; b >= 0; --b) {
+      const uint32_t c = cnt[b];
+      cnt[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < m; j += ORDER_THREADS) {
+    const uint32_t i = base + j * step, t = tile_begin + i;
+    const uint32_t k = atomicAdd(&cnt[order_bucket(tile_work(ranges, cost, t))], 1u);
+    order[base + k * step] = i;
+  }
+}
+
+void launch_tile_order(hipStream_t st, const uint32_t *ranges, const uint32_t *cost, const StepMeta *meta,
+                       uint32_t tile_begin, uint32_t n, bool contiguous, uint32_t *order) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_tile_order, dim3(NUM_XCD), dim3(ORDER_THREADS), 0, st, ranges, cost, meta, tile_begin, n,
+                     contiguous ? 1u : 0u, order);
 }
 
 // The next work item of a persistent launch (false: none left, or one
@@ -436,7 +516,11 @@ __device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, u
   const size_t P = (size_t)V.W * V.H;
   float *col = a.color + 3 * V.pix_off;
   const uint32_t start = a.ranges[tg], end = a.ranges[tg + 1];
+  // per tile, for the backward's work order: how many entries of its list
+  // some pixel visits (a wave that renders the whole tile writes it)
+  uint32_t *const cost = wq == 0xFu ? a.tile_cost : nullptr;
   if (start >= end) {
+    if (cost && (threadIdx.x & 63u) == 0u) cost[tg] = 0u;
 #pragma unroll
     for (int j = 0; j < PPL; ++j) {
       const uint32_t px = pix_x(t, j), py = pix_y(t, j);
@@ -551,6 +635,14 @@ __device__ __forceinline__ void forward_tile(const RasterArgs &a, uint32_t tg, u
       a.alpha[V.pix_off + lp] = A[j];
       a.last[V.pix_off + lp] = last[j];
     }
+  }
+  if (cost) {
+    uint32_t m = start;
+#pragma unroll
+    for (int j = 0; j < PPL; ++j)
+      if (pix_x(t, j) < V.W && pix_y(t, j) < V.H) m = max(m, last[j] + 1u);
+    m = wave_max_u32(m);
+    if ((threadIdx.x & 63u) == 0u) cost[tg] = m - start;
   }
 }
 

@@ -136,7 +136,7 @@ const Field kFields[] = {FU(tile_size), FU(batch), FU(raster_px), FU(sh_deg), FF
                          FF(lr_opacity), FF(lr_scale), FF(lr_rot), FF(beta1), FF(beta2), FF(eps),
                          FU(sh_update_interval), FU(pos_lr_schedule), FU(render_f16), FF(importance_decay),
                          FU(persistent), FU(use_graph), FU(keep_grads), FU(plain_wf_flags), FU(xcd_remap),
-                         FU(binned_build), FU(geom_flags), FU(split_backward)};
+                         FU(binned_build), FU(geom_flags), FU(split_backward), FU(tile_order)};
 
 napi_value config_to_js(napi_env env, const sv_config &k) {
   napi_value o;

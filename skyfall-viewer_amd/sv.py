@@ -39,7 +39,7 @@ class Config(C.Structure):
         ("render_f16", C.c_uint32), ("importance_decay", C.c_float),
         ("persistent", C.c_uint32), ("use_graph", C.c_uint32), ("keep_grads", C.c_uint32),
         ("plain_wf_flags", C.c_uint32), ("xcd_remap", C.c_uint32), ("binned_build", C.c_uint32),
-        ("geom_flags", C.c_uint32), ("split_backward", C.c_uint32),
+        ("geom_flags", C.c_uint32), ("split_backward", C.c_uint32), ("tile_order", C.c_uint32),
     ]
 
 
