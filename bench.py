@@ -138,8 +138,8 @@ def run_mode(mode, g, cams, targets, args, graph=0):
     if mode == sv.MODE_BATCHED and args.autotune:
         tune = ctx.autotune(views, 3)
         best = min(tune, key=lambda e: e["ms_per_step"])
-        log("autotune: " + ", ".join(f"t{e['tile']}/b{e['batch']}/p{e['persistent']}/x{e['xcd_remap']} "
-                                      f"{e['ms_per_step']:.2f}" for e in tune) + f" -> {best}")
+        log("autotune: " + ", ".join(f"t{e['tile']}/b{e['batch']}/p{e['persistent']}/x{e['xcd_remap']}"
+                                      f"/s{e['split_backward']} {e['ms_per_step']:.2f}" for e in tune) + f" -> {best}")
     for i in range(args.warmup):
         ctx.step_views(views, mode)
     ctx.sync()
@@ -169,7 +169,8 @@ def run_mode(mode, g, cams, targets, args, graph=0):
         eager = (time.perf_counter() - e0) / args.steps
     st = ctx.step_views(views, mode, stats=True)
     cfg = ctx.config()
-    chosen = {"batch": cfg.batch, "persistent": cfg.persistent, "xcd_remap": cfg.xcd_remap, "raster_px": cfg.raster_px}
+    chosen = {"batch": cfg.batch, "persistent": cfg.persistent, "xcd_remap": cfg.xcd_remap, "raster_px": cfg.raster_px,
+              "split_backward": cfg.split_backward, "tile_order": cfg.tile_order}
     ctx.close()
     return (t1 - t0) / args.steps, timings, phase_steps, st, eager, tune, chosen
 
@@ -695,6 +696,7 @@ def main():
         "config": {"workload": workload, "splats": N, "views": V, "width": W, "height": H, "pixels": P,
                    "tile": 16, "raster_tile": chosen_b["raster_px"], "lds_batch": chosen_b["batch"],
                    "persistent": chosen_b["persistent"], "graph": args.graph, "xcd_remap": chosen_b["xcd_remap"],
+                   "split_backward": chosen_b["split_backward"], "tile_order": chosen_b["tile_order"],
                    "binned_build": args.binned},
         "autotune": tune_b,
         "eager_launch_ms_per_step": 1e3 * eager_b if eager_b else None,

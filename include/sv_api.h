@@ -95,7 +95,8 @@ typedef struct sv_config {
                                its backward as two launches over halves of
                                the views, the first half's record gather
                                overlapping the second half's backward on a
-                               second stream (same results) */
+                               second stream (same results; default 0, the
+                               autotuner tries both) */
   uint32_t tile_order;      /* 1 (default): the 16-px forward and backward
                                take each XCD's tiles longest list first (a
                                per-step bucket sort of the tiles by list
@@ -150,6 +151,7 @@ typedef struct sv_tune_entry {
   uint32_t persistent;  /* persistent raster launch */
   uint32_t xcd_remap;   /* XCD-contiguous tile ranges */
   uint32_t tile_size;   /* raster tile of the forward: 8, 16 or 32 */
+  uint32_t split_backward; /* the split backward of the batched step */
   double ms_per_step;   /* mean device time of one batched step */
 } sv_tune_entry;
 
@@ -442,10 +444,11 @@ sv_status sv_load_checkpoint(sv_ctx *ctx, const char *path);
  * constants: 16x16 tiles, 64-entry batches, backward-tiles-local.wgsl:49-53,
  * 185). Times `steps` batched steps of `views` (after one untimed step) for
  * every candidate raster tile 8/16/32 x LDS batch 64/128/256 x XCD tile
- * mapping 0/1 (and persistent launch 0/1 with 16-px raster tiles),
- * restoring parameters, moments, statistics and the step counter before
- * each candidate and at the end, and leaves the fastest candidate in the
- * configuration. The binning tile stays 16 (gaussian-renderer.ts:366): it
+ * mapping 0/1 (and persistent launch 0/1 with 16-px raster tiles), then,
+ * with two views or more, the fastest of those with split_backward
+ * switched, restoring parameters, moments, statistics and the step counter
+ * before each candidate and at the end, and leaves the fastest candidate in
+ * the configuration. The binning tile stays 16 (gaussian-renderer.ts:366): it
  * decides which fringe pixels (power between the 3-sigma radius and
  * ln(255 a)) a splat is composited into; the raster tile only regroups the
  * same per-pixel work, so every candidate renders the same image.

@@ -1435,7 +1435,7 @@ sv_status sv_default_config(sv_config *o) {
   o->xcd_remap = 1;
   o->binned_build = 1;
   o->geom_flags = 0;
-  o->split_backward = 1;
+  o->split_backward = 0;
   o->tile_order = 1;
   return SV_OK;
 }
@@ -2736,10 +2736,16 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
   // raster tile 16 x batch x persistent x XCD mapping; the 8- and 32-px
   // raster tiles (forward only) launch one workgroup per quadrant / block
   const uint32_t rasters[3] = {8, 16, 32};
+  const uint32_t split0 = c->cfg.split_backward;
   for (uint32_t r : rasters)
     for (uint32_t b : batches)
       for (uint32_t p = 0; p < (r == 16 ? 2u : 1u); ++p)
-        for (uint32_t x = 0; x < 2; ++x) cand.push_back(sv_tune_entry{b, p, x, r, 0.0});
+        for (uint32_t x = 0; x < 2; ++x) cand.push_back(sv_tune_entry{b, p, x, r, split0, 0.0});
+  // then the fastest of those with the split backward switched (it decides
+  // only with two views or more): one candidate filled in after the sweep
+  const bool try_split = nv >= 2;
+  const size_t n_sweep = cand.size();
+  if (try_split) cand.push_back(sv_tune_entry{0, 0, 0, 0, split0 ? 0u : 1u, 0.0});
   if (!out) {
     *count = (int)cand.size();
     return SV_OK;
@@ -2773,11 +2779,22 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
     st = fail(c, SV_E_HIP, "hipEventCreate failed");
   if (!st) st = copy_all(true);
   for (size_t i = 0; i < cand.size() && !st; ++i) {
+    if (i == n_sweep) {
+      // the switched split backward on the sweep's fastest candidate
+      size_t best = 0;
+      for (size_t j = 1; j < n_sweep; ++j)
+        if (cand[j].ms_per_step < cand[best].ms_per_step) best = j;
+      const uint32_t sp = cand[i].split_backward;
+      cand[i] = cand[best];
+      cand[i].split_backward = sp;
+      cand[i].ms_per_step = 0.0;
+    }
     c->cfg = saved_cfg;
     c->cfg.batch = cand[i].batch;
     c->cfg.persistent = cand[i].persistent;
     c->cfg.xcd_remap = cand[i].xcd_remap;
     c->cfg.raster_px = cand[i].tile_size;
+    c->cfg.split_backward = cand[i].split_backward;
     c->step_t = t0;
     if ((st = step_batched(c, vs, nv))) break;  // untimed: compiles, grows buffers, captures
     if ((st = copy_all(false))) break;
@@ -2810,6 +2827,7 @@ sv_status sv_autotune(sv_ctx *c, const int *vs, int nv, int steps, sv_tune_entry
   c->cfg.persistent = cand[best].persistent;
   c->cfg.xcd_remap = cand[best].xcd_remap;
   c->cfg.raster_px = cand[best].tile_size;
+  c->cfg.split_backward = cand[best].split_backward;
   const int n = std::min(*count, (int)cand.size());
   for (int i = 0; i < n; ++i) out[i] = cand[i];
   *count = (int)cand.size();

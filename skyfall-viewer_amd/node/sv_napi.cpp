@@ -922,9 +922,9 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   for (int i = 0; i < n; ++i) {
     napi_value o, v;
     CK(napi_create_object(env, &o));
-    const uint32_t f[4] = {t[i].batch, t[i].persistent, t[i].xcd_remap, t[i].tile_size};
-    const char *names[4] = {"batch", "persistent", "xcd_remap", "tile"};
-    for (int k = 0; k < 4; ++k) {
+    const uint32_t f[5] = {t[i].batch, t[i].persistent, t[i].xcd_remap, t[i].tile_size, t[i].split_backward};
+    const char *names[5] = {"batch", "persistent", "xcd_remap", "tile", "splitBackward"};
+    for (int k = 0; k < 5; ++k) {
       CK(napi_create_uint32(env, f[k], &v));
       CK(napi_set_named_property(env, o, names[k], v));
     }

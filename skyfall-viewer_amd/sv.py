@@ -67,7 +67,7 @@ class StepStats(C.Structure):
 
 class TuneEntry(C.Structure):
     _fields_ = [("batch", C.c_uint32), ("persistent", C.c_uint32), ("xcd_remap", C.c_uint32),
-                ("tile_size", C.c_uint32), ("ms_per_step", C.c_double)]
+                ("tile_size", C.c_uint32), ("split_backward", C.c_uint32), ("ms_per_step", C.c_double)]
 
 
 class KernelTime(C.Structure):
@@ -563,7 +563,7 @@ class Context:
         arr = (TuneEntry * n.value)()
         self._ck(lib().sv_autotune(self.h, vs, len(views), steps, arr, C.byref(n)), "autotune")
         return [{"batch": e.batch, "persistent": e.persistent, "xcd_remap": e.xcd_remap, "tile": e.tile_size,
-                 "ms_per_step": e.ms_per_step} for e in arr]
+                 "split_backward": e.split_backward, "ms_per_step": e.ms_per_step} for e in arr]
 
     def reset_opacity(self, max_opacity: float = 0.1) -> int:
         n = C.c_size_t()

@@ -34,13 +34,21 @@ def test_autotune_restores_state_and_keeps_the_fastest(sv, orc):
     table = c.autotune([0, 1, 2, 3], 2)
     print("[tune] " + ", ".join(f"t{e['tile']}/b{e['batch']}/p{e['persistent']}/x{e['xcd_remap']} "
                                 f"{e['ms_per_step']:.3f} ms" for e in table), flush=True)
-    # raster tile 16: batch x persistent x XCD mapping (12); 8 and 32: batch x XCD mapping (6 each)
-    assert len(table) == 24 and all(e["ms_per_step"] > 0 for e in table)
+    # raster tile 16: batch x persistent x XCD mapping (12); 8 and 32: batch x XCD mapping (6 each); then
+    # the sweep's fastest with the split backward switched (four views)
+    assert len(table) == 25 and all(e["ms_per_step"] > 0 for e in table)
     assert sorted({e["tile"] for e in table}) == [8, 16, 32]
+    split0 = table[0]["split_backward"]
+    assert all(e["split_backward"] == split0 for e in table[:24])
+    sweep_best = min(table[:24], key=lambda e: e["ms_per_step"])
+    last = table[24]
+    assert last["split_backward"] == 1 - split0
+    assert (last["tile"], last["batch"], last["persistent"], last["xcd_remap"]) == (
+        sweep_best["tile"], sweep_best["batch"], sweep_best["persistent"], sweep_best["xcd_remap"])
     best = min(table, key=lambda e: e["ms_per_step"])
     cfg = c.config()
-    assert (cfg.raster_px, cfg.batch, cfg.persistent, cfg.xcd_remap) == (best["tile"], best["batch"],
-                                                                        best["persistent"], best["xcd_remap"])
+    assert (cfg.raster_px, cfg.batch, cfg.persistent, cfg.xcd_remap, cfg.split_backward) == (
+        best["tile"], best["batch"], best["persistent"], best["xcd_remap"], best["split_backward"])
     after, m1b = c.get_gaussians(), c.get_moments(1)
     for k in KEYS:
         assert np.array_equal(before[k], after[k]) and np.array_equal(m1[k], m1b[k]), k
@@ -48,7 +56,7 @@ def test_autotune_restores_state_and_keeps_the_fastest(sv, orc):
     tuned = c.get_gaussians()
     c.close()
     f = _ctx(sv, g, cams, targets, use_graph=1, batch=best["batch"], persistent=best["persistent"],
-             xcd_remap=best["xcd_remap"], raster_px=best["tile"])
+             xcd_remap=best["xcd_remap"], raster_px=best["tile"], split_backward=best["split_backward"])
     f.step_views([0, 1, 2, 3], sv.MODE_BATCHED)
     f.step_views([0, 1, 2, 3], sv.MODE_BATCHED)
     fresh = f.get_gaussians()
