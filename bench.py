@@ -79,6 +79,8 @@ def kernel_bytes(phase: str, N: int, V: int, I: int, P: int, key_bytes: int, pas
         return I * 76.0 + P * 20.0
     if phase == "vjp_adam":  # per Gaussian: 59 x (p, m, v) read + write; per (G, view) the 36 B 2-D grad record
         return N * 1416.0 + N * V * 36.0
+    if phase == "project":  # per Gaussian: 59 parameters read; per (G, view): 64 B record, 8 B aux, 24 B counts
+        return N * 236.0 + N * V * 96.0
     if phase == "sort":
         return passes * I * (key_bytes + 2 * (key_bytes + 4.0))
     return 0.0
@@ -314,6 +316,26 @@ def run_growth4m(args):
     c.sync()
     dt = time.perf_counter() - t0
     P = V * W * H
+    # per kernel at the final size (after the timed run): three more steps
+    # with the phase timers (eager launches), their time and algorithmic HBM
+    # rate from the byte model, and the tile-list key rate
+    c.reset_timings()
+    c.enable_timing(True)
+    for _ in range(3):
+        st_k = c.step_views(views, sv.MODE_BATCHED, stats=True)
+    tim = c.timings()
+    c.enable_timing(False)
+    I_k, N_k = int(st_k.n_isect), c.count
+    kb = 8 if st_k.key_bits > 32 else 4
+    per_kernel = {}
+    for k, (ms, per_step) in phase_table(tim, 3).items():
+        b = kernel_bytes(k, N_k, V, I_k, P, kb, (int(st_k.key_bits) + 7) // 8) if k != "sort" else 0.0
+        per_kernel[k] = {"ms_per_launch": ms, "launches_per_step": per_step,
+                         "algorithmic_GBps": b / (ms * 1e-3) / 1e9 if b and ms > 0 else None}
+    list_ms, list_kps = list_build(tim, 3, I_k)
+    log("growth4m final size: " + ", ".join(f"{k} {v['ms_per_launch']:.3f} ms" +
+                                             (f" {v['algorithmic_GBps']:.0f} GB/s" if v["algorithmic_GBps"] else "")
+                                             for k, v in per_kernel.items()))
     out = {
         "metric": "train steps/s, growth run (1M to 4M splats, 16 views at 1080p)", "value": args.steps / dt, "unit": "steps/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt / args.steps,
@@ -333,6 +355,10 @@ def run_growth4m(args):
                    "unit: lr 1e-3 every group, SH every step; loss 0.8 L2 + 0.2 L1"},
         "intersections_per_s": isects / dt, "final_mean_l1": float(st.l1_sum) / (3.0 * P) if st else None,
         "events": events,
+        "final_size": {"splats": N_k, "intersections": I_k, "keys_per_s": list_kps, "tile_list_ms": list_ms,
+                       "per_kernel": per_kernel,
+                       "note": "three steps after the timed run with the phase timers (eager launches); "
+                               "algorithmic_GBps = the byte model's bytes of the launch / its time"},
     }
     c.close()
     print(json.dumps(out), flush=True)

@@ -590,28 +590,50 @@ __device__ __forceinline__ float signf_ref(float x) {
   return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
 }
 
+// One workgroup per 16x16 pixel tile. With the SSIM term, the tile's
+// 20x20 patch of both images (every pixel the tile's 5x5 windows reach,
+// coordinates clamped at the image border as the shader clamps them) is
+// staged in LDS once; each pixel then sums its window from there in the
+// shader's order (rows, then columns, channels apart), so every sum is the
+// same float as from the image itself.
+constexpr int LOSS_T = 16;               // output tile side
+constexpr int LOSS_R = 2;                // window radius (5 x 5)
+constexpr int LOSS_P = LOSS_T + 2 * LOSS_R;
+
 __global__ __launch_bounds__(256) void k_loss(uint32_t W, uint32_t H, DevConfig cfg,
                                               const float *__restrict__ color,
                                               const float *__restrict__ target,
                                               float *__restrict__ residual) {
-  const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const uint32_t y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= W || y >= H) return;
+  const uint32_t tx = threadIdx.x & 15u, ty = threadIdx.x >> 4;
+  const uint32_t x = blockIdx.x * LOSS_T + tx;
+  const uint32_t y = blockIdx.y * LOSS_T + ty;
+  const bool in = x < W && y < H;
   const size_t P = (size_t)W * H;
   const size_t pix = (size_t)y * W + x;
   float gs[3] = {0.f, 0.f, 0.f};
-  if (cfg.w_ssim > 0.0f) {
+  if (cfg.w_ssim > 0.0f) {  // the same for the whole workgroup
+    __shared__ float sx[3][LOSS_P][LOSS_P + 1], sy[3][LOSS_P][LOSS_P + 1];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(LOSS_P * LOSS_P); i += 256) {
+      const int pr = (int)(i / LOSS_P), pc = (int)(i % LOSS_P);
+      const int gy = min(max((int)(blockIdx.y * LOSS_T) - LOSS_R + pr, 0), (int)H - 1);
+      const int gx = min(max((int)(blockIdx.x * LOSS_T) - LOSS_R + pc, 0), (int)W - 1);
+      const size_t q = (size_t)gy * W + gx;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sx[c][pr][pc] = color[c * P + q];
+        sy[c][pr][pc] = target[c * P + q];
+      }
+    }
+    __syncthreads();
+    // pixel (x + dx, y + dy) of the window is patch entry (ty + 2 + dy, tx + 2 + dx)
     float mux[3] = {0, 0, 0}, muy[3] = {0, 0, 0};
     const float n = 25.0f;
-    for (int dy = -2; dy <= 2; ++dy)
-      for (int dx = -2; dx <= 2; ++dx) {
-        const int xp = min(max((int)x + dx, 0), (int)W - 1);
-        const int yp = min(max((int)y + dy, 0), (int)H - 1);
-        const size_t q = (size_t)yp * W + xp;
+    for (int r = 0; r < 2 * LOSS_R + 1; ++r)
+      for (int k = 0; k < 2 * LOSS_R + 1; ++k) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          mux[c] = mux[c] + color[c * P + q];
-          muy[c] = muy[c] + target[c * P + q];
+          mux[c] = mux[c] + sx[c][ty + r][tx + k];
+          muy[c] = muy[c] + sy[c][ty + r][tx + k];
         }
       }
 #pragma unroll
@@ -620,15 +642,12 @@ __global__ __launch_bounds__(256) void k_loss(uint32_t W, uint32_t H, DevConfig 
       muy[c] = muy[c] / n;
     }
     float sx2[3] = {0, 0, 0}, sy2[3] = {0, 0, 0}, sxy[3] = {0, 0, 0};
-    for (int dy = -2; dy <= 2; ++dy)
-      for (int dx = -2; dx <= 2; ++dx) {
-        const int xp = min(max((int)x + dx, 0), (int)W - 1);
-        const int yp = min(max((int)y + dy, 0), (int)H - 1);
-        const size_t q = (size_t)yp * W + xp;
+    for (int r = 0; r < 2 * LOSS_R + 1; ++r)
+      for (int k = 0; k < 2 * LOSS_R + 1; ++k) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          const float a = color[c * P + q] - mux[c];
-          const float b = target[c * P + q] - muy[c];
+          const float a = sx[c][ty + r][tx + k] - mux[c];
+          const float b = sy[c][ty + r][tx + k] - muy[c];
           sx2[c] = sx2[c] + a * a;
           sy2[c] = sy2[c] + b * b;
           sxy[c] = sxy[c] + a * b;
@@ -646,9 +665,10 @@ __global__ __launch_bounds__(256) void k_loss(uint32_t W, uint32_t H, DevConfig 
       const float den2 = (sx2[c] + sy2[c]) + C2;
       const float ssim = (num1 * num2) / (den1 * den2);
       const float dssim = (1.0f - ssim) * 0.5f;
-      gs[c] = dssim * (color[c * P + pix] - target[c * P + pix]);
+      gs[c] = dssim * (sx[c][ty + LOSS_R][tx + LOSS_R] - sy[c][ty + LOSS_R][tx + LOSS_R]);
     }
   }
+  if (!in) return;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const float d = color[c * P + pix] - target[c * P + pix];
