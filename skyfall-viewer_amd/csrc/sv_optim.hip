@@ -252,6 +252,10 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
   const uint32_t cnt = a.counts[vg];
   if (cnt) {
+    // this (view, Gaussian)'s conic, for the moments at the end: loaded
+    // first, in flight during the walk
+    const float4 q0 = a.rec[vg * 4];
+    const float cc = a.rec[vg * 4 + 1].x;
     const uint2 rc = a.rects[vg];
     const uint32_t ra = rc.x, rb = rc.y;
     const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
@@ -309,8 +313,6 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     // m_xy, m_yy, m_x, m_y); with this (view, Gaussian)'s conic (a, b, c):
     // dL/dconic = -(m_xx / 2, m_xy, m_yy / 2), dL/dmean = (a m_x + b m_y,
     // b m_x + c m_y)
-    const float4 q0 = a.rec[vg * 4];
-    const float cc = a.rec[vg * 4 + 1].x;
     const float mx = r[7], my = r[8];
     r[4] = -0.5f * r[4];
     r[5] = -r[5];
