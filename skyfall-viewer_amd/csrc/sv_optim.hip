@@ -252,10 +252,6 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
   for (int q = 0; q < REC9; ++q) r[q] = 0.0f;
   const uint32_t cnt = a.counts[vg];
   if (cnt) {
-    // this (view, Gaussian)'s conic, for the moments at the end: loaded
-    // first, in flight during the walk
-    const float4 q0 = a.rec[vg * 4];
-    const float cc = a.rec[vg * 4 + 1].x;
     const uint2 rc = a.rects[vg];
     const uint32_t ra = rc.x, rb = rc.y;
     const uint32_t x0 = ra & 0xffffu, y0 = ra >> 16, x1 = rb & 0xffffu, y1 = rb >> 16;
@@ -312,13 +308,18 @@ __global__ __launch_bounds__(256) void k_gather_records(VjpArgs a) {
     // the records hold moments of -dL/dpower over pixel offsets (m_xx,
     // m_xy, m_yy, m_x, m_y); with this (view, Gaussian)'s conic (a, b, c):
     // dL/dconic = -(m_xx / 2, m_xy, m_yy / 2), dL/dmean = (a m_x + b m_y,
-    // b m_x + c m_y)
-    const float mx = r[7], my = r[8];
-    r[4] = -0.5f * r[4];
-    r[5] = -r[5];
-    r[6] = -0.5f * r[6];
-    r[7] = fmaf(q0.z, mx, q0.w * my);
-    r[8] = fmaf(q0.w, mx, cc * my);
+    // b m_x + c m_y). A pair no pixel reached (most of them: occluded
+    // splats) has no record and keeps its zeros, without loading the conic.
+    if (!first) {
+      const float4 q0 = a.rec[vg * 4];
+      const float cc = a.rec[vg * 4 + 1].x;
+      const float mx = r[7], my = r[8];
+      r[4] = -0.5f * r[4];
+      r[5] = -r[5];
+      r[6] = -0.5f * r[6];
+      r[7] = fmaf(q0.z, mx, q0.w * my);
+      r[8] = fmaf(q0.w, mx, cc * my);
+    }
   }
 #pragma unroll
   for (int q = 0; q < REC9; ++q) a.vrec[(size_t)q * M + vg] = r[q];
