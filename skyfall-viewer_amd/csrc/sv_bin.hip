@@ -178,17 +178,20 @@ __device__ __forceinline__ Chunk chunk_of(uint32_t c, const StepMeta *m, const u
   return ch;
 }
 
-// Wave w of a chunk workgroup takes pairs lo + 64 w .. lo + 64 w + 63. For
-// each of the 64 tiles of the supertile, the wave's ballot of the pairs
-// covering it; s_cnt[w][t] = its popcount. Returns the lane's pair id and
-// rectangle (x0, x1, y0, y1 in bits 0-2, 3-5, 6-8, 9-11).
-__device__ __forceinline__ bool chunk_pair(const Chunk &ch, const uint32_t *keys, const uint32_t *vals, uint32_t kb,
-                                           uint32_t *r, uint32_t *g) {
+// Thread i of a chunk workgroup takes pair lo + i (wave w pairs lo + 64 w
+// .. lo + 64 w + 63). Loads the lane's key and Gaussian id (the key's bits
+// from kb on hold the pair's rectangle: x0, x1, y0, y1 in bits 0-2, 3-5,
+// 6-8, 9-11) and returns whether the lane has a pair. The loads are
+// unconditional (lanes past the chunk's end read its last pair, then
+// unused), so no branch makes the wave wait for them before their use.
+__device__ __forceinline__ bool chunk_pair(const Chunk &ch, const uint32_t *keys, const uint32_t *vals, uint32_t *key,
+                                           uint32_t *g) {
   const uint32_t k = ch.lo + threadIdx.x;
-  if (k >= ch.hi) return false;
-  *r = keys[k] >> kb;
-  *g = vals[k];
-  return true;
+  if (ch.hi <= ch.lo) return false;  // uniform
+  const uint32_t kc = min(k, ch.hi - 1u);
+  *key = keys[kc];
+  *g = vals[kc];
+  return k < ch.hi;
 }
 // The 64 tiles of the supertile a pair's rectangle covers, as a bit mask
 // (bit lx + 8 ly).
@@ -251,10 +254,10 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_count(const uint32_t *__restric
   for (uint32_t c = blockIdx.x;; c += gridDim.x) {
     const Chunk ch = chunk_of(c, m, chunk_super, cbase, sranges);
     if (!ch.live) return;
-    uint32_t r = 0, g = 0;
-    const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, kb, &r, &g);
+    uint32_t key = 0, g = 0;
+    const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, &key, &g);
     // lane t: the wave's pairs covering tile t
-    const uint64_t col = wave_transpose64(ok ? cover_mask(r) : 0ull);
+    const uint64_t col = wave_transpose64(ok ? cover_mask(key >> kb) : 0ull);
     s_cnt[wave][lane] = (uint32_t)__popcll(col);
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -323,10 +326,25 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
   const Chunk ch = chunk_of(blockIdx.x, m, chunk_super, cbase, sranges);
   if (!ch.live) return;
   const uint32_t sel = m[1].sel, kb = m[1].key_bits;
-  uint32_t r = 0, g = 0;
-  const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, kb, &r, &g);
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const uint64_t mine = ok ? cover_mask(r) : 0ull;
+  uint32_t key = 0, g = 0;
+  const bool ok = chunk_pair(ch, sel ? keys1 : keys0, sel ? vals1 : vals0, &key, &g);
+  // while the pairs load, the first wave looks up its tile's list range and
+  // the chunk's offset in it (a chain of dependent loads: view, ranges),
+  // summed only after the transpose so that nothing waits for them here
+  uint32_t rb = 0, co = 0, e = 0;
+  if (tid < 64) {
+    const DevView &V = views[super_view[ch.s]];
+    const uint32_t ls = ch.s - V.super_base;
+    const uint32_t tx = (ls % V.super_x) * BIN_S + (tid & 7u), ty = (ls / V.super_x) * BIN_S + (tid >> 3);
+    if (tx < V.tiles_x && ty < V.tiles_y) {
+      const uint32_t tg = V.tile_base + ty * V.tiles_x + tx;
+      rb = ranges[tg];
+      co = ccount[(size_t)blockIdx.x * 64 + tid];
+      e = ranges[tg + 1];
+    }
+  }
+  const uint64_t mine = ok ? cover_mask(key >> kb) : 0ull;
   const uint64_t col = wave_transpose64(mine);
   s_pre[wave][lane] = (uint32_t)__popcll(col);
   s_gid[wave][lane] = g;
@@ -340,16 +358,7 @@ __global__ __launch_bounds__(BIN_CH) void k_fill_write(const uint32_t *__restric
       run += c;
     }
     s_tot[tid] = run;
-    const DevView &V = views[super_view[ch.s]];
-    const uint32_t ls = ch.s - V.super_base;
-    const uint32_t tx = (ls % V.super_x) * BIN_S + (tid & 7u), ty = (ls / V.super_x) * BIN_S + (tid >> 3);
-    uint32_t b = 0, e = 0;
-    if (tx < V.tiles_x && ty < V.tiles_y) {
-      const uint32_t tg = V.tile_base + ty * V.tiles_x + tx;
-      b = ranges[tg] + ccount[(size_t)blockIdx.x * 64 + tid];
-      e = ranges[tg + 1];
-    }
-    s_base[tid] = b;
+    s_base[tid] = rb + co;
     s_end[tid] = e;
   }
   __syncthreads();
